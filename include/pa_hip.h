@@ -1,0 +1,171 @@
+/* pa_hip.h -- C ABI of the MI355X (gfx950) radix-sort library libpa_hip.so.
+ *
+ * Every entry point is extern "C" with plain pointers and sizes.  It is the
+ * boundary that the drop-in C++ headers in this directory (RadixSortLsdParallel.h,
+ * HistogramParallel.h, RadixSelect.h, ParallelMerge.h) forward to, and what an
+ * FFI (ctypes, cgo, JNI, N-API) would bind.
+ *
+ * Conventions
+ *  - Array pointers may be device memory (hipMalloc) or host memory.  With
+ *    device pointers a call only enqueues work on `stream` and returns; the
+ *    caller synchronises.  When any array argument is host memory the library
+ *    stages it through device memory and returns after the result is back in
+ *    host memory (the reference's blocking semantics).
+ *  - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *  - Calls are thread safe.  Calls on one stream are serialised by the
+ *    library; device scratch is held per stream, so calls on different streams
+ *    run concurrently without sharing scratch.
+ *  - Return value: PA_OK (0) or a pa_status code; nothing is retried or
+ *    silently redirected to the CPU.
+ *  - Sizes: the sort entry points accept n < 2^32 elements.
+ *
+ * Each declaration names the reference interface it replaces
+ * (DragonSpit/ParallelAlgorithms, file:line).
+ */
+#ifndef PA_HIP_H
+#define PA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* pa_stream_t;
+
+enum pa_status {
+  PA_OK = 0,
+  PA_ERR_ARG = 1,       /* bad argument (null pointer with n > 0, k >= n, misaligned key pointer, ...) */
+  PA_ERR_OOM = 2,       /* device or pinned host allocation failed */
+  PA_ERR_HIP = 3,       /* a HIP runtime call failed; see pa_last_error_string() */
+  PA_ERR_TOO_LARGE = 4, /* n beyond the documented limit of the entry point */
+  PA_ERR_NO_DEVICE = 5  /* no usable gfx950 device */
+};
+
+/* Library info and errors. */
+const char* pa_status_string(int status);
+const char* pa_last_error_string(void);
+int pa_version(void);
+/* Selects the device used by subsequent calls on this host thread (default 0). */
+int pa_set_device(int device);
+/* Blocks until all work enqueued on `stream` has finished. */
+int pa_synchronize(pa_stream_t stream);
+
+/* ---- LSD radix sort (ascending, stable) --------------------------------- */
+
+/* Replaces ParallelAlgorithms::SortRadixPar(unsigned* a, unsigned* tmp, size_t n,
+ * size_t thr) (RadixSortLsdParallel.h:367-387) and SortRadixPar(unsigned* a,
+ * size_t n, size_t thr) (:340-364).  Result in `keys`; `tmp` (n elements) is
+ * scratch whose contents are unspecified afterwards; tmp may be NULL, then the
+ * library uses its own device scratch. */
+int pa_sort_u32(uint32_t* keys, uint32_t* tmp, size_t n, pa_stream_t stream);
+
+/* Replaces RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase(unsigned* a,
+ * unsigned* b, unsigned long n) (RadixSortLsdParallel.h:86-106): sorted result
+ * in BOTH a and b (the reference copies it to b, :80-82).  Also serves
+ * ..._TwoPhase_DeRandomize (:491-511), which is a no-op stub for n >= 100 in the
+ * reference. */
+int pa_sort_two_phase_u32(uint32_t* a, uint32_t* b, size_t n, pa_stream_t stream);
+
+/* 64-bit keys, 8 passes.  No reference counterpart on the parallel path (the
+ * serial u64 LSD at RadixSortLSD.h:356-400 mis-sorts keys >= 2^32). */
+int pa_sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, pa_stream_t stream);
+
+/* float32 keys (IEEE bits) through the order-preserving transform
+ * f(u) = u ^ (sign ? 0xFFFFFFFF : 0x80000000), inverse applied on the last
+ * pass.  Order: -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN, NaN payloads
+ * kept.  No reference counterpart (RadixSortCommon.h:62-68 is the only order
+ * transform in the reference, for signed integers, and is unused). */
+int pa_sort_f32(float* keys, float* tmp, size_t n, pa_stream_t stream);
+
+/* Stable key-value sorts: keys ascending, vals permuted with their keys, ties
+ * keep input order.  No reference counterpart (new component). */
+int pa_sort_kv_u32_u32(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, size_t n, pa_stream_t stream);
+int pa_sort_kv_u64_u32(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, size_t n, pa_stream_t stream);
+
+/* ---- Histograms ----------------------------------------------------------- */
+
+/* All 4 byte-digit histograms of u32 keys in one read.  counts: 4 x 256
+ * uint64 (row d = byte d).  Replaces HistogramByteComponentsParallel<256,8>
+ * (HistogramParallel.h:249-329), whose 8-row return shape the drop-in header
+ * rebuilds. */
+int pa_hist_digits_u32(const uint32_t* keys, size_t n, uint64_t* counts, pa_stream_t stream);
+
+/* Histogram of (key >> shift) & (2^bits - 1), bits in 1..12; counts has 2^bits
+ * uint64.  Replaces HistogramOneByteComponentParallel<R,L>(unsigned[], l, r,
+ * shift, thr) (HistogramParallel.h:535-576) and the _2 variant (:466-533). */
+int pa_hist_u32(const uint32_t* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, pa_stream_t stream);
+
+/* Per-work-quantum histogram of byte `which_byte`: counts[q][b] with
+ * q = index / quantum, nq = ceil(n / quantum) rows of 256 uint64.  Replaces
+ * HistogramByteComponentsQCPar<256,8> (HistogramParallel.h:444-464, with
+ * QCParInner :399-442 and AcrossWorkQuantasQC :331-397). */
+int pa_hist_quanta_u32(const uint32_t* keys, size_t n, size_t quantum, unsigned which_byte, uint64_t* counts,
+                       pa_stream_t stream);
+
+/* start[q][b] = sum_{b'<b} size[b'] + sum_{q'<q} count[q'][b] for byte `digit`
+ * (nq x 256 uint64).  Replaces ComputeStartOfBinsPar<256,8>
+ * (RadixSortLsdParallel.h:108-165). */
+int pa_start_of_bins_u32(const uint32_t* keys, size_t n, size_t quantum, unsigned digit, uint64_t* start,
+                         pa_stream_t stream);
+
+/* 256-bin histogram of a byte array.  Replaces
+ * HistogramOneByteComponentParallel{,_2,_3,_4}<256>(unsigned char[], l, r, thr)
+ * (HistogramParallel.h:16-247). */
+int pa_hist_u8(const uint8_t* bytes, size_t n, uint64_t* counts, pa_stream_t stream);
+
+/* ---- Radix select ---------------------------------------------------------- */
+
+/* *result = k-th smallest (0-based) of keys[0..n).  keys is left unmodified.
+ * Replaces SelectRadix(unsigned[], size_t len, size_t k) and
+ * SelectRadix(unsigned[], size_t start, size_t len, size_t k)
+ * (RadixSelect.h:139-168).  result may be host or device memory. */
+int pa_select_u32(const uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream);
+
+/* ---- Merge ----------------------------------------------------------------- */
+
+/* out[0..na+nb) = stable merge of sorted a[0..na) and b[0..nb) (ties take a
+ * first).  Replaces merge_parallel_L5<T>(t, p1, r1, p2, r2, a, p3, thr)
+ * (ParallelMerge.h:285-315) and the rest of the ParallelMerge.h family, which
+ * the drop-in header maps onto it.  out must not overlap a or b. */
+int pa_merge_u32(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, uint32_t* out, pa_stream_t stream);
+int pa_merge_u64(const uint64_t* a, size_t na, const uint64_t* b, size_t nb, uint64_t* out, pa_stream_t stream);
+
+/* ---- Segmented (batched) sort ---------------------------------------------- */
+
+/* Sorts num_segments independent arrays in one batch.  Segment i is
+ * keys[offsets[i] .. offsets[i+1]); offsets (num_segments + 1 entries,
+ * non-decreasing) is host memory; keys/tmp are device memory.  Each segment's
+ * result equals pa_sort_u32 on that segment.  No reference counterpart (the
+ * reference only benchmarks bundling small tasks, BundleSmallWorkItems.cpp:36-144). */
+int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets, size_t num_segments,
+                          pa_stream_t stream);
+
+/* ---- Measurement helpers (no reference counterpart) ------------------------ */
+
+/* Plain device copy kernel, dst[i] = src[i]; the known-size calibration point
+ * for HBM byte counters and the achievable-bandwidth probe. */
+int pa_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, pa_stream_t stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around each
+ * kernel the library launches.  enable=1 starts recording, 0 stops. */
+int pa_profile_enable(int enable);
+/* Totals for kernels whose name starts with `prefix` since the last reset;
+ * waits for the recorded events. */
+int pa_profile_query(const char* prefix, double* total_ms, uint64_t* launches);
+int pa_profile_reset(void);
+
+/* ---- Deterministic host-side input generators ------------------------------ */
+
+/* std::mt19937_64(seed) outputs truncated to 32 bits (the reference's
+ * generator: ParallelAlgorithms.cpp:91,114-115).  Host memory. */
+void pa_gen_mt19937_u32(uint64_t seed, uint32_t* out, size_t n);
+/* Full 64-bit std::mt19937_64(seed) outputs. */
+void pa_gen_mt19937_u64(uint64_t seed, uint64_t* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PA_HIP_H */

@@ -1,0 +1,231 @@
+"""ctypes binding of libpa_hip.so, the gfx950 radix-sort library.
+
+The product is the C ABI declared in include/pa_hip.h (and the drop-in C++
+headers above it).  This module binds that ABI for Python callers: tests and
+bench.py pass device tensors (torch, ROCm) or host numpy arrays straight
+through.  Nothing here computes on the CPU: if the shared library is missing,
+every call raises.
+"""
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libpa_hip.so")
+HEADER = os.path.join(ROOT, "include", "pa_hip.h")
+
+PA_OK = 0
+STATUS = {0: "PA_OK", 1: "PA_ERR_ARG", 2: "PA_ERR_OOM", 3: "PA_ERR_HIP", 4: "PA_ERR_TOO_LARGE", 5: "PA_ERR_NO_DEVICE"}
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_u = ctypes.c_uint
+_i = ctypes.c_int
+_u64 = ctypes.c_uint64
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pa_status_string": (ctypes.c_char_p, [_i]),
+    "pa_last_error_string": (ctypes.c_char_p, []),
+    "pa_version": (_i, []),
+    "pa_set_device": (_i, [_i]),
+    "pa_synchronize": (_i, [_vp]),
+    "pa_sort_u32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_sort_two_phase_u32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_sort_u64": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_sort_f32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_sort_kv_u32_u32": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
+    "pa_sort_kv_u64_u32": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
+    "pa_hist_digits_u32": (_i, [_vp, _sz, _vp, _vp]),
+    "pa_hist_u32": (_i, [_vp, _sz, _u, _u, _vp, _vp]),
+    "pa_hist_quanta_u32": (_i, [_vp, _sz, _sz, _u, _vp, _vp]),
+    "pa_start_of_bins_u32": (_i, [_vp, _sz, _sz, _u, _vp, _vp]),
+    "pa_hist_u8": (_i, [_vp, _sz, _vp, _vp]),
+    "pa_select_u32": (_i, [_vp, _sz, _sz, _vp, _vp]),
+    "pa_merge_u32": (_i, [_vp, _sz, _vp, _sz, _vp, _vp]),
+    "pa_merge_u64": (_i, [_vp, _sz, _vp, _sz, _vp, _vp]),
+    "pa_segmented_sort_u32": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "pa_copy_u32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_profile_enable": (_i, [_i]),
+    "pa_profile_query": (_i, [ctypes.c_char_p, _vp, _vp]),
+    "pa_profile_reset": (_i, []),
+    "pa_gen_mt19937_u32": (None, [_u64, _vp, _sz]),
+    "pa_gen_mt19937_u64": (None, [_u64, _vp, _sz]),
+}
+
+_lib = None
+
+
+class PAError(RuntimeError):
+    def __init__(self, name, status, detail):
+        super().__init__(f"{name} -> {STATUS.get(status, status)}: {detail}")
+        self.status = status
+
+
+def header_symbols(path=HEADER):
+    """Names of the functions declared in include/pa_hip.h."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pa_[a-z0-9_]+)\s*\(", text)))
+
+
+def load():
+    """Loads libpa_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python __graft_entry__.py` "
+                           "or parallelalgorithms_amd/build.py")
+    try:  # share torch's HIP runtime when torch is present (same soname)
+        import torch  # noqa: F401
+    except Exception:
+        pass
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(x):
+    """Raw address of a torch tensor, numpy array, int or None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):
+        return x.ctypes.data
+    raise TypeError(f"cannot take the address of {type(x)}")
+
+
+def _stream(stream):
+    if stream is None:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                return torch.cuda.current_stream().cuda_stream
+        except Exception:
+            pass
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    return stream
+
+
+def call(name, *args):
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st != PA_OK:
+        raise PAError(name, st, lib.pa_last_error_string().decode())
+    return st
+
+
+def numel(x):
+    return x.numel() if hasattr(x, "numel") else x.size
+
+
+def sort_u32(keys, tmp=None, stream=None):
+    call("pa_sort_u32", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
+
+
+def sort_two_phase_u32(a, b, stream=None):
+    call("pa_sort_two_phase_u32", _ptr(a), _ptr(b), numel(a), _stream(stream))
+
+
+def sort_u64(keys, tmp=None, stream=None):
+    call("pa_sort_u64", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
+
+
+def sort_f32(keys, tmp=None, stream=None):
+    call("pa_sort_f32", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
+
+
+def sort_kv_u32_u32(keys, vals, ktmp=None, vtmp=None, stream=None):
+    call("pa_sort_kv_u32_u32", _ptr(keys), _ptr(vals), _ptr(ktmp), _ptr(vtmp), numel(keys), _stream(stream))
+
+
+def sort_kv_u64_u32(keys, vals, ktmp=None, vtmp=None, stream=None):
+    call("pa_sort_kv_u64_u32", _ptr(keys), _ptr(vals), _ptr(ktmp), _ptr(vtmp), numel(keys), _stream(stream))
+
+
+def hist_digits_u32(keys, counts, stream=None):
+    call("pa_hist_digits_u32", _ptr(keys), numel(keys), _ptr(counts), _stream(stream))
+
+
+def hist_u32(keys, shift, bits, counts, stream=None):
+    call("pa_hist_u32", _ptr(keys), numel(keys), shift, bits, _ptr(counts), _stream(stream))
+
+
+def hist_quanta_u32(keys, quantum, which_byte, counts, stream=None):
+    call("pa_hist_quanta_u32", _ptr(keys), numel(keys), quantum, which_byte, _ptr(counts), _stream(stream))
+
+
+def start_of_bins_u32(keys, quantum, digit, out, stream=None):
+    call("pa_start_of_bins_u32", _ptr(keys), numel(keys), quantum, digit, _ptr(out), _stream(stream))
+
+
+def hist_u8(data, counts, stream=None):
+    call("pa_hist_u8", _ptr(data), numel(data), _ptr(counts), _stream(stream))
+
+
+def select_u32(keys, k, result, stream=None):
+    call("pa_select_u32", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
+
+
+def merge_u32(a, b, out, stream=None):
+    call("pa_merge_u32", _ptr(a), numel(a), _ptr(b), numel(b), _ptr(out), _stream(stream))
+
+
+def merge_u64(a, b, out, stream=None):
+    call("pa_merge_u64", _ptr(a), numel(a), _ptr(b), numel(b), _ptr(out), _stream(stream))
+
+
+def segmented_sort_u32(keys, tmp, offsets, stream=None):
+    """offsets: host numpy uint64 array of num_segments + 1 entries."""
+    call("pa_segmented_sort_u32", _ptr(keys), _ptr(tmp), _ptr(offsets), len(offsets) - 1, _stream(stream))
+
+
+def copy_u32(dst, src, stream=None):
+    call("pa_copy_u32", _ptr(dst), _ptr(src), numel(src), _stream(stream))
+
+
+def synchronize(stream=None):
+    call("pa_synchronize", _stream(stream))
+
+
+def profile_enable(on=True):
+    call("pa_profile_enable", 1 if on else 0)
+
+
+def profile_reset():
+    call("pa_profile_reset")
+
+
+def profile_query(prefix):
+    """(total_ms, launches) for kernels whose name starts with prefix."""
+    tot = ctypes.c_double(0.0)
+    cnt = ctypes.c_uint64(0)
+    call("pa_profile_query", prefix.encode(), ctypes.addressof(tot), ctypes.addressof(cnt))
+    return tot.value, cnt.value
+
+
+def gen_u32(seed, n):
+    import numpy as np
+    out = np.empty(n, dtype=np.uint32)
+    load().pa_gen_mt19937_u32(seed, out.ctypes.data, n)
+    return out
+
+
+def gen_u64(seed, n):
+    import numpy as np
+    out = np.empty(n, dtype=np.uint64)
+    load().pa_gen_mt19937_u64(seed, out.ctypes.data, n)
+    return out

@@ -1,0 +1,129 @@
+// pa_internal.h -- shared internals of libpa_hip.so (host runtime + device helpers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "../../include/pa_hip.h"
+
+namespace pa {
+
+// ---------------------------------------------------------------------------
+// Error plumbing
+// ---------------------------------------------------------------------------
+void set_last_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define PA_HIP(call)                                                         \
+  do {                                                                       \
+    hipError_t pa_e_ = (call);                                               \
+    if (pa_e_ != hipSuccess) return ::pa::hip_fail(pa_e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define PA_TRY(call)            \
+  do {                          \
+    int pa_s_ = (call);         \
+    if (pa_s_ != PA_OK) return pa_s_; \
+  } while (0)
+
+int arg_error(const char* msg);
+
+// ---------------------------------------------------------------------------
+// Per-stream context: a mutex that serialises enqueues on one stream and the
+// device scratch owned by that stream.  Work on one stream runs in order, so
+// consecutive calls on the same stream can reuse the same scratch.
+// ---------------------------------------------------------------------------
+struct StreamCtx {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void* pinned = nullptr;  // pinned host staging for small results
+  size_t pinned_bytes = 0;
+};
+
+StreamCtx& stream_ctx(hipStream_t s);
+// Ensures ctx.scratch holds >= bytes; on growth waits for the stream first.
+int ensure_scratch(StreamCtx& ctx, size_t bytes, void** out);
+int ensure_pinned(StreamCtx& ctx, size_t bytes, void** out);
+
+// True when p is device-accessible memory (hipMalloc / managed / pinned).
+bool is_device_ptr(const void* p);
+
+// Bump layout inside a scratch block (256-byte aligned slices).
+struct Layout {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    size_t o = (off + 255) & ~size_t(255);
+    off = o + bytes;
+    return o;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Kernel timing hooks (pa_profile_*): records a pair of events around a launch.
+// ---------------------------------------------------------------------------
+struct ProfScope {
+  ProfScope(const char* name, hipStream_t s);
+  ~ProfScope();
+  const char* name;
+  hipStream_t stream;
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// Host-staging helper: when a caller passes host arrays, they are copied into
+// device scratch, the device routine runs, and results are copied back.
+// ---------------------------------------------------------------------------
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace pa
+
+// ---------------------------------------------------------------------------
+// Device helpers (wave64, gfx950)
+// ---------------------------------------------------------------------------
+namespace pa {
+namespace dev {
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// popcount(mask & lanes below this lane)
+__device__ __forceinline__ uint32_t count_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Order-preserving float transform and inverse on IEEE-754 bit patterns.
+__device__ __forceinline__ uint32_t f32_fwd(uint32_t u) { return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u); }
+__device__ __forceinline__ uint32_t f32_inv(uint32_t v) { return v ^ (((v >> 31) - 1u) | 0x80000000u); }
+
+template <typename K>
+__device__ __forceinline__ uint32_t digit8(K k, uint32_t shift) {
+  return (uint32_t)(k >> shift) & 0xFFu;
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= (uint32_t)off) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t atomic_load_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace dev
+}  // namespace pa

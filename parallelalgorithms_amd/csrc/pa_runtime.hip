@@ -1,0 +1,226 @@
+// pa_runtime.hip -- host runtime of libpa_hip.so: errors, per-stream scratch,
+// pointer classification, kernel timing hooks, copy kernel, generators.
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <random>
+#include <vector>
+
+#include "pa_internal.h"
+
+namespace pa {
+
+static thread_local std::string t_last_error;
+
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  set_last_error(buf);
+  (void)hipGetLastError();
+  return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PA_ERR_OOM : PA_ERR_HIP;
+}
+
+int arg_error(const char* msg) {
+  set_last_error(msg);
+  return PA_ERR_ARG;
+}
+
+// ---------------------------------------------------------------------------
+// Stream contexts
+// ---------------------------------------------------------------------------
+static std::mutex g_ctx_mu;
+static std::map<std::pair<int, hipStream_t>, StreamCtx*> g_ctx;
+
+StreamCtx& stream_ctx(hipStream_t s) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  auto key = std::make_pair(dev, s);
+  auto it = g_ctx.find(key);
+  if (it != g_ctx.end()) return *it->second;
+  StreamCtx* c = new StreamCtx();  // lives until process exit
+  c->stream = s;
+  c->device = dev;
+  g_ctx[key] = c;
+  return *c;
+}
+
+int ensure_scratch(StreamCtx& ctx, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 256;
+  if (ctx.scratch_bytes < bytes) {
+    if (ctx.scratch) {
+      PA_HIP(hipStreamSynchronize(ctx.stream));
+      PA_HIP(hipFree(ctx.scratch));
+      ctx.scratch = nullptr;
+      ctx.scratch_bytes = 0;
+    }
+    size_t want = align_up(bytes + bytes / 8, size_t(1) << 20);
+    PA_HIP(hipMalloc(&ctx.scratch, want));
+    ctx.scratch_bytes = want;
+  }
+  *out = ctx.scratch;
+  return PA_OK;
+}
+
+int ensure_pinned(StreamCtx& ctx, size_t bytes, void** out) {
+  if (ctx.pinned_bytes < bytes) {
+    if (ctx.pinned) {
+      PA_HIP(hipStreamSynchronize(ctx.stream));
+      PA_HIP(hipHostFree(ctx.pinned));
+      ctx.pinned = nullptr;
+      ctx.pinned_bytes = 0;
+    }
+    size_t want = align_up(bytes < 65536 ? 65536 : bytes, 4096);
+    PA_HIP(hipHostMalloc(&ctx.pinned, want, hipHostMallocDefault));
+    ctx.pinned_bytes = want;
+  }
+  *out = ctx.pinned;
+  return PA_OK;
+}
+
+bool is_device_ptr(const void* p) {
+  if (p == nullptr) return false;
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel timing
+// ---------------------------------------------------------------------------
+struct ProfRec {
+  std::string name;
+  hipEvent_t start, stop;
+};
+static std::atomic<bool> g_prof_on{false};
+static std::mutex g_prof_mu;
+static std::vector<ProfRec> g_prof;
+
+ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
+  if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
+    (void)hipGetLastError();
+    start = stop = nullptr;
+    return;
+  }
+  (void)hipEventRecord(start, stream);
+}
+
+ProfScope::~ProfScope() {
+  if (!start) return;
+  (void)hipEventRecord(stop, stream);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.push_back({name, start, stop});
+}
+
+// ---------------------------------------------------------------------------
+// Copy kernel (bandwidth calibration)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                                  size_t nvec, size_t n) {
+  const uint4* s4 = reinterpret_cast<const uint4*>(src);
+  uint4* d4 = reinterpret_cast<uint4*>(dst);
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) d4[i] = s4[i];
+  if (blockIdx.x == 0)
+    for (size_t i = nvec * 4 + threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace pa
+
+using namespace pa;
+
+extern "C" {
+
+const char* pa_status_string(int status) {
+  switch (status) {
+    case PA_OK: return "ok";
+    case PA_ERR_ARG: return "invalid argument";
+    case PA_ERR_OOM: return "out of memory";
+    case PA_ERR_HIP: return "HIP runtime error";
+    case PA_ERR_TOO_LARGE: return "input too large for this entry point";
+    case PA_ERR_NO_DEVICE: return "no usable device";
+    default: return "unknown status";
+  }
+}
+
+const char* pa_last_error_string(void) { return t_last_error.c_str(); }
+
+int pa_version(void) { return 10000; }
+
+int pa_set_device(int device) {
+  PA_HIP(hipSetDevice(device));
+  return PA_OK;
+}
+
+int pa_synchronize(pa_stream_t stream) {
+  PA_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return PA_OK;
+}
+
+int pa_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, pa_stream_t stream) {
+  if (n == 0) return PA_OK;
+  if (!dst || !src) return arg_error("pa_copy_u32: null pointer");
+  if (((uintptr_t)dst | (uintptr_t)src) & 15) return arg_error("pa_copy_u32: pointers must be 16-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  size_t nvec = n / 4;
+  unsigned grid = (unsigned)std::min<size_t>((nvec + 255) / 256 + 1, 4096);
+  ProfScope ps("k_copy_u32", s);
+  hipLaunchKernelGGL(k_copy_u32, dim3(grid), dim3(256), 0, s, dst, src, nvec, n);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+int pa_profile_enable(int enable) {
+  g_prof_on.store(enable != 0);
+  return PA_OK;
+}
+
+int pa_profile_query(const char* prefix, double* total_ms, uint64_t* launches) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  double tot = 0.0;
+  uint64_t cnt = 0;
+  size_t plen = prefix ? std::strlen(prefix) : 0;
+  for (auto& r : g_prof) {
+    if (plen && r.name.compare(0, plen, prefix) != 0) continue;
+    PA_HIP(hipEventSynchronize(r.stop));
+    float ms = 0.f;
+    PA_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+    tot += ms;
+    ++cnt;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = cnt;
+  return PA_OK;
+}
+
+int pa_profile_reset(void) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (auto& r : g_prof) {
+    (void)hipEventSynchronize(r.stop);
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  g_prof.clear();
+  return PA_OK;
+}
+
+void pa_gen_mt19937_u32(uint64_t seed, uint32_t* out, size_t n) {
+  std::mt19937_64 g(seed);
+  for (size_t i = 0; i < n; ++i) out[i] = static_cast<uint32_t>(g());
+}
+
+void pa_gen_mt19937_u64(uint64_t seed, uint64_t* out, size_t n) {
+  std::mt19937_64 g(seed);
+  for (size_t i = 0; i < n; ++i) out[i] = g();
+}
+
+}  // extern "C"
