@@ -97,17 +97,21 @@ int pa_hist_digits_u32(const uint32_t* keys, size_t n, uint64_t* counts, pa_stre
  * shift, thr) (HistogramParallel.h:535-576) and the _2 variant (:466-533). */
 int pa_hist_u32(const uint32_t* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, pa_stream_t stream);
 
-/* Per-work-quantum histogram of byte `which_byte`: counts[q][b] with
- * q = index / quantum, nq = ceil(n / quantum) rows of 256 uint64.  Replaces
- * HistogramByteComponentsQCPar<256,8> (HistogramParallel.h:444-464, with
- * QCParInner :399-442 and AcrossWorkQuantasQC :331-397). */
-int pa_hist_quanta_u32(const uint32_t* keys, size_t n, size_t quantum, unsigned which_byte, uint64_t* counts,
-                       pa_stream_t stream);
+/* Per-work-quantum histogram of byte `which_byte` over keys[first, last):
+ * counts[q][b] (nq rows of 256 uint64) with q = index / quantum on the
+ * absolute index; rows outside [first/quantum, (last-1)/quantum] stay zero.
+ * Replaces HistogramByteComponentsQCPar<256,8>(in, l, r, workQuanta,
+ * numberOfQuantas, whichByte, thr) (HistogramParallel.h:444-464, with
+ * QCParInner :399-442 and AcrossWorkQuantasQC :331-397): first = l,
+ * last = r + 1. */
+int pa_hist_quanta_u32(const uint32_t* keys, size_t first, size_t last, size_t quantum, size_t nq, unsigned which_byte,
+                       uint64_t* counts, pa_stream_t stream);
 
 /* start[q][b] = sum_{b'<b} size[b'] + sum_{q'<q} count[q'][b] for byte `digit`
- * (nq x 256 uint64).  Replaces ComputeStartOfBinsPar<256,8>
+ * of keys[0, n) (nq rows of 256 uint64).  Replaces ComputeStartOfBinsPar<256,8>
+ * (in, size, workQuanta, numberOfQuantas, digit, thr)
  * (RadixSortLsdParallel.h:108-165). */
-int pa_start_of_bins_u32(const uint32_t* keys, size_t n, size_t quantum, unsigned digit, uint64_t* start,
+int pa_start_of_bins_u32(const uint32_t* keys, size_t n, size_t quantum, size_t nq, unsigned digit, uint64_t* start,
                          pa_stream_t stream);
 
 /* 256-bin histogram of a byte array.  Replaces
@@ -123,6 +127,12 @@ int pa_hist_u8(const uint8_t* bytes, size_t n, uint64_t* counts, pa_stream_t str
  * (RadixSelect.h:139-168).  result may be host or device memory. */
 int pa_select_u32(const uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream);
 
+/* As pa_select_u32, and also stores the selected value at keys[k] -- the
+ * post-condition SelectRadix's callers rely on (they read a[k] after the call,
+ * RadixSelectBenchmark.cpp:57-58).  The rest of keys is unchanged (the
+ * reference leaves it reordered and partly overwritten, RadixSelect.h:52). */
+int pa_select_radix_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream);
+
 /* ---- Merge ----------------------------------------------------------------- */
 
 /* out[0..na+nb) = stable merge of sorted a[0..na) and b[0..nb) (ties take a
@@ -131,6 +141,15 @@ int pa_select_u32(const uint32_t* keys, size_t n, size_t k, uint32_t* result, pa
  * the drop-in header maps onto it.  out must not overlap a or b. */
 int pa_merge_u32(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, uint32_t* out, pa_stream_t stream);
 int pa_merge_u64(const uint64_t* a, size_t na, const uint64_t* b, size_t nb, uint64_t* out, pa_stream_t stream);
+
+/* In-place merge of the adjacent sorted runs t[0..n1) and t[n1..n1+n2):
+ * merged into device scratch, then copied back over t.  Replaces the in-place
+ * family of ParallelMerge.h (merge_truly_in_place :400-446, merge_in_place
+ * :448-494, p_merge_in_place_2 :499-557, p_merge_truly_in_place :591-651,
+ * p_merge_in_place_adaptive :653-668 and the preventative-adaptive forms
+ * :670-726), whose result is the same merged run. */
+int pa_merge_inplace_u32(uint32_t* t, size_t n1, size_t n2, pa_stream_t stream);
+int pa_merge_inplace_u64(uint64_t* t, size_t n1, size_t n2, pa_stream_t stream);
 
 /* ---- Segmented (batched) sort ---------------------------------------------- */
 

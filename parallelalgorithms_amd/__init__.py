@@ -39,12 +39,15 @@ _SIGS = {
     "pa_sort_kv_u64_u32": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
     "pa_hist_digits_u32": (_i, [_vp, _sz, _vp, _vp]),
     "pa_hist_u32": (_i, [_vp, _sz, _u, _u, _vp, _vp]),
-    "pa_hist_quanta_u32": (_i, [_vp, _sz, _sz, _u, _vp, _vp]),
-    "pa_start_of_bins_u32": (_i, [_vp, _sz, _sz, _u, _vp, _vp]),
+    "pa_hist_quanta_u32": (_i, [_vp, _sz, _sz, _sz, _sz, _u, _vp, _vp]),
+    "pa_start_of_bins_u32": (_i, [_vp, _sz, _sz, _sz, _u, _vp, _vp]),
     "pa_hist_u8": (_i, [_vp, _sz, _vp, _vp]),
     "pa_select_u32": (_i, [_vp, _sz, _sz, _vp, _vp]),
+    "pa_select_radix_u32": (_i, [_vp, _sz, _sz, _vp, _vp]),
     "pa_merge_u32": (_i, [_vp, _sz, _vp, _sz, _vp, _vp]),
     "pa_merge_u64": (_i, [_vp, _sz, _vp, _sz, _vp, _vp]),
+    "pa_merge_inplace_u32": (_i, [_vp, _sz, _sz, _vp]),
+    "pa_merge_inplace_u64": (_i, [_vp, _sz, _sz, _vp]),
     "pa_segmented_sort_u32": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "pa_copy_u32": (_i, [_vp, _vp, _sz, _vp]),
     "pa_profile_enable": (_i, [_i]),
@@ -164,12 +167,12 @@ def hist_u32(keys, shift, bits, counts, stream=None):
     call("pa_hist_u32", _ptr(keys), numel(keys), shift, bits, _ptr(counts), _stream(stream))
 
 
-def hist_quanta_u32(keys, quantum, which_byte, counts, stream=None):
-    call("pa_hist_quanta_u32", _ptr(keys), numel(keys), quantum, which_byte, _ptr(counts), _stream(stream))
+def hist_quanta_u32(keys, first, last, quantum, nq, which_byte, counts, stream=None):
+    call("pa_hist_quanta_u32", _ptr(keys), first, last, quantum, nq, which_byte, _ptr(counts), _stream(stream))
 
 
-def start_of_bins_u32(keys, quantum, digit, out, stream=None):
-    call("pa_start_of_bins_u32", _ptr(keys), numel(keys), quantum, digit, _ptr(out), _stream(stream))
+def start_of_bins_u32(keys, quantum, nq, digit, out, stream=None):
+    call("pa_start_of_bins_u32", _ptr(keys), numel(keys), quantum, nq, digit, _ptr(out), _stream(stream))
 
 
 def hist_u8(data, counts, stream=None):
@@ -180,12 +183,24 @@ def select_u32(keys, k, result, stream=None):
     call("pa_select_u32", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
 
 
+def select_radix_u32(keys, k, result, stream=None):
+    call("pa_select_radix_u32", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
+
+
 def merge_u32(a, b, out, stream=None):
     call("pa_merge_u32", _ptr(a), numel(a), _ptr(b), numel(b), _ptr(out), _stream(stream))
 
 
 def merge_u64(a, b, out, stream=None):
     call("pa_merge_u64", _ptr(a), numel(a), _ptr(b), numel(b), _ptr(out), _stream(stream))
+
+
+def merge_inplace_u32(t, n1, stream=None):
+    call("pa_merge_inplace_u32", _ptr(t), n1, numel(t) - n1, _stream(stream))
+
+
+def merge_inplace_u64(t, n1, stream=None):
+    call("pa_merge_inplace_u64", _ptr(t), n1, numel(t) - n1, _stream(stream))
 
 
 def segmented_sort_u32(keys, tmp, offsets, stream=None):

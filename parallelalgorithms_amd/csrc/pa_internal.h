@@ -82,6 +82,10 @@ struct ProfScope {
 // ---------------------------------------------------------------------------
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Fused 4-digit histogram of u32 keys into zeroed u32 counts[4][256]
+// (defined with the sort kernels).
+int launch_hist_digits_u32(const uint32_t* keys, size_t n, uint32_t* counts, hipStream_t s);
+
 }  // namespace pa
 
 // ---------------------------------------------------------------------------
@@ -116,6 +120,28 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
     if (lane >= (uint32_t)off) v += t;
   }
   return v;
+}
+
+// Wave-aggregated LDS increment of PER digits held by one lane: when every
+// active lane holds PER copies of one digit (constant input, long presorted
+// runs), a single lane adds the whole wave's count instead of 64 lanes
+// serialising on one LDS address.
+template <int PER>
+__device__ __forceinline__ void lds_count(uint32_t* __restrict__ bins, const uint32_t (&dg)[PER]) {
+  bool same = true;
+#pragma unroll
+  for (int j = 1; j < PER; ++j) same = same && (dg[j] == dg[0]);
+  const uint32_t first = __builtin_amdgcn_readfirstlane(dg[0]);
+  const uint64_t active = __ballot(true);
+  const uint64_t uni = __ballot(same && dg[0] == first);
+  if (uni == active) {
+    if (count_below(active) == 0) atomicAdd(&bins[first], (uint32_t)(PER * __popcll(active)));
+  } else if (same) {
+    atomicAdd(&bins[dg[0]], (uint32_t)PER);
+  } else {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) atomicAdd(&bins[dg[j]], 1u);
+  }
 }
 
 __device__ __forceinline__ uint32_t atomic_load_agent(const uint32_t* p) {

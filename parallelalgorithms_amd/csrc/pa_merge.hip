@@ -1,0 +1,201 @@
+// pa_merge.hip -- merge-path merge of two sorted runs for gfx950.
+//
+// Reference: merge_parallel_L5 (ParallelMerge.h:285-315) splits the longer
+// run at its midpoint, binary-searches the pivot in the other run
+// (my_binary_search, BinarySearch.h:33-47), places it and recurses in parallel
+// down to a 32K cut-off, where merge_ptr_1 (:52-75) merges sequentially with
+// `a <= b` taking the first run's element.
+//
+// GPU form: the output is cut into tiles of kMergeTile elements.  A partition
+// kernel finds, for every tile boundary (a diagonal of the merge grid), how many
+// elements come from run A by a binary search along the diagonal -- the same
+// lower-bound search the reference recurses on, done for all splits at once.
+// Each workgroup then loads its A and B slices into LDS, every thread finds its
+// own sub-diagonal in LDS, merges kMergeItems elements (ties take A), and the
+// tile is written out coalesced through LDS.
+#include <algorithm>
+
+#include "pa_internal.h"
+
+namespace pa {
+
+constexpr int kMergeBlock = 256;
+constexpr int kMergeItems = 16;
+constexpr int kMergeTile = kMergeBlock * kMergeItems;
+
+// Number of elements taken from A among the first d outputs of the stable
+// merge (ties take A first).
+template <typename K>
+__device__ __forceinline__ size_t merge_path(const K* a, size_t na, const K* b, size_t nb, size_t d) {
+  size_t lo = d > nb ? d - nb : 0;
+  size_t hi = d < na ? d : na;
+  while (lo < hi) {
+    const size_t mid = lo + (hi - lo) / 2;
+    if (a[mid] <= b[d - 1 - mid])
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_merge_partition(const K* __restrict__ a, size_t na, const K* __restrict__ b,
+                                                         size_t nb, size_t ntiles, size_t* __restrict__ split) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const size_t d = std::min(t * (size_t)kMergeTile, na + nb);
+  split[t] = merge_path(a, na, b, nb, d);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kMergeBlock) void k_merge_tile(const K* __restrict__ a, size_t na, const K* __restrict__ b,
+                                                            size_t nb, const size_t* __restrict__ split,
+                                                            K* __restrict__ out) {
+  __shared__ K s_in[kMergeTile];
+  __shared__ K s_out[kMergeTile];
+  const uint32_t tid = threadIdx.x;
+  const size_t tile = blockIdx.x;
+  const size_t d0 = tile * kMergeTile;
+  const size_t d1 = std::min(d0 + kMergeTile, na + nb);
+  const size_t i0 = split[tile], i1 = split[tile + 1];
+  const size_t j0 = d0 - i0, j1 = d1 - i1;
+  const uint32_t la = (uint32_t)(i1 - i0), lb = (uint32_t)(j1 - j0);
+  for (uint32_t x = tid; x < la; x += kMergeBlock) s_in[x] = a[i0 + x];
+  for (uint32_t x = tid; x < lb; x += kMergeBlock) s_in[la + x] = b[j0 + x];
+  __syncthreads();
+  const K* sa = s_in;
+  const K* sb = s_in + la;
+  const uint32_t total = la + lb;
+  const uint32_t dt = std::min<uint32_t>(tid * kMergeItems, total);
+  uint32_t ia = (uint32_t)merge_path(sa, la, sb, lb, dt);
+  uint32_t ib = dt - ia;
+  K r[kMergeItems];
+#pragma unroll
+  for (int x = 0; x < kMergeItems; ++x) {
+    if (dt + x < total) {
+      const bool take_a = ib >= lb || (ia < la && sa[ia] <= sb[ib]);
+      r[x] = take_a ? sa[ia] : sb[ib];
+      ia += take_a;
+      ib += !take_a;
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < kMergeItems; ++x)
+    if (dt + x < total) s_out[dt + x] = r[x];
+  __syncthreads();
+  for (uint32_t x = tid; x < total; x += kMergeBlock) out[d0 + x] = s_out[x];
+}
+
+template <typename K>
+int merge_entry(const K* a, size_t na, const K* b, size_t nb, K* out, hipStream_t s, const char* who) {
+  const size_t n = na + nb;
+  if (n == 0) return PA_OK;
+  if (!out || (na && !a) || (nb && !b)) return arg_error(who);
+  StreamCtx& ctx = stream_ctx(s);
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  const bool da = na == 0 || is_device_ptr(a);
+  const bool db = nb == 0 || is_device_ptr(b);
+  const bool dout = is_device_ptr(out);
+  const size_t ntiles = (n + kMergeTile - 1) / kMergeTile;
+  Layout L;
+  const size_t o_split = L.take((ntiles + 1) * sizeof(size_t));
+  const size_t o_a = da ? 0 : L.take(na * sizeof(K));
+  const size_t o_b = db ? 0 : L.take(nb * sizeof(K));
+  const size_t o_o = dout ? 0 : L.take(n * sizeof(K));
+  void* base;
+  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  char* p = (char*)base;
+  size_t* split = (size_t*)(p + o_split);
+  const K* dA = a;
+  const K* dB = b;
+  K* dO = dout ? out : (K*)(p + o_o);
+  if (!da) {
+    PA_HIP(hipMemcpyAsync(p + o_a, a, na * sizeof(K), hipMemcpyHostToDevice, s));
+    dA = (const K*)(p + o_a);
+  }
+  if (!db) {
+    PA_HIP(hipMemcpyAsync(p + o_b, b, nb * sizeof(K), hipMemcpyHostToDevice, s));
+    dB = (const K*)(p + o_b);
+  }
+  {
+    ProfScope ps("k_merge_partition", s);
+    hipLaunchKernelGGL(k_merge_partition<K>, dim3((unsigned)((ntiles + 1 + 255) / 256)), dim3(256), 0, s, dA, na, dB,
+                       nb, ntiles, split);
+  }
+  {
+    ProfScope ps("k_merge_tile", s);
+    hipLaunchKernelGGL(k_merge_tile<K>, dim3((unsigned)ntiles), dim3(kMergeBlock), 0, s, dA, na, dB, nb, split, dO);
+  }
+  PA_HIP(hipGetLastError());
+  if (!dout) {
+    PA_HIP(hipMemcpyAsync(out, dO, n * sizeof(K), hipMemcpyDeviceToHost, s));
+  }
+  if (!da || !db || !dout) PA_HIP(hipStreamSynchronize(s));
+  return PA_OK;
+}
+
+// In-place form: merge t[0..n1) and t[n1..n1+n2) into scratch, copy back.
+template <typename K>
+int merge_inplace_entry(K* t, size_t n1, size_t n2, hipStream_t s, const char* who) {
+  const size_t n = n1 + n2;
+  if (n1 == 0 || n2 == 0) return PA_OK;  // one run: already merged
+  if (!t) return arg_error(who);
+  const bool dt = is_device_ptr(t);
+  if (!dt) {
+    // Host runs: both are staged to device memory before the merged result is
+    // copied back over t (same stream, in order), so t may be the output.
+    return merge_entry<K>(t, n1, t + n1, n2, t, s, who);
+  }
+  StreamCtx& ctx = stream_ctx(s);
+  K* out = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    Layout L;
+    const size_t ntiles = (n + kMergeTile - 1) / kMergeTile;
+    const size_t o_split = L.take((ntiles + 1) * sizeof(size_t));
+    const size_t o_out = L.take(n * sizeof(K));
+    void* base;
+    PA_TRY(ensure_scratch(ctx, L.off, &base));
+    char* p = (char*)base;
+    size_t* split = (size_t*)(p + o_split);
+    out = (K*)(p + o_out);
+    {
+      ProfScope ps("k_merge_partition", s);
+      hipLaunchKernelGGL(k_merge_partition<K>, dim3((unsigned)((ntiles + 1 + 255) / 256)), dim3(256), 0, s, t, n1,
+                         t + n1, n2, ntiles, split);
+    }
+    {
+      ProfScope ps("k_merge_tile", s);
+      hipLaunchKernelGGL(k_merge_tile<K>, dim3((unsigned)ntiles), dim3(kMergeBlock), 0, s, t, n1, t + n1, n2, split,
+                         out);
+    }
+    PA_HIP(hipGetLastError());
+    PA_HIP(hipMemcpyAsync(t, out, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+  }
+  return PA_OK;
+}
+
+}  // namespace pa
+
+using namespace pa;
+
+extern "C" {
+
+int pa_merge_u32(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, uint32_t* out, pa_stream_t stream) {
+  return merge_entry<uint32_t>(a, na, b, nb, out, (hipStream_t)stream, "pa_merge_u32: null pointer");
+}
+
+int pa_merge_u64(const uint64_t* a, size_t na, const uint64_t* b, size_t nb, uint64_t* out, pa_stream_t stream) {
+  return merge_entry<uint64_t>(a, na, b, nb, out, (hipStream_t)stream, "pa_merge_u64: null pointer");
+}
+
+int pa_merge_inplace_u32(uint32_t* t, size_t n1, size_t n2, pa_stream_t stream) {
+  return merge_inplace_entry<uint32_t>(t, n1, n2, (hipStream_t)stream, "pa_merge_inplace_u32: null pointer");
+}
+
+int pa_merge_inplace_u64(uint64_t* t, size_t n1, size_t n2, pa_stream_t stream) {
+  return merge_inplace_entry<uint64_t>(t, n1, n2, (hipStream_t)stream, "pa_merge_inplace_u64: null pointer");
+}
+
+}  // extern "C"

@@ -1,0 +1,252 @@
+// pa_segsort.hip -- segmented (batched) LSD radix sort of many u32 arrays in
+// one batch on gfx950.
+//
+// No reference counterpart: the reference sorts one array per call
+// (SortRadixPar, RadixSortLsdParallel.h:367-387) and only benchmarks bundling
+// many small tasks (BundleSmallWorkItems.cpp:36-144).  Each segment's result
+// equals SortRadixPar on that segment (a stable sort of scalar keys is unique).
+//
+// Size classes:
+//   len <= 1              nothing to do.
+//   len <= kSmallTile     one workgroup sorts the segment entirely in LDS
+//                         (4 stable 8-bit passes over LDS, one read and one
+//                         write of HBM per key).  One launch for all of them.
+//   larger                segmented onesweep: one fused read builds every
+//                         segment's 4 digit histograms, then one onesweep
+//                         launch per digit covers every tile of every large
+//                         segment; each segment has its own look-back chain.
+// So the whole batch costs 7 launches however many segments it holds, where
+// per-array calls cost 6 launches per array.
+#include <algorithm>
+#include <vector>
+
+#include "pa_sort_kernels.h"
+
+namespace pa {
+
+constexpr int kSmallBlock = 256;
+constexpr int kSmallItems = 16;
+constexpr uint32_t kSmallTile = kSmallBlock * kSmallItems;  // 4096 keys
+
+// Segmented onesweep tile shape (keys only).
+constexpr int kSegBlock = 512;
+constexpr int kSegItems = 16;
+constexpr int kSegLbw = 8;
+constexpr uint32_t kSegTile = kSegBlock * kSegItems;
+
+// Whole-segment LDS sort.  Padding slots hold 0xFFFFFFFF: a stable sort
+// keeps them after every real key, so the first len slots are the answer.
+__global__ __launch_bounds__(kSmallBlock) void k_seg_small(uint32_t* __restrict__ keys,
+                                                           const uint64_t* __restrict__ begin,
+                                                           const uint32_t* __restrict__ len) {
+  constexpr int WAVES = kSmallBlock / 64;
+  constexpr int WAVE_SPAN = 64 * kSmallItems;
+  __shared__ uint32_t s_keys[kSmallTile];
+  __shared__ uint32_t s_whist[WAVES * 256];
+  __shared__ uint32_t s_wsum[WAVES];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t* seg = keys + begin[blockIdx.x];
+  const uint32_t n = len[blockIdx.x];
+  uint32_t key[kSmallItems];
+  const uint32_t wbase = wave * WAVE_SPAN + lane;
+#pragma unroll
+  for (int i = 0; i < kSmallItems; ++i) {
+    const uint32_t idx = wbase + i * 64;
+    key[i] = idx < n ? seg[idx] : 0xFFFFFFFFu;
+  }
+  uint32_t* wh = s_whist + wave * 256;
+  for (uint32_t shift = 0; shift < 32; shift += 8) {
+    for (uint32_t i = tid; i < WAVES * 256; i += kSmallBlock) s_whist[i] = 0;
+    __syncthreads();
+    uint32_t rank[kSmallItems];
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+      const uint32_t d = (key[i] >> shift) & 0xFFu;
+      uint64_t peers = ~0ull;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+      }
+      const uint32_t below = dev::count_below(peers);
+      const uint32_t base = wh[d];
+      rank[i] = base + below;
+      if (below == 0) wh[d] = base + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // Digit totals, wave prefixes and the block-wide digit offsets.
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = s_whist[w * 256 + tid];
+      s_whist[w * 256 + tid] = cnt;
+      cnt += c;
+    }
+    const uint32_t inc = dev::wave_incl_scan(cnt, lane);
+    if (lane == 63) s_wsum[wave] = inc;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
+    const uint32_t start = pre + inc - cnt;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) s_whist[w * 256 + tid] += start;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) s_keys[wh[(key[i] >> shift) & 0xFFu] + rank[i]] = key[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) key[i] = s_keys[wbase + i * 64];
+    __syncthreads();
+  }
+  for (uint32_t j = tid; j < n; j += kSmallBlock) seg[j] = s_keys[j];
+}
+
+__global__ __launch_bounds__(256) void k_seg_prepare(uint32_t* __restrict__ counts, size_t ncounts,
+                                                     uint32_t* __restrict__ counters) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncounts; i += stride) counts[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;
+}
+
+// One tile per block: 4 digit histograms of the tile's keys into its
+// segment's counts; also clears the look-back table.
+__global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint32_t* __restrict__ keys,
+                                                        const uint64_t* __restrict__ tile_info,
+                                                        const uint64_t* __restrict__ seg_begin,
+                                                        const uint32_t* __restrict__ seg_len,
+                                                        uint32_t* __restrict__ counts, uint4* __restrict__ zero_ptr,
+                                                        size_t zero_vec) {
+  __shared__ uint32_t s_hist[4 * 256];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < 4 * 256; i += kSegBlock) s_hist[i] = 0;
+  const size_t stride = (size_t)gridDim.x * kSegBlock;
+  for (size_t i = (size_t)blockIdx.x * kSegBlock + tid; i < zero_vec; i += stride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  const uint64_t info = tile_info[blockIdx.x];
+  const uint32_t seg = (uint32_t)(info >> 32), local = (uint32_t)info;
+  const uint32_t* k = keys + seg_begin[seg];
+  const uint32_t n = seg_len[seg];
+  const uint32_t lo = local * kSegTile, hi = min(lo + kSegTile, n);
+  for (uint32_t idx = lo + tid; idx < hi; idx += kSegBlock) {
+    const uint32_t key = k[idx];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const uint32_t dg[1] = {(key >> (8 * d)) & 0xFFu};
+      dev::lds_count<1>(s_hist + d * 256, dg);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < 4 * 256; i += kSegBlock)
+    if (s_hist[i]) atomicAdd(&counts[(size_t)seg * 1024 + i], s_hist[i]);
+}
+
+}  // namespace pa
+
+using namespace pa;
+
+extern "C" {
+
+int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets, size_t num_segments,
+                          pa_stream_t stream) {
+  if (num_segments == 0) return PA_OK;
+  if (!offsets) return arg_error("pa_segmented_sort_u32: null offsets");
+  for (size_t i = 0; i < num_segments; ++i)
+    if (offsets[i + 1] < offsets[i]) return arg_error("pa_segmented_sort_u32: offsets must be non-decreasing");
+  const uint64_t lo = offsets[0], hi = offsets[num_segments];
+  if (hi == lo) return PA_OK;
+  if (!keys) return arg_error("pa_segmented_sort_u32: null keys");
+  hipStream_t s = (hipStream_t)stream;
+  if (!is_device_ptr(keys) || (tmp && !is_device_ptr(tmp)))
+    return arg_error("pa_segmented_sort_u32: keys and tmp must be device memory");
+
+  // Host-side plan: size classes and the tile table of the large segments.
+  std::vector<uint64_t> small_begin, large_begin, tile_info;
+  std::vector<uint32_t> small_len, large_len;
+  for (size_t i = 0; i < num_segments; ++i) {
+    const uint64_t len = offsets[i + 1] - offsets[i];
+    if (len <= 1) continue;
+    if (len >= (1ull << 30)) {
+      set_last_error("pa_segmented_sort_u32: a segment holds 2^30 or more keys");
+      return PA_ERR_TOO_LARGE;
+    }
+    if (len <= kSmallTile) {
+      small_begin.push_back(offsets[i]);
+      small_len.push_back((uint32_t)len);
+    } else {
+      const uint64_t seg = large_begin.size();
+      large_begin.push_back(offsets[i]);
+      large_len.push_back((uint32_t)len);
+      const uint32_t tiles = (uint32_t)((len + kSegTile - 1) / kSegTile);
+      for (uint32_t t = 0; t < tiles; ++t) tile_info.push_back((seg << 32) | t);
+    }
+  }
+  const size_t ns = small_begin.size(), nl = large_begin.size(), nt = tile_info.size();
+  if (nt >= (1ull << 32)) return arg_error("pa_segmented_sort_u32: too many tiles");
+
+  StreamCtx& ctx = stream_ctx(s);
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  // Tables go through pinned staging so the upload stays asynchronous.
+  Layout T;
+  const size_t t_sb = T.take(ns * 8), t_sl = T.take(ns * 4), t_lb = T.take(nl * 8), t_ll = T.take(nl * 4),
+               t_ti = T.take(nt * 8);
+  const size_t table_bytes = T.off;
+  Layout L;
+  const size_t o_tab = L.take(table_bytes);
+  const size_t o_cnt = L.take(nl * 1024 * 4);
+  const size_t o_ctr = L.take(16);
+  const size_t lb_bytes = (size_t)4 * nt * 256 * 4;
+  const size_t o_lb = L.take(lb_bytes);
+  const size_t o_tmp = tmp ? 0 : L.take((size_t)hi * 4);
+  void* base;
+  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  void* pinned;
+  PA_HIP(hipStreamSynchronize(s));  // the pinned staging may still feed an earlier call
+  PA_TRY(ensure_pinned(ctx, table_bytes, &pinned));
+  char* pin = (char*)pinned;
+  std::copy(small_begin.begin(), small_begin.end(), (uint64_t*)(pin + t_sb));
+  std::copy(small_len.begin(), small_len.end(), (uint32_t*)(pin + t_sl));
+  std::copy(large_begin.begin(), large_begin.end(), (uint64_t*)(pin + t_lb));
+  std::copy(large_len.begin(), large_len.end(), (uint32_t*)(pin + t_ll));
+  std::copy(tile_info.begin(), tile_info.end(), (uint64_t*)(pin + t_ti));
+  char* b = (char*)base;
+  char* tab = b + o_tab;
+  PA_HIP(hipMemcpyAsync(tab, pin, table_bytes, hipMemcpyHostToDevice, s));
+  const uint64_t* d_sb = (const uint64_t*)(tab + t_sb);
+  const uint32_t* d_sl = (const uint32_t*)(tab + t_sl);
+  const uint64_t* d_lb = (const uint64_t*)(tab + t_lb);
+  const uint32_t* d_ll = (const uint32_t*)(tab + t_ll);
+  const uint64_t* d_ti = (const uint64_t*)(tab + t_ti);
+  uint32_t* counts = (uint32_t*)(b + o_cnt);
+  uint32_t* ctr = (uint32_t*)(b + o_ctr);
+  uint32_t* lbp = (uint32_t*)(b + o_lb);
+  uint32_t* work = tmp ? tmp : (uint32_t*)(b + o_tmp);
+
+  if (ns) {
+    ProfScope ps("k_seg_small", s);
+    hipLaunchKernelGGL(k_seg_small, dim3((unsigned)ns), dim3(kSmallBlock), 0, s, keys, d_sb, d_sl);
+  }
+  if (nl) {
+    hipLaunchKernelGGL(k_seg_prepare, dim3((unsigned)std::min<size_t>((nl * 1024 + 255) / 256, 4096)), dim3(256), 0, s,
+                       counts, nl * 1024, ctr);
+    {
+      ProfScope ps("k_seg_hist", s);
+      hipLaunchKernelGGL(k_seg_hist, dim3((unsigned)nt), dim3(kSegBlock), 0, s, keys, d_ti, d_lb, d_ll, counts,
+                         (uint4*)lbp, lb_bytes / 16);
+    }
+    uint32_t* kin = keys;
+    uint32_t* kout = work;
+    for (int p = 0; p < 4; ++p) {
+      ProfScope ps("k_onesweep_seg", s);
+      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, true>), dim3((unsigned)nt),
+                         dim3(kSegBlock), 0, s, kin, kout, (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
+                         counts + p * 256, lbp + (size_t)p * nt * 256, ctr + p, (uint32_t)(8 * p), 0u, d_ti, d_lb,
+                         d_ll, 4u);
+      std::swap(kin, kout);
+    }
+  }
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,294 @@
+// pa_sort_kernels.h -- device kernels of the LSD radix sort, shared by the
+// plain sort (pa_sort.hip) and the segmented batch sort (pa_segsort.hip).
+#pragma once
+
+#include "pa_internal.h"
+
+namespace pa {
+
+constexpr int kSortBlock = 512;  // histogram kernel block
+
+// Look-back status word: 2 flag bits + 30-bit count.
+constexpr uint32_t kLbAgg = 1u << 30;
+constexpr uint32_t kLbInc = 2u << 30;
+constexpr uint32_t kLbMask = (1u << 30) - 1;
+constexpr size_t kMaxSortN = (size_t(1) << 30) - 1;
+
+enum : uint32_t { kFloatIn = 1u, kFloatOut = 2u };
+
+
+// One read of the keys -> D digit histograms (D = sizeof(K)).
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict__ keys, size_t n,
+                                                            uint32_t* __restrict__ counts, uint4* __restrict__ zero_ptr,
+                                                            size_t zero_vec, uint32_t float_keys) {
+  constexpr int D = sizeof(K);
+  constexpr int PER = 16 / sizeof(K);
+  __shared__ uint32_t s_hist[D * 256];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < D * 256; i += kSortBlock) s_hist[i] = 0;
+
+  // Clear the look-back table used by the following onesweep passes.
+  const size_t gstride = (size_t)gridDim.x * kSortBlock;
+  for (size_t i = (size_t)blockIdx.x * kSortBlock + tid; i < zero_vec; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+
+  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
+  if (head > n) head = n;
+  const size_t nvec = (n - head) / PER;
+  const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
+  for (size_t v = (size_t)blockIdx.x * kSortBlock + tid; v < nvec; v += gstride) {
+    uint4 x = vk[v];
+    K k[PER];
+    __builtin_memcpy(k, &x, 16);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      uint32_t dg[PER];
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        K kk = k[j];
+        if (sizeof(K) == 4 && float_keys) kk = (K)dev::f32_fwd((uint32_t)kk);
+        dg[j] = dev::digit8<K>(kk, 8 * d);
+      }
+      dev::lds_count<PER>(s_hist + d * 256, dg);
+    }
+  }
+  // Head and tail elements that do not fill a 16-byte vector.
+  if (blockIdx.x == 0) {
+    const size_t tail_start = head + nvec * PER;
+    const size_t extra = head + (n - tail_start);
+    for (size_t e = tid; e < extra; e += kSortBlock) {
+      size_t idx = e < head ? e : tail_start + (e - head);
+      K kk = keys[idx];
+      if (sizeof(K) == 4 && float_keys) kk = (K)dev::f32_fwd((uint32_t)kk);
+#pragma unroll
+      for (int d = 0; d < D; ++d) atomicAdd(&s_hist[d * 256 + dev::digit8<K>(kk, 8 * d)], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < D * 256; i += kSortBlock) {
+    uint32_t c = s_hist[i];
+    if (c) atomicAdd(&counts[i], c);
+  }
+}
+
+// One stable digit pass over a tile of BLOCK x ITEMS keys.  Tiles are claimed
+// in order from a global counter so every tile a block waits on in the
+// look-back is already held by a running block.  The look-back reads LBW
+// predecessor status words per round trip.
+//
+// Segmented mode (SEG): the keys hold many independent segments; tile_info[t]
+// = (segment << 32) | tile index within the segment, seg_begin/seg_len give
+// each segment's range, gcount is strided by 256 * gstride per segment, and
+// the look-back stops at the segment's first tile.  Tiles never span segments.
+template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool SEG = false>
+__global__ __launch_bounds__(BLOCK) void k_onesweep(const K* __restrict__ kin, K* __restrict__ kout,
+                                                    const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
+                                                    uint32_t n, const uint32_t* __restrict__ gcount,
+                                                    uint32_t* __restrict__ lb, uint32_t* __restrict__ tile_counter,
+                                                    uint32_t shift, uint32_t fmode,
+                                                    const uint64_t* __restrict__ tile_info = nullptr,
+                                                    const uint64_t* __restrict__ seg_begin = nullptr,
+                                                    const uint32_t* __restrict__ seg_len = nullptr,
+                                                    uint32_t gstride = 0) {
+  static_assert(BLOCK >= 256 && BLOCK % 64 == 0, "one thread per digit is needed");
+  constexpr int WAVES = BLOCK / 64;
+  constexpr int TILE = BLOCK * ITEMS;
+  constexpr int WAVE_SPAN = 64 * ITEMS;
+  __shared__ K s_keys[TILE];
+  __shared__ uint32_t s_whist[WAVES * 256];
+  __shared__ uint32_t s_goff[256];
+  __shared__ uint32_t s_wsum[2][4];
+  __shared__ uint32_t s_tile;
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63;
+  const uint32_t wave = tid >> 6;
+
+  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+  for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_whist[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  uint32_t local = tile;  // tile index within its segment
+  if constexpr (SEG) {
+    const uint64_t info = tile_info[tile];
+    const uint32_t seg = (uint32_t)(info >> 32);
+    local = (uint32_t)info;
+    kin += seg_begin[seg];
+    kout += seg_begin[seg];
+    if constexpr (KV) {
+      vin += seg_begin[seg];
+      vout += seg_begin[seg];
+    }
+    n = seg_len[seg];
+    gcount += (size_t)seg * gstride * 256;
+  }
+  const uint32_t first_tile = tile - local;  // look-back stops here
+  const uint32_t tile_base = local * (uint32_t)TILE;
+  const uint32_t valid = min((uint32_t)TILE, n - tile_base);
+  const bool full = valid == (uint32_t)TILE;
+
+  // Wave-striped load: wave w owns [tile_base + w*WAVE_SPAN, +WAVE_SPAN); item
+  // i of lane l is element i*64 + l of that span, so (item, lane) order is the
+  // input order and every load instruction reads 64 consecutive keys.
+  const uint32_t wbase = tile_base + wave * WAVE_SPAN + lane;
+  K key[ITEMS];
+  uint32_t val[KV ? ITEMS : 1];
+  if (full) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) key[i] = kin[wbase + i * 64];
+    if constexpr (KV) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) val[i] = vin[wbase + i * 64];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      uint32_t idx = wbase + i * 64;
+      key[i] = idx < n ? kin[idx] : K(0);
+    }
+    if constexpr (KV) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        uint32_t idx = wbase + i * 64;
+        val[i] = idx < n ? vin[idx] : 0u;
+      }
+    }
+  }
+  if (sizeof(K) == 4 && (fmode & kFloatIn)) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) key[i] = (K)dev::f32_fwd((uint32_t)key[i]);
+  }
+
+  // Stable rank within the wave: lanes holding the same digit are found with
+  // 8 ballots; the lowest such lane bumps the wave's private counter.
+  uint32_t rank[ITEMS];
+  uint32_t* wh = s_whist + wave * 256;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t d = dev::digit8<K>(key[i], shift);
+    const bool ok = full || (wbase + i * 64 < n);
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t below = dev::count_below(peers);
+    const uint32_t base = wh[d];
+    rank[i] = base + below;
+    if (ok && below == 0) wh[d] = base + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+
+  // Per digit: exclusive prefix over waves, tile total, publish the aggregate.
+  uint32_t cnt = 0, gval = 0;
+  if (tid < 256) {
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+      uint32_t c = s_whist[w * 256 + tid];
+      s_whist[w * 256 + tid] = cnt;
+      cnt += c;
+    }
+    dev::atomic_store_agent(&lb[(size_t)tile * 256 + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
+    gval = gcount[tid];
+  }
+  // Exclusive scans over the 256 digits of (tile count, global count).
+  uint32_t cinc = 0, ginc = 0;
+  if (tid < 256) {
+    cinc = dev::wave_incl_scan(cnt, lane);
+    ginc = dev::wave_incl_scan(gval, lane);
+    if (lane == 63) {
+      s_wsum[0][wave] = cinc;
+      s_wsum[1][wave] = ginc;
+    }
+  }
+  __syncthreads();
+  if (tid < 256) {
+    uint32_t cpre = 0, gpre = 0;
+    for (uint32_t w = 0; w < wave; ++w) {
+      cpre += s_wsum[0][w];
+      gpre += s_wsum[1][w];
+    }
+    const uint32_t tstart = cpre + cinc - cnt;   // tile-local start of this digit
+    const uint32_t gstart = gpre + ginc - gval;  // global start of this digit
+
+    // Decoupled look-back over earlier tiles of the segment for this digit, LBW
+    // status words per round trip; words before the first tile read as
+    // "inclusive 0".
+    uint32_t excl = 0;
+    if (local > 0) {
+      int64_t p = (int64_t)tile - 1;
+      for (;;) {
+        uint32_t w[LBW];
+#pragma unroll
+        for (int j = 0; j < LBW; ++j)
+          w[j] = (p - j >= (int64_t)first_tile) ? dev::atomic_load_agent(&lb[(size_t)(p - j) * 256 + tid]) : kLbInc;
+        int used = 0;
+        bool stop = false, done = false;
+#pragma unroll
+        for (int j = 0; j < LBW; ++j) {
+          if (!stop) {
+            const uint32_t flag = w[j] & ~kLbMask;
+            if (flag == 0) {
+              stop = true;  // not published yet: poll again from here
+            } else {
+              excl += w[j] & kLbMask;
+              ++used;
+              if (flag == kLbInc) stop = done = true;
+            }
+          }
+        }
+        if (done) break;
+        p -= used;
+        if (used == 0) __builtin_amdgcn_s_sleep(1);
+      }
+      dev::atomic_store_agent(&lb[(size_t)tile * 256 + tid], kLbInc | (excl + cnt));
+    }
+    s_goff[tid] = gstart + excl - tstart;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) s_whist[w * 256 + tid] += tstart;
+  }
+  __syncthreads();
+
+  // Stage the tile in LDS in digit order.
+  uint32_t lpos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t d = dev::digit8<K>(key[i], shift);
+    lpos[i] = wh[d] + rank[i];
+    if (full || (wbase + i * 64 < n)) s_keys[lpos[i]] = key[i];
+  }
+  __syncthreads();
+
+  // Coalesced write-out: consecutive slots of one digit go to consecutive
+  // addresses of that digit's bin.
+  uint32_t gpos[KV ? ITEMS : 1];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t j = tid + i * BLOCK;
+    if (j < valid) {
+      K k = s_keys[j];
+      const uint32_t p = s_goff[dev::digit8<K>(k, shift)] + j;
+      if (sizeof(K) == 4 && (fmode & kFloatOut)) k = (K)dev::f32_inv((uint32_t)k);
+      kout[p] = k;
+      if constexpr (KV) gpos[i] = p;
+    }
+  }
+  if constexpr (KV) {
+    __syncthreads();
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if (full || (wbase + i * 64 < n)) s_vals[lpos[i]] = val[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t j = tid + i * BLOCK;
+      if (j < valid) vout[gpos[i]] = s_vals[j];
+    }
+  }
+}
+
+}  // namespace pa

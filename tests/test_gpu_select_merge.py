@@ -1,0 +1,88 @@
+"""GPU parity of radix select and merge against the oracle.
+
+Select mirrors RadixSelectBenchmark.cpp:33-67 (SelectRadix vs nth_element at
+k = n - 21) plus the configured ranks k = n/2 and n/100; the selected value is
+unique, so equality is exact.  Merge mirrors merge_parallel_L5
+(ParallelMerge.h:285-315): for scalar keys the merged output is unique.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import dev_empty, make_u32, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dist", ["uniform", "presorted", "constant", "reversed", "nearly", "few"])
+@pytest.mark.parametrize("n", [1, 2, 7, 100, 4097, 1_000_003])
+def test_select_matches_oracle(pa, orc, dist, n):
+    a = make_u32(dist, n)
+    d = to_dev(a)
+    for k in sorted({0, n // 100, n // 2, max(0, n - 21), n - 1}):
+        expect, _ = orc.select_radix_u32(a, k)
+        res = np.zeros(1, dtype=np.uint32)
+        pa.select_u32(d, k, res)  # host result: synchronous
+        assert int(res[0]) == expect, (dist, n, k)
+    np.testing.assert_array_equal(to_host(d, np.uint32), a)  # input untouched
+
+
+def test_select_device_result_and_host_keys(pa, orc):
+    n = 300_001
+    a = make_u32("uniform", n, seed=21)
+    k = 12345
+    expect, _ = orc.select_radix_u32(a, k)
+    r = dev_empty(1, np.uint32)
+    pa.select_u32(to_dev(a), k, r)
+    assert int(to_host(r, np.uint32)[0]) == expect
+    res = np.zeros(1, dtype=np.uint32)
+    pa.select_u32(a, k, res)
+    assert int(res[0]) == expect
+
+
+def test_select_rejects_bad_k(pa):
+    import parallelalgorithms_amd as P
+    a = to_dev(np.arange(10, dtype=np.uint32))
+    res = np.zeros(1, dtype=np.uint32)
+    with pytest.raises(P.PAError):
+        pa.select_u32(a, 10, res)
+
+
+def _sorted_u32(seed, n, mod=None):
+    import parallelalgorithms_amd as P
+    x = P.gen_u32(seed, n)
+    if mod:
+        x = x % np.uint32(mod)
+    return np.sort(x)
+
+
+@pytest.mark.parametrize("na,nb", [(0, 0), (0, 9), (9, 0), (1, 1), (4096, 1), (5000, 70_001), (1_000_003, 999_999)])
+@pytest.mark.parametrize("mod", [None, 100])
+def test_merge_u32_matches_oracle(pa, orc, na, nb, mod):
+    x, y = _sorted_u32(37, na, mod), _sorted_u32(38, nb, mod)
+    t = np.concatenate([x, y]).astype(np.uint32)
+    expect = np.zeros(na + nb, dtype=np.uint32)
+    if na + nb:
+        wrap = lambda p, m: (p + m - 1) % (1 << 64)
+        orc.merge_parallel_l5_u32(t, 0, wrap(0, na), na, wrap(na, nb), expect, 0)
+    out = dev_empty(max(1, na + nb), np.uint32)
+    dx = to_dev(x) if na else dev_empty(1, np.uint32)
+    dy = to_dev(y) if nb else dev_empty(1, np.uint32)
+    import parallelalgorithms_amd as P
+    P.call("pa_merge_u32", dx.data_ptr(), na, dy.data_ptr(), nb, out.data_ptr(), P._stream(None))
+    got = to_host(out, np.uint32)[: na + nb]
+    np.testing.assert_array_equal(got, expect)
+
+
+def test_merge_u64_and_host_pointers(pa, orc):
+    import parallelalgorithms_amd as P
+    x = np.sort(P.gen_u64(39, 300_000))
+    y = np.sort(P.gen_u64(40, 200_001))
+    t = np.concatenate([x, y])
+    expect = np.zeros(t.size, dtype=np.uint64)
+    orc.merge_parallel_l5_u64(t, 0, x.size - 1, x.size, t.size - 1, expect, 0)
+    out = dev_empty(t.size, np.uint64)
+    pa.merge_u64(to_dev(x), to_dev(y), out)
+    np.testing.assert_array_equal(to_host(out, np.uint64), expect)
+    hout = np.zeros(t.size, dtype=np.uint64)
+    pa.merge_u64(x, y, hout)  # host memory: staged, synchronous
+    np.testing.assert_array_equal(hout, expect)

@@ -1,0 +1,308 @@
+"""Pins the CPU oracle (oracle/pa_oracle.cpp) before it is trusted as the
+GPU's checker.  The reference ships no golden vectors or fixtures and its
+parallel path does not build here (TBB headers absent), so every oracle
+function is checked against what the mathematics fixes: numpy closed forms
+(sort, stable argsort, bincount, partition), brute force on small inputs,
+exact invariants, and -- for the serial helpers whose reference headers do
+compile -- the partial reference build in oracle/_ref.  Each check is chosen
+so that a plausible slip (wrong digit shift, off-by-one bound, unstable
+scatter, wrong quantum row) fails it.
+"""
+import numpy as np
+import pytest
+
+import parallelalgorithms_amd as P
+
+EDGE = [0, 1, 2, 99, 100, 101, 1000, 65537]
+
+
+def gen(dist, n, seed=1234):
+    if dist == "uniform":
+        return P.gen_u32(seed, n)
+    if dist == "presorted":
+        return np.arange(n, dtype=np.uint32)
+    if dist == "constant":
+        return np.full(n, 0x5A5A5A5A, dtype=np.uint32)
+    if dist == "reversed":
+        return np.arange(n, 0, -1, dtype=np.uint64).astype(np.uint32)
+    if dist == "few":
+        return (P.gen_u32(seed, n) % 5).astype(np.uint32) * np.uint32(0x01000001)
+    raise ValueError(dist)
+
+
+# ---------------------------------------------------------------- generator
+def test_mt19937_64_standard_value():
+    # C++ [rand.predef]: the 10000th consecutive invocation of a default-
+    # constructed mt19937_64 (seed 5489) produces 9981545732273789042.
+    assert int(P.gen_u64(5489, 10000)[-1]) == 9981545732273789042
+
+
+def test_u32_generator_truncates_the_u64_stream():
+    # ParallelAlgorithms.cpp:114-115: static_cast<unsigned>(dist()).
+    assert np.array_equal(P.gen_u32(1234, 1000), (P.gen_u64(1234, 1000) & 0xFFFFFFFF).astype(np.uint32))
+
+
+# ---------------------------------------------------------------- sorts
+@pytest.mark.parametrize("dist", ["uniform", "presorted", "constant", "reversed", "few"])
+@pytest.mark.parametrize("n", EDGE + [1_000_003])
+def test_sort_radix_par_is_sorted_permutation(orc, dist, n):
+    a = gen(dist, n)
+    np.testing.assert_array_equal(orc.sort_radix_par_u32(a), np.sort(a))
+    np.testing.assert_array_equal(orc.sort_radix_par_alloc_u32(a), np.sort(a))
+
+
+@pytest.mark.parametrize("thr", [1, 100, 4096, 512 * 1024])
+def test_sort_radix_par_any_quantum(orc, thr):
+    # Quantum sizing only changes the split; the result must not.
+    a = gen("uniform", 300_007, seed=5)
+    np.testing.assert_array_equal(orc.sort_radix_par_u32(a, thr), np.sort(a))
+
+
+@pytest.mark.parametrize("n", [0, 1, 99, 100, 5000])
+def test_two_phase_result_in_both_buffers(orc, n):
+    a = gen("uniform", n, seed=2)
+    x, y = orc.sort_two_phase_u32(a)
+    np.testing.assert_array_equal(x, np.sort(a))
+    np.testing.assert_array_equal(y, np.sort(a))
+
+
+@pytest.mark.parametrize("shift", [0, 8, 16, 24])
+def test_counting_pass_is_stable_digit_sort(orc, shift):
+    a = gen("uniform", 50_000, seed=3)
+    d = (a >> np.uint32(shift)) & np.uint32(0xFF)
+    np.testing.assert_array_equal(orc.counting_pass_u32(a, shift), a[np.argsort(d, kind="stable")])
+
+
+@pytest.mark.parametrize("n", [0, 1, 100, 100_003])
+def test_sort_u64_full_range(orc, n):
+    a = P.gen_u64(1235, n)
+    np.testing.assert_array_equal(orc.sort_u64(a), np.sort(a))
+
+
+@pytest.mark.parametrize("n", [1, 100, 100_003])
+def test_sort_kv_is_the_stable_permutation(orc, n):
+    k = P.gen_u64(1235, n)
+    k[::4] = k[0] if n else 0
+    v = np.arange(n, dtype=np.uint32)
+    ek, ev = orc.sort_kv_u64_u32(k, v)
+    order = np.argsort(k, kind="stable")
+    np.testing.assert_array_equal(ev, order.astype(np.uint32))
+    np.testing.assert_array_equal(ek, k[order])
+    k32 = (k & 0xF).astype(np.uint32)
+    ek, ev = orc.sort_kv_u32_u32(k32, v)
+    order = np.argsort(k32, kind="stable")
+    np.testing.assert_array_equal(ev, order.astype(np.uint32))
+
+
+def test_sort_kv_stability_brute_force_tiny(orc):
+    # All 4^6 key sequences of length 6 over 4 values, payload = index.
+    import itertools
+    for keys in itertools.islice(itertools.product(range(4), repeat=6), 0, 4096, 7):
+        k = np.array(keys, dtype=np.uint32) * np.uint32(0x01010101)
+        v = np.arange(6, dtype=np.uint32)
+        ek, ev = orc.sort_kv_u32_u32(k, v)
+        expect = sorted(range(6), key=lambda i: (k[i], i))
+        assert list(ev) == expect
+
+
+# ---------------------------------------------------------------- floats
+def test_f32_transform_roundtrip_exhaustive(orc):
+    assert orc.load().orc_f32_transform_exhaustive_check() == 0
+
+
+def test_f32_transform_orders_like_ieee(orc):
+    vals = np.array([-np.inf, -3.5, -1e-45, -0.0, 0.0, 1e-45, 2.0, np.inf], dtype=np.float32)
+    bits = vals.view(np.uint32)
+    f = [orc.f32_to_ordered(int(b)) for b in bits]
+    assert f == sorted(f) and len(set(f)) == len(f)  # strictly increasing, -0 < +0
+    nan_neg, nan_pos = orc.f32_to_ordered(0xFFC00000), orc.f32_to_ordered(0x7FC00000)
+    assert nan_neg < f[0] and nan_pos > f[-1]
+
+
+def test_sort_f32_matches_numpy_on_finite(orc):
+    x = (P.gen_u64(1236, 100_000) >> np.uint64(40)).astype(np.float32) - np.float32(1 << 23)
+    bits = x.view(np.uint32)
+    got = orc.sort_f32_bits(bits).view(np.float32)
+    np.testing.assert_array_equal(got, np.sort(x))
+
+
+# ---------------------------------------------------------------- histograms
+@pytest.mark.parametrize("dist", ["uniform", "presorted", "constant"])
+@pytest.mark.parametrize("thr", [1, 1000, 64 * 1024])
+def test_hist_byte_components_rows(orc, dist, thr):
+    n = 200_003
+    a = gen(dist, n)
+    h = orc.hist_byte_components_u32(a, 0, n - 1, thr)
+    assert h.shape == (8, 256) and not h[4:].any()  # the reference's 8-row shape
+    for d in range(4):
+        np.testing.assert_array_equal(h[d], np.bincount((a >> np.uint32(8 * d)) & 0xFF, minlength=256))
+
+
+def test_hist_byte_components_matches_reference_serial(orc):
+    ref = orc.load_ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    a = gen("uniform", 70_001, seed=6)
+    out = np.zeros(1024, dtype=np.uint64)
+    ref.ref_hist_byte_components_serial_u32(a.ctypes.data, 0, a.size - 1, out.ctypes.data)
+    np.testing.assert_array_equal(orc.hist_byte_components_u32(a, 0, a.size - 1)[:4].ravel(), out)
+
+
+@pytest.mark.parametrize("shift,radix", [(24, 256), (0, 256), (12, 4096), (30, 4)])
+def test_hist_one_byte_bincount(orc, shift, radix):
+    n = 100_001
+    a = gen("uniform", n, seed=9)
+    np.testing.assert_array_equal(orc.hist_one_byte_u32(a, 0, n - 1, shift, radix),
+                                  np.bincount((a >> np.uint32(shift)) & np.uint32(radix - 1), minlength=radix))
+
+
+def test_hist_top_byte_invariants_100M(orc):
+    # SURVEY invariants for the 100M top-byte config: presorted 0..n-1 puts
+    # 2^24 keys in bins 0..4 and 16,113,920 in bin 5; constant 0x5A5A5A5A puts
+    # all n in bin 0x5A.
+    n = 100_000_000
+    h = orc.hist_one_byte_u32(np.arange(n, dtype=np.uint32), 0, n - 1, 24)
+    assert list(h[:5]) == [16_777_216] * 5 and h[5] == 16_113_920 and h.sum() == n
+    h = orc.hist_one_byte_u32(np.full(n, 0x5A5A5A5A, dtype=np.uint32), 0, n - 1, 24)
+    assert h[0x5A] == n and h.sum() == n
+
+
+def test_hist_one_byte_2_bug_documented(orc):
+    # The reference's _2 variant (HistogramParallel.h:499) drops the last
+    # element of every leaf; at n = 1,000,003 with the default 64K threshold
+    # the total is n - 16 (16 leaves).  The product returns the full counts.
+    n = 1_000_003
+    a = gen("uniform", n)
+    h = orc.hist_one_byte_2_bugcompat_u32(a, 0, n - 1, 24)
+    assert h.sum() == n - 16
+
+
+@pytest.mark.parametrize("thr", [1, 64, 64 * 1024])
+def test_hist_bytes(orc, thr):
+    data = (P.gen_u32(4, 100_003) & 0xFF).astype(np.uint8)
+    np.testing.assert_array_equal(orc.hist_bytes(data, 0, data.size, thr), np.bincount(data, minlength=256))
+    np.testing.assert_array_equal(orc.hist_bytes(data, 17, 9000, thr), np.bincount(data[17:9000], minlength=256))
+
+
+@pytest.mark.parametrize("quantum", [1, 7, 1000, 65536])
+@pytest.mark.parametrize("byte", [0, 2])
+def test_hist_qc_par_closed_form(orc, quantum, byte):
+    n = 200_003
+    a = gen("uniform", n, seed=10)
+    l, r = 3, n - 5
+    nq = (n + quantum - 1) // quantum
+    got = orc.hist_qc_par_u32(a, l, r, quantum, nq, byte)
+    idx = np.arange(l, r + 1)
+    flat = (idx // quantum) * 256 + ((a[l:r + 1] >> np.uint32(8 * byte)) & 0xFF)
+    np.testing.assert_array_equal(got.ravel(), np.bincount(flat, minlength=nq * 256))
+
+
+@pytest.mark.parametrize("quantum", [1, 1000, 65536, 300_000])
+def test_compute_start_of_bins_closed_form(orc, quantum):
+    n = 200_003
+    a = gen("uniform", n, seed=12)
+    nq = (n + quantum - 1) // quantum
+    for digit in range(4):
+        got = orc.compute_start_of_bins_u32(a, quantum, nq, digit)
+        dg = (a >> np.uint32(8 * digit)) & 0xFF
+        cnt = np.bincount((np.arange(n) // quantum) * 256 + dg, minlength=nq * 256).reshape(nq, 256)
+        size = cnt.sum(axis=0)
+        start = np.concatenate([[0], np.cumsum(size)[:-1]])[None, :] + np.concatenate(
+            [np.zeros((1, 256), dtype=np.int64), np.cumsum(cnt, axis=0)[:-1]], axis=0)
+        np.testing.assert_array_equal(got, start.astype(np.uint64))
+        # Scattering each quantum from its start offsets is the stable pass.
+        assert got[0, 0] == 0
+
+
+# ---------------------------------------------------------------- select
+@pytest.mark.parametrize("dist", ["uniform", "presorted", "constant", "few", "reversed"])
+@pytest.mark.parametrize("kfrac", [0.0, 0.01, 0.5, 0.999, 1.0])
+def test_select_radix_is_kth_of_sorted(orc, dist, kfrac):
+    n = 100_003
+    a = gen(dist, n, seed=13)
+    k = min(n - 1, int(kfrac * n))
+    v, counts = orc.select_radix_u32(a, k)
+    s = np.sort(a)
+    assert v == s[k]
+    # Pass p's candidate range = keys sharing the top p digits with sorted[k]
+    # (RadixSelect.h:116-125), until a bin of one element ends the search.
+    assert counts[0] == n
+    for p in (1, 2, 3):
+        if counts[p] == 0:
+            break
+        mask = np.uint32(0xFFFFFFFF << (32 - 8 * p) & 0xFFFFFFFF)
+        assert counts[p] == np.count_nonzero((a & mask) == (s[k] & mask))
+
+
+def test_select_radix_subrange(orc):
+    a = gen("uniform", 50_000, seed=14)
+    start, length = 1000, 20_000
+    for k in (start, start + 7, start + length - 1):
+        v, _ = orc.select_radix_u32(a, k, start, length)
+        assert v == np.sort(a[start:start + length])[k - start]
+
+
+def test_select_radix_benchmark_k(orc):
+    # RadixSelectBenchmark.cpp:37: k = n - 21 checked against nth_element.
+    a = gen("uniform", 1_000_000)
+    k = a.size - 21
+    assert orc.select_radix_u32(a, k)[0] == np.partition(a, k)[k]
+
+
+# ---------------------------------------------------------------- merge
+@pytest.mark.parametrize("n1,n2", [(0, 0), (0, 5), (5, 0), (1, 1), (40_000, 3), (50_000, 70_001), (100_000, 100_000)])
+def test_merge_l5_equals_sorted_union(orc, n1, n2):
+    x = np.sort(P.gen_u32(15, n1) % 1000) if n1 else np.zeros(0, np.uint32)
+    y = np.sort(P.gen_u32(16, n2) % 1000) if n2 else np.zeros(0, np.uint32)
+    t = np.concatenate([x, y]).astype(np.uint32)
+    out = np.zeros(n1 + n2 + 3, dtype=np.uint32)
+    # Bounds are inclusive; an empty run is r = p - 1 with size_t wrap.
+    wrap = lambda p, m: (p + m - 1) % (1 << 64)
+    orc.merge_parallel_l5_u32(t, 0, wrap(0, n1), n1, wrap(n1, n2), out, 3)
+    np.testing.assert_array_equal(out[3:], np.sort(t))
+
+
+def test_merge_l5_u64(orc):
+    x = np.sort(P.gen_u64(17, 70_000))
+    y = np.sort(P.gen_u64(18, 50_000))
+    t = np.concatenate([x, y])
+    out = np.zeros(t.size, dtype=np.uint64)
+    orc.merge_parallel_l5_u64(t, 0, x.size - 1, x.size, t.size - 1, out, 0)
+    np.testing.assert_array_equal(out, np.sort(t))
+
+
+# ---------------------------------------------------------------- helpers vs reference headers
+def test_binary_search_lower_bound(orc):
+    a = np.sort(P.gen_u32(19, 1000) % 100).astype(np.uint32)
+    ref = orc.load_ref()
+    for v in range(0, 102, 3):
+        for left, right in ((0, 999), (10, 500), (7, 6)):
+            got = orc.binary_search_u32(v, a, left, right)
+            if right >= left:
+                assert got == left + np.searchsorted(a[left:right + 1], v, side="left")
+            else:
+                assert got == left
+            if ref is not None:
+                assert got == ref.ref_binary_search_u32(v, a.ctypes.data, left, right)
+
+
+def test_insertion_sort(orc):
+    for n in (0, 1, 2, 50, 99):
+        a = P.gen_u32(20, n) % 17
+        got = orc.insertion_sort_u32(a)
+        np.testing.assert_array_equal(got, np.sort(a))
+        ref = orc.load_ref()
+        if ref is not None:
+            b = a.astype(np.uint32).copy()
+            ref.ref_insertion_sort_u32(b.ctypes.data, n)
+            np.testing.assert_array_equal(got, b)
+
+
+def test_select_histogram_matches_reference_helper(orc):
+    ref = orc.load_ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    a = gen("uniform", 10_007, seed=21)
+    out = np.zeros(256, dtype=np.uint64)
+    ref.ref_hist_one_component_opt_u32(a.ctypes.data, 5, 10_000, 24, 8, out.ctypes.data)
+    np.testing.assert_array_equal(out, np.bincount(a[5:10_000] >> np.uint32(24), minlength=256))
