@@ -1,0 +1,79 @@
+"""GPU parity of the segmented (batched) sort: every segment must equal the
+oracle's SortRadixPar of that segment and the plain per-array GPU call, bit
+for bit.  Covers every size class (empty, one key, LDS-resident small
+segments, single- and multi-tile onesweep segments) and presorted segments."""
+import numpy as np
+import pytest
+
+from gpu_util import dev_empty, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _segments(seed, count, lo, hi, presorted_frac=0.05):
+    import parallelalgorithms_amd as P
+    r = P.gen_u64(seed, 2 * count + 1)
+    u = (r[:count] >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    lens = np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.uint64)
+    offs = np.zeros(count + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    keys = P.gen_u32(seed + 1, int(offs[-1]))
+    pres = ((r[count:2 * count] >> np.uint64(11)).astype(np.float64) / float(1 << 53)) < presorted_frac
+    for i in np.nonzero(pres)[0]:
+        a, b = int(offs[i]), int(offs[i + 1])
+        keys[a:b] = np.sort(keys[a:b])
+    return keys, offs
+
+
+def _check(keys, offs, got, orc=None, oracle_every=0):
+    for i in range(len(offs) - 1):
+        a, b = int(offs[i]), int(offs[i + 1])
+        expect = np.sort(keys[a:b])
+        if orc is not None and oracle_every and i % oracle_every == 0:
+            expect = orc.sort_radix_par_u32(keys[a:b])
+        np.testing.assert_array_equal(got[a:b], expect, err_msg=f"segment {i} [{a},{b})")
+
+
+def test_segmented_edge_classes(pa, orc):
+    lens = [0, 1, 2, 3, 63, 64, 4095, 4096, 4097, 8191, 8192, 8193, 20_000, 0, 1, 100_003, 7]
+    offs = np.zeros(len(lens) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    import parallelalgorithms_amd as P
+    keys = P.gen_u32(77, int(offs[-1]))
+    keys[offs[5]:offs[6]] = 0x5A5A5A5A  # a constant segment
+    d = to_dev(keys)
+    pa.segmented_sort_u32(d, dev_empty(keys.size, np.uint32), offs)
+    _check(keys, offs, to_host(d, np.uint32), orc, oracle_every=1)
+
+
+def test_segmented_tmp_null_and_offset_base(pa):
+    import parallelalgorithms_amd as P
+    keys = P.gen_u32(78, 300_000)
+    offs = np.array([1000, 1000, 5000, 70_000, 290_000], dtype=np.uint64)  # starts past 0
+    d = to_dev(keys)
+    pa.segmented_sort_u32(d, None, offs)
+    got = to_host(d, np.uint32)
+    np.testing.assert_array_equal(got[:1000], keys[:1000])        # outside: untouched
+    np.testing.assert_array_equal(got[290_000:], keys[290_000:])
+    _check(keys, offs, got)
+
+
+def test_segmented_equals_per_array_calls(pa, orc):
+    keys, offs = _segments(1239, 400, 64, 400_000)
+    d = to_dev(keys)
+    pa.segmented_sort_u32(d, dev_empty(keys.size, np.uint32), offs)
+    got = to_host(d, np.uint32)
+    single = to_dev(keys)
+    tmp = dev_empty(keys.size, np.uint32)
+    for i in range(len(offs) - 1):
+        a, b = int(offs[i]), int(offs[i + 1])
+        pa.sort_u32(single[a:b], tmp[a:b])
+    np.testing.assert_array_equal(got, to_host(single, np.uint32))
+    _check(keys, offs, got, orc, oracle_every=37)
+
+
+def test_segmented_rejects_decreasing_offsets(pa):
+    import parallelalgorithms_amd as P
+    d = dev_empty(10, np.uint32)
+    with pytest.raises(P.PAError):
+        pa.segmented_sort_u32(d, None, np.array([0, 5, 3], dtype=np.uint64))
