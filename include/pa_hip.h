@@ -161,6 +161,53 @@ int pa_merge_inplace_u64(uint64_t* t, size_t n1, size_t n2, pa_stream_t stream);
 int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets, size_t num_segments,
                           pa_stream_t stream);
 
+/* As pa_segmented_sort_u32 for segments that live in separate device
+ * buffers: segment i is keys[i][0 .. lens[i]) (keys and lens are host arrays;
+ * each keys[i] is device memory).  Used to batch independent requests. */
+int pa_segmented_sort_ptrs_u32(uint32_t* const* keys, const uint64_t* lens, size_t num_segments, pa_stream_t stream);
+
+/* ---- Serving: continuous batching of sort / select / histogram jobs --------- */
+/* No reference counterpart (the reference is a blocking call-per-array
+ * library; BundleSmallWorkItems.cpp:36-144 and MemoryUsage.cpp:20-86 hold
+ * the nearest ideas).  Jobs name device-resident arrays; results are what the
+ * same call returns alone. */
+
+enum pa_op { PA_OP_SORT = 0, PA_OP_SELECT = 1, PA_OP_HIST = 2 };
+enum pa_key_type { PA_KEY_U32 = 0, PA_KEY_U64 = 1 };
+
+typedef struct pa_job {
+  int op;              /* pa_op */
+  int key_type;        /* pa_key_type; select and histogram take PA_KEY_U32 */
+  void* keys;          /* device memory, n keys; sorted in place for PA_OP_SORT */
+  size_t n;
+  size_t k;            /* PA_OP_SELECT: rank */
+  unsigned shift, bits; /* PA_OP_HIST: digit (key >> shift) & (2^bits - 1) */
+  void* out;           /* device memory: uint32 (select) or 2^bits uint64 (hist) */
+} pa_job;
+
+typedef struct pa_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t launches;           /* units launched (a coalesced batch counts once) */
+  uint64_t coalesced_batches;  /* batches of two or more small sorts */
+  uint64_t coalesced_jobs;     /* jobs that ran inside such batches */
+  uint64_t peak_bytes_in_flight;
+  uint64_t pending;            /* jobs waiting for admission */
+} pa_server_stats;
+
+typedef struct pa_server pa_server;
+
+/* hbm_budget_bytes: device scratch admitted at once (0 = 64 GiB);
+ * num_streams: concurrent HIP streams; coalesce_max_keys: u32 sorts up to
+ * this size are batched together (0 = 1 Mi keys).  NULL on failure. */
+pa_server* pa_server_create(size_t hbm_budget_bytes, int num_streams, size_t coalesce_max_keys);
+int pa_server_submit(pa_server* server, const pa_job* job, uint64_t* ticket);
+/* Blocks until the job is done; returns its status and its latency
+ * (completion - submission, host steady clock).  Each ticket is waited once. */
+int pa_server_wait(pa_server* server, uint64_t ticket, double* latency_ms);
+int pa_server_drain(pa_server* server);
+int pa_server_get_stats(pa_server* server, pa_server_stats* out);
+void pa_server_destroy(pa_server* server);
+
 /* ---- Measurement helpers (no reference counterpart) ------------------------ */
 
 /* Plain device copy kernel, dst[i] = src[i]; the known-size calibration point

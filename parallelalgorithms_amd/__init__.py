@@ -49,7 +49,14 @@ _SIGS = {
     "pa_merge_inplace_u32": (_i, [_vp, _sz, _sz, _vp]),
     "pa_merge_inplace_u64": (_i, [_vp, _sz, _sz, _vp]),
     "pa_segmented_sort_u32": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "pa_segmented_sort_ptrs_u32": (_i, [_vp, _vp, _sz, _vp]),
     "pa_copy_u32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_server_create": (_vp, [_sz, _i, _sz]),
+    "pa_server_submit": (_i, [_vp, _vp, _vp]),
+    "pa_server_wait": (_i, [_vp, _u64, _vp]),
+    "pa_server_drain": (_i, [_vp]),
+    "pa_server_get_stats": (_i, [_vp, _vp]),
+    "pa_server_destroy": (None, [_vp]),
     "pa_profile_enable": (_i, [_i]),
     "pa_profile_query": (_i, [ctypes.c_char_p, _vp, _vp]),
     "pa_profile_reset": (_i, []),
@@ -206,6 +213,65 @@ def merge_inplace_u64(t, n1, stream=None):
 def segmented_sort_u32(keys, tmp, offsets, stream=None):
     """offsets: host numpy uint64 array of num_segments + 1 entries."""
     call("pa_segmented_sort_u32", _ptr(keys), _ptr(tmp), _ptr(offsets), len(offsets) - 1, _stream(stream))
+
+
+def segmented_sort_ptrs_u32(tensors, stream=None):
+    """Sorts each device array in `tensors` (a list) in one batch."""
+    import numpy as np
+    ptrs = np.array([_ptr(t) or 0 for t in tensors], dtype=np.uint64)
+    lens = np.array([numel(t) for t in tensors], dtype=np.uint64)
+    call("pa_segmented_sort_ptrs_u32", ptrs.ctypes.data, lens.ctypes.data, len(tensors), _stream(stream))
+
+
+OP_SORT, OP_SELECT, OP_HIST = 0, 1, 2
+KEY_U32, KEY_U64 = 0, 1
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int), ("key_type", ctypes.c_int), ("keys", ctypes.c_void_p), ("n", ctypes.c_size_t),
+                ("k", ctypes.c_size_t), ("shift", ctypes.c_uint), ("bits", ctypes.c_uint), ("out", ctypes.c_void_p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in ("submitted", "completed", "failed", "launches",
+                                                      "coalesced_batches", "coalesced_jobs",
+                                                      "peak_bytes_in_flight", "pending")]
+
+
+class Server:
+    """The continuous-batching server (pa_server_*): jobs on device arrays."""
+
+    def __init__(self, budget_bytes=64 << 30, streams=4, coalesce_max_keys=1 << 20):
+        self.lib = load()
+        self.h = self.lib.pa_server_create(budget_bytes, streams, coalesce_max_keys)
+        if not self.h:
+            raise PAError("pa_server_create", 3, self.lib.pa_last_error_string().decode())
+
+    def submit(self, op, keys, key_type=KEY_U32, k=0, shift=0, bits=8, out=None):
+        job = Job(op, key_type, _ptr(keys), numel(keys), k, shift, bits, _ptr(out))
+        t = ctypes.c_uint64(0)
+        call("pa_server_submit", self.h, ctypes.addressof(job), ctypes.addressof(t))
+        return t.value
+
+    def wait(self, ticket):
+        lat = ctypes.c_double(0.0)
+        st = self.lib.pa_server_wait(self.h, ticket, ctypes.addressof(lat))
+        if st != PA_OK:
+            raise PAError("pa_server_wait", st, self.lib.pa_last_error_string().decode())
+        return lat.value
+
+    def drain(self):
+        call("pa_server_drain", self.h)
+
+    def stats(self):
+        s = ServerStats()
+        call("pa_server_get_stats", self.h, ctypes.addressof(s))
+        return {name: getattr(s, name) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.pa_server_destroy(self.h)
+            self.h = None
 
 
 def copy_u32(dst, src, stream=None):

@@ -37,6 +37,12 @@ int arg_error(const char* msg);
 // device scratch owned by that stream.  Work on one stream runs in order, so
 // consecutive calls on the same stream can reuse the same scratch.
 // ---------------------------------------------------------------------------
+struct PinnedSlot {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipEvent_t done = nullptr;  // recorded after the copy that reads the slot
+};
+
 struct StreamCtx {
   std::mutex mu;
   hipStream_t stream = nullptr;
@@ -45,12 +51,18 @@ struct StreamCtx {
   size_t scratch_bytes = 0;
   void* pinned = nullptr;  // pinned host staging for small results
   size_t pinned_bytes = 0;
+  PinnedSlot ring[4];      // pinned upload buffers reused once their copy is done
 };
 
 StreamCtx& stream_ctx(hipStream_t s);
 // Ensures ctx.scratch holds >= bytes; on growth waits for the stream first.
 int ensure_scratch(StreamCtx& ctx, size_t bytes, void** out);
 int ensure_pinned(StreamCtx& ctx, size_t bytes, void** out);
+// A pinned upload buffer of >= bytes whose previous copy has finished (waits
+// only if all ring slots are still in flight).  After enqueuing the copy that
+// reads it, call release_pinned(ctx, slot) to record its completion event.
+int acquire_pinned(StreamCtx& ctx, size_t bytes, void** out, int* slot);
+int release_pinned(StreamCtx& ctx, int slot);
 
 // True when p is device-accessible memory (hipMalloc / managed / pinned).
 bool is_device_ptr(const void* p);

@@ -81,6 +81,38 @@ int ensure_pinned(StreamCtx& ctx, size_t bytes, void** out) {
   return PA_OK;
 }
 
+int acquire_pinned(StreamCtx& ctx, size_t bytes, void** out, int* slot) {
+  const int nslots = (int)(sizeof(ctx.ring) / sizeof(ctx.ring[0]));
+  int pick = -1;
+  for (int i = 0; i < nslots && pick < 0; ++i) {
+    PinnedSlot& r = ctx.ring[i];
+    if (!r.done || hipEventQuery(r.done) == hipSuccess) pick = i;
+  }
+  (void)hipGetLastError();  // hipEventQuery reports hipErrorNotReady for busy slots
+  if (pick < 0) {           // every slot in flight: wait for the oldest use
+    pick = 0;
+    PA_HIP(hipEventSynchronize(ctx.ring[0].done));
+  }
+  PinnedSlot& r = ctx.ring[pick];
+  if (!r.done) PA_HIP(hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
+  if (r.bytes < bytes) {
+    if (r.ptr) PA_HIP(hipHostFree(r.ptr));
+    r.ptr = nullptr;
+    r.bytes = 0;
+    const size_t want = align_up(bytes < 65536 ? 65536 : bytes, 4096);
+    PA_HIP(hipHostMalloc(&r.ptr, want, hipHostMallocDefault));
+    r.bytes = want;
+  }
+  *out = r.ptr;
+  *slot = pick;
+  return PA_OK;
+}
+
+int release_pinned(StreamCtx& ctx, int slot) {
+  PA_HIP(hipEventRecord(ctx.ring[slot].done, ctx.stream));
+  return PA_OK;
+}
+
 bool is_device_ptr(const void* p) {
   if (p == nullptr) return false;
   hipPointerAttribute_t attr;

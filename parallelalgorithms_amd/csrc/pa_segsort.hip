@@ -36,16 +36,16 @@ constexpr uint32_t kSegTile = kSegBlock * kSegItems;
 
 // Whole-segment LDS sort.  Padding slots hold 0xFFFFFFFF: a stable sort
 // keeps them after every real key, so the first len slots are the answer.
-__global__ __launch_bounds__(kSmallBlock) void k_seg_small(uint32_t* __restrict__ keys,
-                                                           const uint64_t* __restrict__ begin,
+__global__ __launch_bounds__(kSmallBlock) void k_seg_small(const uint64_t* __restrict__ ptr,
                                                            const uint32_t* __restrict__ len) {
   constexpr int WAVES = kSmallBlock / 64;
   constexpr int WAVE_SPAN = 64 * kSmallItems;
   __shared__ uint32_t s_keys[kSmallTile];
   __shared__ uint32_t s_whist[WAVES * 256];
+  __shared__ uint64_t s_match[WAVES * 256];
   __shared__ uint32_t s_wsum[WAVES];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint32_t* seg = keys + begin[blockIdx.x];
+  uint32_t* seg = reinterpret_cast<uint32_t*>(ptr[blockIdx.x]);
   const uint32_t n = len[blockIdx.x];
   uint32_t key[kSmallItems];
   const uint32_t wbase = wave * WAVE_SPAN + lane;
@@ -55,24 +55,26 @@ __global__ __launch_bounds__(kSmallBlock) void k_seg_small(uint32_t* __restrict_
     key[i] = idx < n ? seg[idx] : 0xFFFFFFFFu;
   }
   uint32_t* wh = s_whist + wave * 256;
+  uint64_t* wm = s_match + wave * 256;
+  for (uint32_t i = tid; i < WAVES * 256; i += kSmallBlock) s_match[i] = 0ull;
   for (uint32_t shift = 0; shift < 32; shift += 8) {
     for (uint32_t i = tid; i < WAVES * 256; i += kSmallBlock) s_whist[i] = 0;
     __syncthreads();
+    // Peers of one digit within the wave: each lane ORs its bit into the
+    // wave's LDS slot for its digit; the lowest peer clears the slot.
     uint32_t rank[kSmallItems];
 #pragma unroll
     for (int i = 0; i < kSmallItems; ++i) {
       const uint32_t d = (key[i] >> shift) & 0xFFu;
-      uint64_t peers = ~0ull;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
-      }
+      atomicOr(reinterpret_cast<unsigned long long*>(&wm[d]), 1ull << lane);
+      const uint64_t peers = wm[d];
       const uint32_t below = dev::count_below(peers);
       const uint32_t base = wh[d];
       rank[i] = base + below;
-      if (below == 0) wh[d] = base + (uint32_t)__popcll(peers);
+      if (below == 0) {
+        wh[d] = base + (uint32_t)__popcll(peers);
+        wm[d] = 0ull;
+      }
     }
     __syncthreads();
     // Digit totals, wave prefixes and the block-wide digit offsets.
@@ -111,9 +113,8 @@ __global__ __launch_bounds__(256) void k_seg_prepare(uint32_t* __restrict__ coun
 
 // One tile per block: 4 digit histograms of the tile's keys into its
 // segment's counts; also clears the look-back table.
-__global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint32_t* __restrict__ keys,
-                                                        const uint64_t* __restrict__ tile_info,
-                                                        const uint64_t* __restrict__ seg_begin,
+__global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restrict__ tile_info,
+                                                        const uint64_t* __restrict__ seg_in,
                                                         const uint32_t* __restrict__ seg_len,
                                                         uint32_t* __restrict__ counts, uint4* __restrict__ zero_ptr,
                                                         size_t zero_vec) {
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint32_t* __restri
   __syncthreads();
   const uint64_t info = tile_info[blockIdx.x];
   const uint32_t seg = (uint32_t)(info >> 32), local = (uint32_t)info;
-  const uint32_t* k = keys + seg_begin[seg];
+  const uint32_t* k = reinterpret_cast<const uint32_t*>(seg_in[seg]);
   const uint32_t n = seg_len[seg];
   const uint32_t lo = local * kSegTile, hi = min(lo + kSegTile, n);
   for (uint32_t idx = lo + tid; idx < hi; idx += kSegBlock) {
@@ -141,6 +142,110 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint32_t* __restri
     if (s_hist[i]) atomicAdd(&counts[(size_t)seg * 1024 + i], s_hist[i]);
 }
 
+// Sorts the segments described by (address, length) lists; large segments get
+// ping-pong space from `work` (their tmp addresses).  ctx.mu must be held.
+struct SegPlan {
+  std::vector<uint64_t> small_ptr, large_in, large_tmp, tile_info;
+  std::vector<uint32_t> small_len, large_len;
+  void add(uint64_t ptr, uint64_t len, uint64_t tmp_ptr) {
+    if (len <= 1) return;
+    if (len <= kSmallTile) {
+      small_ptr.push_back(ptr);
+      small_len.push_back((uint32_t)len);
+      return;
+    }
+    const uint64_t seg = large_in.size();
+    large_in.push_back(ptr);
+    large_tmp.push_back(tmp_ptr);
+    large_len.push_back((uint32_t)len);
+    const uint32_t tiles = (uint32_t)((len + kSegTile - 1) / kSegTile);
+    for (uint32_t t = 0; t < tiles; ++t) tile_info.push_back((seg << 32) | t);
+  }
+};
+
+static int run_plan(StreamCtx& ctx, const SegPlan& P, char* dev_base, size_t o_tab, size_t o_cnt, size_t o_ctr,
+                    size_t o_lb, hipStream_t s) {
+  const size_t ns = P.small_ptr.size(), nl = P.large_in.size(), nt = P.tile_info.size();
+  Layout T;
+  const size_t t_sp = T.take(ns * 8), t_sl = T.take(ns * 4), t_li = T.take(nl * 8), t_lt = T.take(nl * 8),
+               t_ll = T.take(nl * 4), t_ti = T.take(nt * 8);
+  const size_t table_bytes = T.off;
+  void* pinned;
+  int slot;
+  PA_TRY(acquire_pinned(ctx, table_bytes, &pinned, &slot));
+  char* pin = (char*)pinned;
+  std::copy(P.small_ptr.begin(), P.small_ptr.end(), (uint64_t*)(pin + t_sp));
+  std::copy(P.small_len.begin(), P.small_len.end(), (uint32_t*)(pin + t_sl));
+  std::copy(P.large_in.begin(), P.large_in.end(), (uint64_t*)(pin + t_li));
+  std::copy(P.large_tmp.begin(), P.large_tmp.end(), (uint64_t*)(pin + t_lt));
+  std::copy(P.large_len.begin(), P.large_len.end(), (uint32_t*)(pin + t_ll));
+  std::copy(P.tile_info.begin(), P.tile_info.end(), (uint64_t*)(pin + t_ti));
+  char* tab = dev_base + o_tab;
+  PA_HIP(hipMemcpyAsync(tab, pin, table_bytes, hipMemcpyHostToDevice, s));
+  PA_TRY(release_pinned(ctx, slot));
+  const uint64_t* d_sp = (const uint64_t*)(tab + t_sp);
+  const uint32_t* d_sl = (const uint32_t*)(tab + t_sl);
+  const uint64_t* d_li = (const uint64_t*)(tab + t_li);
+  const uint64_t* d_lt = (const uint64_t*)(tab + t_lt);
+  const uint32_t* d_ll = (const uint32_t*)(tab + t_ll);
+  const uint64_t* d_ti = (const uint64_t*)(tab + t_ti);
+  uint32_t* counts = (uint32_t*)(dev_base + o_cnt);
+  uint32_t* ctr = (uint32_t*)(dev_base + o_ctr);
+  uint32_t* lbp = (uint32_t*)(dev_base + o_lb);
+  const size_t lb_bytes = (size_t)4 * nt * 256 * 4;
+
+  if (ns) {
+    ProfScope ps("k_seg_small", s);
+    hipLaunchKernelGGL(k_seg_small, dim3((unsigned)ns), dim3(kSmallBlock), 0, s, d_sp, d_sl);
+  }
+  if (nl) {
+    hipLaunchKernelGGL(k_seg_prepare, dim3((unsigned)std::min<size_t>((nl * 1024 + 255) / 256, 4096)), dim3(256), 0, s,
+                       counts, nl * 1024, ctr);
+    {
+      ProfScope ps("k_seg_hist", s);
+      hipLaunchKernelGGL(k_seg_hist, dim3((unsigned)nt), dim3(kSegBlock), 0, s, d_ti, d_li, d_ll, counts, (uint4*)lbp,
+                         lb_bytes / 16);
+    }
+    for (int p = 0; p < 4; ++p) {
+      const uint64_t* tin = (p & 1) ? d_lt : d_li;
+      const uint64_t* tout = (p & 1) ? d_li : d_lt;
+      ProfScope ps("k_onesweep_seg", s);
+      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, true>), dim3((unsigned)nt),
+                         dim3(kSegBlock), 0, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)nullptr,
+                         (uint32_t*)nullptr, 0u, counts + p * 256, lbp + (size_t)p * nt * 256, ctr + p,
+                         (uint32_t)(8 * p), 0u, 1u, 0u, d_ti, tin, tout, d_ll, 4u);
+    }
+  }
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+// Scratch layout for a plan: tables, per-segment counts, tile counters,
+// look-back table, and `extra` bytes of ping-pong space.
+static int run_plan_alloc(const SegPlan& P, size_t extra, hipStream_t s, StreamCtx& ctx, char** work_out,
+                          bool fill_tmp, SegPlan* fix) {
+  const size_t ns = P.small_ptr.size(), nl = P.large_in.size(), nt = P.tile_info.size();
+  if (nt >= (1ull << 32)) return arg_error("segmented sort: too many tiles");
+  Layout L;
+  const size_t o_tab = L.take(ns * 12 + nl * 20 + nt * 8 + 6 * 256);
+  const size_t o_cnt = L.take(nl * 1024 * 4);
+  const size_t o_ctr = L.take(16);
+  const size_t o_lb = L.take((size_t)4 * nt * 256 * 4);
+  const size_t o_work = L.take(extra);
+  void* base;
+  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  char* b = (char*)base;
+  *work_out = b + o_work;
+  if (fill_tmp) {  // large segments without a caller tmp get consecutive work space
+    uint64_t off = 0;
+    for (size_t i = 0; i < fix->large_tmp.size(); ++i) {
+      fix->large_tmp[i] = (uint64_t)(uintptr_t)(b + o_work) + off * 4;
+      off += fix->large_len[i];
+    }
+  }
+  return run_plan(ctx, *fix, b, o_tab, o_cnt, o_ctr, o_lb, s);
+}
+
 }  // namespace pa
 
 using namespace pa;
@@ -153,100 +258,51 @@ int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets
   if (!offsets) return arg_error("pa_segmented_sort_u32: null offsets");
   for (size_t i = 0; i < num_segments; ++i)
     if (offsets[i + 1] < offsets[i]) return arg_error("pa_segmented_sort_u32: offsets must be non-decreasing");
-  const uint64_t lo = offsets[0], hi = offsets[num_segments];
-  if (hi == lo) return PA_OK;
+  if (offsets[num_segments] == offsets[0]) return PA_OK;
   if (!keys) return arg_error("pa_segmented_sort_u32: null keys");
   hipStream_t s = (hipStream_t)stream;
   if (!is_device_ptr(keys) || (tmp && !is_device_ptr(tmp)))
     return arg_error("pa_segmented_sort_u32: keys and tmp must be device memory");
-
-  // Host-side plan: size classes and the tile table of the large segments.
-  std::vector<uint64_t> small_begin, large_begin, tile_info;
-  std::vector<uint32_t> small_len, large_len;
+  SegPlan P;
+  uint64_t large_total = 0;
   for (size_t i = 0; i < num_segments; ++i) {
     const uint64_t len = offsets[i + 1] - offsets[i];
-    if (len <= 1) continue;
     if (len >= (1ull << 30)) {
       set_last_error("pa_segmented_sort_u32: a segment holds 2^30 or more keys");
       return PA_ERR_TOO_LARGE;
     }
-    if (len <= kSmallTile) {
-      small_begin.push_back(offsets[i]);
-      small_len.push_back((uint32_t)len);
-    } else {
-      const uint64_t seg = large_begin.size();
-      large_begin.push_back(offsets[i]);
-      large_len.push_back((uint32_t)len);
-      const uint32_t tiles = (uint32_t)((len + kSegTile - 1) / kSegTile);
-      for (uint32_t t = 0; t < tiles; ++t) tile_info.push_back((seg << 32) | t);
-    }
+    P.add((uint64_t)(uintptr_t)(keys + offsets[i]), len, tmp ? (uint64_t)(uintptr_t)(tmp + offsets[i]) : 0);
+    if (len > kSmallTile) large_total += len;
   }
-  const size_t ns = small_begin.size(), nl = large_begin.size(), nt = tile_info.size();
-  if (nt >= (1ull << 32)) return arg_error("pa_segmented_sort_u32: too many tiles");
-
   StreamCtx& ctx = stream_ctx(s);
   std::lock_guard<std::mutex> lk(ctx.mu);
-  // Tables go through pinned staging so the upload stays asynchronous.
-  Layout T;
-  const size_t t_sb = T.take(ns * 8), t_sl = T.take(ns * 4), t_lb = T.take(nl * 8), t_ll = T.take(nl * 4),
-               t_ti = T.take(nt * 8);
-  const size_t table_bytes = T.off;
-  Layout L;
-  const size_t o_tab = L.take(table_bytes);
-  const size_t o_cnt = L.take(nl * 1024 * 4);
-  const size_t o_ctr = L.take(16);
-  const size_t lb_bytes = (size_t)4 * nt * 256 * 4;
-  const size_t o_lb = L.take(lb_bytes);
-  const size_t o_tmp = tmp ? 0 : L.take((size_t)hi * 4);
-  void* base;
-  PA_TRY(ensure_scratch(ctx, L.off, &base));
-  void* pinned;
-  PA_HIP(hipStreamSynchronize(s));  // the pinned staging may still feed an earlier call
-  PA_TRY(ensure_pinned(ctx, table_bytes, &pinned));
-  char* pin = (char*)pinned;
-  std::copy(small_begin.begin(), small_begin.end(), (uint64_t*)(pin + t_sb));
-  std::copy(small_len.begin(), small_len.end(), (uint32_t*)(pin + t_sl));
-  std::copy(large_begin.begin(), large_begin.end(), (uint64_t*)(pin + t_lb));
-  std::copy(large_len.begin(), large_len.end(), (uint32_t*)(pin + t_ll));
-  std::copy(tile_info.begin(), tile_info.end(), (uint64_t*)(pin + t_ti));
-  char* b = (char*)base;
-  char* tab = b + o_tab;
-  PA_HIP(hipMemcpyAsync(tab, pin, table_bytes, hipMemcpyHostToDevice, s));
-  const uint64_t* d_sb = (const uint64_t*)(tab + t_sb);
-  const uint32_t* d_sl = (const uint32_t*)(tab + t_sl);
-  const uint64_t* d_lb = (const uint64_t*)(tab + t_lb);
-  const uint32_t* d_ll = (const uint32_t*)(tab + t_ll);
-  const uint64_t* d_ti = (const uint64_t*)(tab + t_ti);
-  uint32_t* counts = (uint32_t*)(b + o_cnt);
-  uint32_t* ctr = (uint32_t*)(b + o_ctr);
-  uint32_t* lbp = (uint32_t*)(b + o_lb);
-  uint32_t* work = tmp ? tmp : (uint32_t*)(b + o_tmp);
+  char* work;
+  return run_plan_alloc(P, tmp ? 0 : large_total * 4, s, ctx, &work, tmp == nullptr, &P);
+}
 
-  if (ns) {
-    ProfScope ps("k_seg_small", s);
-    hipLaunchKernelGGL(k_seg_small, dim3((unsigned)ns), dim3(kSmallBlock), 0, s, keys, d_sb, d_sl);
-  }
-  if (nl) {
-    hipLaunchKernelGGL(k_seg_prepare, dim3((unsigned)std::min<size_t>((nl * 1024 + 255) / 256, 4096)), dim3(256), 0, s,
-                       counts, nl * 1024, ctr);
-    {
-      ProfScope ps("k_seg_hist", s);
-      hipLaunchKernelGGL(k_seg_hist, dim3((unsigned)nt), dim3(kSegBlock), 0, s, keys, d_ti, d_lb, d_ll, counts,
-                         (uint4*)lbp, lb_bytes / 16);
+int pa_segmented_sort_ptrs_u32(uint32_t* const* keys, const uint64_t* lens, size_t num_segments, pa_stream_t stream) {
+  if (num_segments == 0) return PA_OK;
+  if (!keys || !lens) return arg_error("pa_segmented_sort_ptrs_u32: null table");
+  hipStream_t s = (hipStream_t)stream;
+  SegPlan P;
+  uint64_t large_total = 0;
+  for (size_t i = 0; i < num_segments; ++i) {
+    if (lens[i] == 0) continue;
+    if (!keys[i]) return arg_error("pa_segmented_sort_ptrs_u32: null segment pointer");
+    if (lens[i] >= (1ull << 30)) {
+      set_last_error("pa_segmented_sort_ptrs_u32: a segment holds 2^30 or more keys");
+      return PA_ERR_TOO_LARGE;
     }
-    uint32_t* kin = keys;
-    uint32_t* kout = work;
-    for (int p = 0; p < 4; ++p) {
-      ProfScope ps("k_onesweep_seg", s);
-      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, true>), dim3((unsigned)nt),
-                         dim3(kSegBlock), 0, s, kin, kout, (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
-                         counts + p * 256, lbp + (size_t)p * nt * 256, ctr + p, (uint32_t)(8 * p), 0u, d_ti, d_lb,
-                         d_ll, 4u);
-      std::swap(kin, kout);
-    }
+    P.add((uint64_t)(uintptr_t)keys[i], lens[i], 0);
+    if (lens[i] > kSmallTile) large_total += lens[i];
   }
-  PA_HIP(hipGetLastError());
-  return PA_OK;
+  for (size_t i = 0; i < num_segments; ++i)
+    if (lens[i] > 1 && !is_device_ptr(keys[i]))
+      return arg_error("pa_segmented_sort_ptrs_u32: segments must be device memory");
+  StreamCtx& ctx = stream_ctx(s);
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  char* work;
+  return run_plan_alloc(P, large_total * 4, s, ctx, &work, true, &P);
 }
 
 }  // extern "C"

@@ -77,3 +77,14 @@ def test_segmented_rejects_decreasing_offsets(pa):
     d = dev_empty(10, np.uint32)
     with pytest.raises(P.PAError):
         pa.segmented_sort_u32(d, None, np.array([0, 5, 3], dtype=np.uint64))
+
+
+def test_segmented_pointer_table(pa):
+    """Separate device buffers sorted in one batch (the server's coalescing path)."""
+    import parallelalgorithms_amd as P
+    lens = [0, 1, 5, 4096, 4097, 70_001, 3, 250_000]
+    hosts = [P.gen_u32(90 + i, m) for i, m in enumerate(lens)]
+    devs = [to_dev(h) if h.size else dev_empty(1, np.uint32)[:0] for h in hosts]
+    pa.segmented_sort_ptrs_u32(devs)
+    for h, d in zip(hosts, devs):
+        np.testing.assert_array_equal(to_host(d, np.uint32) if h.size else h, np.sort(h))
