@@ -68,6 +68,137 @@ def cpu_baseline(keys_np, iters=3):
     }
 
 
+def log_uniform(u64, lo, hi):
+    import numpy as np
+    u = (u64 >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    return np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.uint64)
+
+
+def bench_segmented(pa, torch, count=10_000, seed=1239):
+    """10,000 u32 arrays, lengths log-uniform in [64, 4,000,000] (mt19937_64 seed
+    1239), keys uniform, 5% presorted: one segmented batch vs per-array calls."""
+    import numpy as np
+    r = pa.gen_u64(seed, 2 * count)
+    lens = log_uniform(r[:count], 64, 4_000_000)
+    presorted = ((r[count:] >> np.uint64(11)).astype(np.float64) / float(1 << 53)) < 0.05
+    offs = np.zeros(count + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    total = int(offs[-1])
+    pristine = torch.empty(total, dtype=torch.int32, device="cuda")
+    pa.fill_random_u32(pristine, seed)
+    pre = [pristine[int(offs[i]):int(offs[i + 1])] for i in np.nonzero(presorted)[0]]
+    pa.segmented_sort_ptrs_u32(pre)  # the presorted 5%
+    tmp = torch.empty_like(pristine)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # one batch
+    work = pristine.clone()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    pa.segmented_sort_u32(work, tmp, offs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_batch = e0.elapsed_time(e1)
+    # plain per-array calls, back to back on the same stream
+    work2 = pristine.clone()
+    lib = pa.load()
+    base, tbase = work2.data_ptr(), tmp.data_ptr()
+    sptr = stream.cuda_stream
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for i in range(count):
+        a = int(offs[i])
+        st = lib.pa_sort_u32(base + 4 * a, tbase + 4 * a, int(lens[i]), sptr)
+        if st:
+            raise RuntimeError("pa_sort_u32 failed in the per-array baseline")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_single = e0.elapsed_time(e1)
+    same = bool(torch.equal(work, work2))
+    del work, work2, tmp, pristine
+    torch.cuda.empty_cache()
+    return {"workload": f"{count} u32 arrays, lengths log-uniform [64, 4e6], 5% presorted", "keys": total,
+            "ms_batched": round(ms_batch, 3), "ms_per_array_calls": round(ms_single, 3),
+            "gkeys_per_s_batched": round(total / ms_batch / 1e6, 2),
+            "gkeys_per_s_per_array": round(total / ms_single / 1e6, 2),
+            "speedup": round(ms_single / ms_batch, 2), "bit_exact_vs_per_array": same}
+
+
+def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
+    """Served stream: jobs 70% sort / 20% select (k log-uniform 1..n/10) / 10%
+    256-bin histogram, sizes log-uniform 4K..64M, sort keys u32 or u64 (50/50),
+    Poisson arrivals at fractions of the measured saturation rate, 64 GB
+    admission budget.  Latency = completion - submission."""
+    import time
+    import numpy as np
+    r = pa.gen_u64(seed, 4 * jobs)
+    sizes = log_uniform(r[:jobs], 4096, 64 << 20)
+    kind_u = r[jobs:2 * jobs] % np.uint64(100)
+    wide = (r[2 * jobs:3 * jobs] & np.uint64(1)).astype(bool)
+    spec = []
+    for i in range(jobs):
+        n = int(sizes[i])
+        if kind_u[i] < 70:
+            op, kt = pa.OP_SORT, (pa.KEY_U64 if wide[i] else pa.KEY_U32)
+        elif kind_u[i] < 90:
+            op, kt = pa.OP_SELECT, pa.KEY_U32
+        else:
+            op, kt = pa.OP_HIST, pa.KEY_U32
+        u = float(r[3 * jobs + i] >> np.uint64(11)) / float(1 << 53)
+        k = int(np.floor(np.exp(u * np.log(max(2.0, n / 10.0))))) - 1
+        spec.append((op, kt, n, min(max(k, 0), n - 1)))
+
+    def make_inputs(tag):
+        arrs, outs = [], []
+        for i, (op, kt, n, k) in enumerate(spec):
+            t = torch.empty(n, dtype=torch.int64 if kt == pa.KEY_U64 else torch.int32, device="cuda")
+            (pa.fill_random_u64 if kt == pa.KEY_U64 else pa.fill_random_u32)(t, seed * 1000 + i)
+            arrs.append(t)
+            outs.append(torch.empty(256, dtype=torch.int64, device="cuda") if op != pa.OP_SORT else None)
+        torch.cuda.synchronize()
+        return arrs, outs
+
+    def run(arrivals):
+        arrs, outs = make_inputs(0)
+        srv = pa.Server(budget_bytes=64 << 30, streams=4, coalesce_max_keys=1 << 20)
+        try:
+            tickets = []
+            t0 = time.perf_counter()
+            for i, (op, kt, n, k) in enumerate(spec):
+                if arrivals is not None:
+                    due = t0 + arrivals[i]
+                    if due - time.perf_counter() > 2e-3:
+                        time.sleep(due - time.perf_counter() - 1e-3)
+                    while time.perf_counter() < due:
+                        pass
+                tickets.append(srv.submit(op, arrs[i], kt, k=k, shift=24, bits=8, out=outs[i]))
+            lat = [srv.wait(t) for t in tickets]
+            span = time.perf_counter() - t0
+            st = srv.stats()
+        finally:
+            srv.close()
+        del arrs, outs
+        torch.cuda.empty_cache()
+        return np.array(lat), span, st
+
+    keys_total = float(sum(n for _, _, n, _ in spec))
+    run(None)  # warm-up (allocations, code objects)
+    lat, span, st = run(None)
+    sat = jobs / span
+    levels = []
+    for i, load in enumerate(loads):
+        rate = load * sat
+        gaps = -np.log1p(-((pa.gen_u64(seed + 10 + i, jobs) >> np.uint64(11)).astype(np.float64) / float(1 << 53))) / rate
+        arrivals = np.cumsum(gaps) - gaps[0]
+        lat, span, st = run(arrivals)
+        levels.append({"load": load, "offered_jobs_per_s": round(rate, 1), "achieved_jobs_per_s": round(jobs / span, 1),
+                       "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                       "gkeys_per_s": round(keys_total / span / 1e9, 2), "coalesced_jobs": st["coalesced_jobs"]})
+    return {"workload": f"{jobs} jobs: 70% sort (u32/u64), 20% select, 10% hist; sizes log-uniform 4K..64M",
+            "saturation_jobs_per_s": round(sat, 1), "levels": levels,
+            "note": "select and histogram jobs use u32 keys"}
+
+
 def load_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
     if not os.path.exists(path):
@@ -88,6 +219,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--no-seg", action="store_true", help="skip the segmented-batch leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -170,7 +303,16 @@ def main():
                         "traffic": load_traffic("k_onesweep", f"{n // 1_000_000}M_u32"),
                         "kernel": "k_onesweep", "algorithmic_bytes_per_launch": int(alg)}
         pa.profile_reset()
-    del bufs
+    del bufs, src, tmp
+    torch.cuda.empty_cache()
+
+    seg = serve = None
+    if not args.no_seg:
+        print("[bench] segmented batch vs per-array calls", file=sys.stderr, flush=True)
+        seg = bench_segmented(pa, torch)
+    if not args.no_serve:
+        print("[bench] served stream", file=sys.stderr, flush=True)
+        serve = bench_serve(pa, torch)
 
     cpu = None
     if not args.no_cpu:
@@ -198,6 +340,8 @@ def main():
         "whole_sort_hbm_frac": round(whole_sort_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "copy_kernel_GBps": round(copy_gbs, 1),
         "kernels": kernels,
+        "segmented": seg,
+        "serve": serve,
         "sorted_ok": ok,
         "baseline_ref": "README.md:9 SortRadixPar 100M u32 random, 48-core AWS C7a: 2865 Mkeys/s",
     }

@@ -229,6 +229,10 @@ int pa_profile_reset(void);
 void pa_gen_mt19937_u32(uint64_t seed, uint32_t* out, size_t n);
 /* Full 64-bit std::mt19937_64(seed) outputs. */
 void pa_gen_mt19937_u64(uint64_t seed, uint64_t* out, size_t n);
+/* Device-side uniform keys for large synthetic workloads: dst[i] =
+ * splitmix64(seed + i * 0x9E3779B97F4A7C15) (truncated for u32). */
+int pa_fill_random_u32(uint32_t* dst, size_t n, uint64_t seed, pa_stream_t stream);
+int pa_fill_random_u64(uint64_t* dst, size_t n, uint64_t seed, pa_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,8 @@ _SIGS = {
     "pa_profile_reset": (_i, []),
     "pa_gen_mt19937_u32": (None, [_u64, _vp, _sz]),
     "pa_gen_mt19937_u64": (None, [_u64, _vp, _sz]),
+    "pa_fill_random_u32": (_i, [_vp, _sz, _u64, _vp]),
+    "pa_fill_random_u64": (_i, [_vp, _sz, _u64, _vp]),
 }
 
 _lib = None
@@ -296,6 +298,14 @@ def profile_query(prefix):
     cnt = ctypes.c_uint64(0)
     call("pa_profile_query", prefix.encode(), ctypes.addressof(tot), ctypes.addressof(cnt))
     return tot.value, cnt.value
+
+
+def fill_random_u32(dst, seed, stream=None):
+    call("pa_fill_random_u32", _ptr(dst), numel(dst), seed, _stream(stream))
+
+
+def fill_random_u64(dst, seed, stream=None):
+    call("pa_fill_random_u64", _ptr(dst), numel(dst), seed, _stream(stream))
 
 
 def gen_u32(seed, n):

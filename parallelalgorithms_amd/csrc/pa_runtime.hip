@@ -1,5 +1,6 @@
 // pa_runtime.hip -- host runtime of libpa_hip.so: errors, per-stream scratch,
 // pointer classification, kernel timing hooks, copy kernel, generators.
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -166,6 +167,22 @@ __global__ __launch_bounds__(256) void k_copy_u32(uint32_t* __restrict__ dst, co
     for (size_t i = nvec * 4 + threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
+// Counter-based generator: x_i = splitmix64(seed + i * golden), one output
+// per index, so any slice can be generated independently.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_random(T* __restrict__ dst, size_t n, uint64_t seed) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = (T)splitmix64(seed + i * 0x9E3779B97F4A7C15ull);
+}
+
 }  // namespace pa
 
 using namespace pa;
@@ -242,6 +259,24 @@ int pa_profile_reset(void) {
     (void)hipEventDestroy(r.stop);
   }
   g_prof.clear();
+  return PA_OK;
+}
+
+int pa_fill_random_u32(uint32_t* dst, size_t n, uint64_t seed, pa_stream_t stream) {
+  if (n == 0) return PA_OK;
+  if (!dst || !is_device_ptr(dst)) return arg_error("pa_fill_random_u32: dst must be device memory");
+  hipLaunchKernelGGL(k_fill_random<uint32_t>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0,
+                     (hipStream_t)stream, dst, n, seed);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+int pa_fill_random_u64(uint64_t* dst, size_t n, uint64_t seed, pa_stream_t stream) {
+  if (n == 0) return PA_OK;
+  if (!dst || !is_device_ptr(dst)) return arg_error("pa_fill_random_u64: dst must be device memory");
+  hipLaunchKernelGGL(k_fill_random<uint64_t>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0,
+                     (hipStream_t)stream, dst, n, seed);
+  PA_HIP(hipGetLastError());
   return PA_OK;
 }
 

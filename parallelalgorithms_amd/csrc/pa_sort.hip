@@ -79,19 +79,19 @@ struct PassArgs {
 };
 
 struct PassVariant {
-  int block, items, lbw, rank, early, minw, chains;
+  int block, items, lbw, rank, early, minw, chains, coop;
   void (*launch)(unsigned grid, hipStream_t s, const PassArgs& a);
 };
 
-template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, int RANK, bool EARLY, int MINW>
+template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, int RANK, bool EARLY, int MINW, int COOP>
 void launch_pass(unsigned grid, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, false, RANK, EARLY, MINW>), dim3(grid), dim3(BLOCK), 0, s,
+  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, false, RANK, EARLY, MINW, COOP>), dim3(grid), dim3(BLOCK), 0, s,
                      static_cast<const K*>(a.kin), static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr,
                      a.shift, a.fmode, a.nchains, a.tpc, nullptr, nullptr, nullptr, nullptr, 0u);
 }
 
-#define PA_VARIANT(K, KV, B, I, W, R, E, M, C) \
-  { B, I, W, R, E, M, C, &launch_pass<K, KV, B, I, W, R, E, M> }
+#define PA_VARIANT(K, KV, B, I, W, R, E, M, C, P) \
+  { B, I, W, R, E, M, C, P, &launch_pass<K, KV, B, I, W, R, E, M, P> }
 
 // Variant 0 of each table is the default; PA_SORT_VARIANT picks another.
 template <typename K, bool KV>
@@ -99,24 +99,26 @@ struct Variants;
 template <>
 struct Variants<uint32_t, false> {
   static constexpr PassVariant table[] = {
-      PA_VARIANT(uint32_t, false, 512, 16, 8, 1, false, 1, 1),  PA_VARIANT(uint32_t, false, 512, 24, 8, 1, false, 1, 1),
-      PA_VARIANT(uint32_t, false, 512, 24, 8, 1, true, 1, 1),   PA_VARIANT(uint32_t, false, 256, 16, 8, 1, false, 1, 1),
-      PA_VARIANT(uint32_t, false, 256, 16, 8, 1, true, 8, 1),   PA_VARIANT(uint32_t, false, 512, 16, 8, 1, true, 6, 1),
-      PA_VARIANT(uint32_t, false, 512, 16, 16, 1, true, 1, 1),  PA_VARIANT(uint32_t, false, 256, 24, 8, 1, false, 1, 1),
+      PA_VARIANT(uint32_t, false, 512, 24, 8, 1, false, 1, 1, 0), PA_VARIANT(uint32_t, false, 512, 16, 8, 1, false, 1, 1, 0),
+      PA_VARIANT(uint32_t, false, 256, 32, 8, 1, false, 1, 1, 0), PA_VARIANT(uint32_t, false, 512, 16, 8, 1, false, 1, 1, 32),
+      PA_VARIANT(uint32_t, false, 512, 16, 8, 1, true, 1, 1, 0),
   };
 };
 template <>
 struct Variants<uint32_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 8, 1, false, 1, 1)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 8, 1, false, 1, 1, 0),
+                                          PA_VARIANT(uint32_t, true, 512, 16, 8, 1, false, 1, 1, 32)};
 };
 template <>
 struct Variants<uint64_t, false> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 8, 1, false, 1, 1),
-                                          PA_VARIANT(uint64_t, false, 512, 12, 8, 1, false, 1, 1)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 8, 1, false, 1, 1, 0),
+                                          PA_VARIANT(uint64_t, false, 512, 12, 8, 1, false, 1, 1, 0),
+                                          PA_VARIANT(uint64_t, false, 512, 8, 8, 1, false, 1, 1, 32)};
 };
 template <>
 struct Variants<uint64_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 8, 1, false, 1, 1)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 8, 1, false, 1, 1, 0),
+                                          PA_VARIANT(uint64_t, true, 512, 8, 8, 1, false, 1, 1, 32)};
 };
 
 template <typename K, bool KV>

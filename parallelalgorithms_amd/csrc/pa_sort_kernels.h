@@ -138,8 +138,12 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 // the look-back stops at the segment's first tile.  Tiles never span segments.
 // MINW: minimum waves per SIMD the register allocation must allow (1 = no
 // constraint); 6 keeps a 512-thread block at 3 resident blocks per CU.
+// COOP > 0: before the per-digit look-back, the whole block loads the status
+// words of the COOP preceding tiles (all 256 digits) into LDS in one round
+// trip; each digit then walks that window in LDS and polls memory only if the
+// window holds no inclusive prefix yet.
 template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool SEG = false, int RANK = 1, bool EARLY = false,
-          int MINW = 1>
+          int MINW = 1, int COOP = 0>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void k_onesweep(const K* __restrict__ kin, K* __restrict__ kout,
                                                     const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                     uint32_t n, const uint32_t* __restrict__ gtab,
@@ -322,6 +326,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       s_wsum[1][wave] = ginc;
     }
   }
+  // Cooperative look-back window: status words of tiles tile-1 .. tile-COOP
+  // for every digit, win[j][d] (the key staging area is idle until the
+  // scatter below).  Words before the chain's first tile read as "inclusive 0".
+  uint32_t* const win = reinterpret_cast<uint32_t*>(s_keys);
+  if constexpr (COOP > 0) {
+    static_assert(COOP * 256 * 4 <= TILE * (int)sizeof(K), "window must fit the key staging area");
+    static_assert((COOP * 256) % BLOCK == 0, "whole window rows per thread");
+    if (local > 0) {
+      uint32_t w[COOP * 256 / BLOCK];
+#pragma unroll
+      for (int r = 0; r < COOP * 256 / BLOCK; ++r) {
+        const uint32_t q = tid + r * BLOCK;
+        const int64_t pt = (int64_t)tile - 1 - (int64_t)(q >> 8);
+        w[r] = pt >= (int64_t)first_tile ? dev::atomic_load_agent(&lb[(size_t)pt * 256 + (q & 255)]) : kLbInc;
+      }
+#pragma unroll
+      for (int r = 0; r < COOP * 256 / BLOCK; ++r) win[tid + r * BLOCK] = w[r];
+    }
+  }
   __syncthreads();
   if (tid < 256) {
     uint32_t cpre = 0, gpre = 0;
@@ -340,7 +363,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     if (local > 0) {
       int64_t p = (int64_t)tile - 1;
       bool resolved = false;
-      if constexpr (EARLY) {
+      if constexpr (COOP > 0) {
+        for (int j = 0; j < COOP && !resolved; ++j) {
+          const uint32_t wv = win[j * 256 + tid];
+          const uint32_t flag = wv & ~kLbMask;
+          if (flag == 0) break;  // not published when the window was read
+          excl += wv & kLbMask;
+          resolved = flag == kLbInc;
+          --p;
+        }
+      } else if constexpr (EARLY) {
         const uint32_t flag = prefetched & ~kLbMask;
         if (flag != 0) {
           excl += prefetched & kLbMask;
