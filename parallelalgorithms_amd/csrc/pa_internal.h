@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "../../include/pa_hip.h"
 

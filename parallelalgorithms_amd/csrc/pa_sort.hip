@@ -37,32 +37,24 @@ __global__ __launch_bounds__(256) void k_sort_prepare(uint32_t* __restrict__ cou
   for (uint32_t i = threadIdx.x; i < ncounters; i += blockDim.x) counters[i] = 0;
 }
 
-// Per pass p (one block each): base[p][c][d] = sum_{d'<d} total[p][d'] +
-// sum_{c'<c} counts[p][c'][d], the first output slot of digit d's keys from
-// chain c.
-__global__ __launch_bounds__(256) void k_chain_bases(const uint32_t* __restrict__ counts, uint32_t nchains,
+// Per pass p (one block each): base[p][d] = sum_{d' < d} counts[p][d'], the
+// first output slot of digit d.
+__global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* __restrict__ counts,
                                                      uint32_t* __restrict__ base) {
   __shared__ uint32_t s_wsum[4];
   const uint32_t d = threadIdx.x, lane = d & 63, wave = d >> 6;
-  const uint32_t* c_in = counts + (size_t)blockIdx.x * nchains * 256;
-  uint32_t* b_out = base + (size_t)blockIdx.x * nchains * 256;
-  uint32_t run = 0;
-  for (uint32_t c = 0; c < nchains; ++c) {
-    b_out[c * 256 + d] = run;
-    run += c_in[c * 256 + d];
-  }
-  const uint32_t inc = dev::wave_incl_scan(run, lane);
+  const uint32_t c = counts[blockIdx.x * 256 + d];
+  const uint32_t inc = dev::wave_incl_scan(c, lane);
   if (lane == 63) s_wsum[wave] = inc;
   __syncthreads();
   uint32_t pre = 0;
   for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
-  const uint32_t start = pre + inc - run;
-  for (uint32_t c = 0; c < nchains; ++c) b_out[c * 256 + d] += start;
+  base[blockIdx.x * 256 + d] = pre + inc - c;
 }
 
 // ---------------------------------------------------------------------------
-// Pass variants (tile shape and look-back width), selectable at run time so
-// the tuner can pick one per key type; PA_SORT_VARIANT overrides the default.
+// Pass variants (tile shape, look-back width, persistent blocks), selectable
+// at run time; PA_SORT_VARIANT overrides the default (entry 0).
 // ---------------------------------------------------------------------------
 struct PassArgs {
   const void* kin;
@@ -73,52 +65,71 @@ struct PassArgs {
   const uint32_t* gtab;
   uint32_t* lb;
   uint32_t* ctr;
+  uint32_t ntiles;
   uint32_t shift;
   uint32_t fmode;
-  uint32_t nchains, tpc;
 };
 
 struct PassVariant {
-  int block, items, lbw, rank, early, minw, chains, coop;
-  void (*launch)(unsigned grid, hipStream_t s, const PassArgs& a);
+  int block, items, lbw, pipe, minw;
+  int (*launch)(hipStream_t s, const PassArgs& a);
 };
 
-template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, int RANK, bool EARLY, int MINW, int COOP>
-void launch_pass(unsigned grid, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, false, RANK, EARLY, MINW, COOP>), dim3(grid), dim3(BLOCK), 0, s,
-                     static_cast<const K*>(a.kin), static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr,
-                     a.shift, a.fmode, a.nchains, a.tpc, nullptr, nullptr, nullptr, nullptr, 0u);
+// Resident blocks of a kernel on the current device (persistent launches).
+template <typename F>
+unsigned resident_blocks(F kernel, int block) {
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
+  (void)hipGetLastError();
+  return (unsigned)std::max(1, cus * std::max(1, per_cu));
 }
 
-#define PA_VARIANT(K, KV, B, I, W, R, E, M, C, P) \
-  { B, I, W, R, E, M, C, P, &launch_pass<K, KV, B, I, W, R, E, M, P> }
+template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool PIPE, int MINW>
+int launch_pass(hipStream_t s, const PassArgs& a) {
+  auto kern = k_onesweep<K, KV, BLOCK, ITEMS, LBW, false, PIPE, MINW>;
+  unsigned grid = a.ntiles;
+  if (PIPE) {
+    static const unsigned resident = resident_blocks(kern, BLOCK);  // one device per process is assumed
+    grid = std::min(grid, resident);
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, static_cast<const K*>(a.kin), static_cast<K*>(a.kout), a.vin,
+                     a.vout, a.n, a.gtab, a.lb, a.ctr, a.ntiles, a.shift, a.fmode, nullptr, nullptr, nullptr, nullptr,
+                     0u);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
 
-// Variant 0 of each table is the default; PA_SORT_VARIANT picks another.
+#define PA_VARIANT(K, KV, B, I, W, P, M) \
+  { B, I, W, P, M, &launch_pass<K, KV, B, I, W, P, M> }
+
 template <typename K, bool KV>
 struct Variants;
 template <>
 struct Variants<uint32_t, false> {
   static constexpr PassVariant table[] = {
-      PA_VARIANT(uint32_t, false, 512, 24, 8, 1, false, 1, 1, 0), PA_VARIANT(uint32_t, false, 512, 16, 8, 1, false, 1, 1, 0),
-      PA_VARIANT(uint32_t, false, 256, 32, 8, 1, false, 1, 1, 0), PA_VARIANT(uint32_t, false, 512, 16, 8, 1, false, 1, 1, 32),
-      PA_VARIANT(uint32_t, false, 512, 16, 8, 1, true, 1, 1, 0),
+      PA_VARIANT(uint32_t, false, 512, 24, 4, false, 6), PA_VARIANT(uint32_t, false, 512, 24, 8, false, 6),
+      PA_VARIANT(uint32_t, false, 512, 24, 2, false, 6), PA_VARIANT(uint32_t, false, 512, 20, 4, false, 6),
+      PA_VARIANT(uint32_t, false, 256, 32, 4, false, 4), PA_VARIANT(uint32_t, false, 256, 48, 4, false, 3),
+      PA_VARIANT(uint32_t, false, 512, 32, 4, false, 4), PA_VARIANT(uint32_t, false, 512, 24, 4, true, 4),
   };
 };
 template <>
 struct Variants<uint32_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 8, 1, false, 1, 1, 0),
-                                          PA_VARIANT(uint32_t, true, 512, 16, 8, 1, false, 1, 1, 32)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 8, false, 5),
+                                          PA_VARIANT(uint32_t, true, 512, 16, 8, false, 4)};
 };
 template <>
 struct Variants<uint64_t, false> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 8, 1, false, 1, 1, 0),
-                                          PA_VARIANT(uint64_t, false, 512, 12, 8, 1, false, 1, 1, 0),
-                                          PA_VARIANT(uint64_t, false, 512, 8, 8, 1, false, 1, 1, 32)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 8, false, 8),
+                                          PA_VARIANT(uint64_t, false, 512, 12, 8, false, 6),
+                                          PA_VARIANT(uint64_t, false, 512, 8, 8, true, 4)};
 };
 template <>
 struct Variants<uint64_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 8, 1, false, 1, 1, 0),
-                                          PA_VARIANT(uint64_t, true, 512, 8, 8, 1, false, 1, 1, 32)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 8, false, 6),
+                                          PA_VARIANT(uint64_t, true, 512, 8, 8, true, 4)};
 };
 
 template <typename K, bool KV>
@@ -132,20 +143,11 @@ const PassVariant& pick_variant() {
   return Variants<K, KV>::table[idx];
 }
 
-// Chain split of one sort: nchains contiguous chains of tpc tiles each.
-struct ChainPlan {
-  size_t tile, tiles;
-  uint32_t nchains, tpc;
-};
-
-inline ChainPlan chain_plan(const PassVariant& v, size_t n) {
-  ChainPlan c;
-  c.tile = (size_t)v.block * v.items;
-  c.tiles = (n + c.tile - 1) / c.tile;
-  const size_t want = std::max<size_t>(1, std::min<size_t>((size_t)v.chains, c.tiles));
-  c.tpc = (uint32_t)((c.tiles + want - 1) / want);
-  c.nchains = (uint32_t)((c.tiles + c.tpc - 1) / c.tpc);
-  return c;
+template <typename K, bool KV>
+size_t sort_tiles(size_t n) {
+  const PassVariant& v = pick_variant<K, KV>();
+  const size_t tile = (size_t)v.block * v.items;
+  return (n + tile - 1) / tile;
 }
 
 // Enqueues the full sort on device buffers.  scratch must hold
@@ -153,12 +155,11 @@ inline ChainPlan chain_plan(const PassVariant& v, size_t n) {
 template <typename K, bool KV>
 size_t sort_scratch_bytes(size_t n) {
   constexpr int D = sizeof(K);
-  const ChainPlan c = chain_plan(pick_variant<K, KV>(), n);
   Layout L;
-  L.take((size_t)D * c.nchains * 256 * 4);  // counts
-  L.take((size_t)D * c.nchains * 256 * 4);  // bases
-  L.take(D * 4);
-  L.take((size_t)D * c.tiles * 256 * 4);
+  L.take((size_t)D * 256 * 4);  // counts
+  L.take((size_t)D * 256 * 4);  // bases
+  L.take(D * 4);                // tile counters
+  L.take((size_t)D * sort_tiles<K, KV>(n) * 256 * 4);
   return L.off;
 }
 
@@ -167,13 +168,13 @@ int enqueue_sort(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, cha
                  bool float_keys) {
   constexpr int D = sizeof(K);
   const PassVariant& var = pick_variant<K, KV>();
-  const ChainPlan c = chain_plan(var, n);
-  const size_t ncount = (size_t)D * c.nchains * 256;
+  const size_t tiles = sort_tiles<K, KV>(n);
+  const size_t ncount = (size_t)D * 256;
   Layout L;
   uint32_t* counts = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
   uint32_t* bases = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
   uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch + L.take(D * 4));
-  const size_t lb_bytes = (size_t)D * c.tiles * 256 * 4;
+  const size_t lb_bytes = (size_t)D * tiles * 256 * 4;
   uint32_t* lb = reinterpret_cast<uint32_t*>(scratch + L.take(lb_bytes));
 
   {
@@ -182,17 +183,15 @@ int enqueue_sort(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, cha
   }
   {
     const size_t nvec = n / (16 / sizeof(K)) + 1;
-    size_t want = std::max<size_t>(std::min<size_t>((nvec + kSortBlock - 1) / kSortBlock, 2048),
-                                   std::min<size_t>(lb_bytes / 16 / kSortBlock + 1, 2048));
-    const unsigned parts = (unsigned)std::max<size_t>(1, want / c.nchains);
+    const size_t grid = std::max<size_t>(std::min<size_t>((nvec + kSortBlock - 1) / kSortBlock, 2048),
+                                         std::min<size_t>(lb_bytes / 16 / kSortBlock + 1, 2048));
     ProfScope ps("k_hist_digits", s);
-    hipLaunchKernelGGL(k_hist_digits<K>, dim3(parts * c.nchains), dim3(kSortBlock), 0, s, keys, n, counts,
-                       reinterpret_cast<uint4*>(lb), lb_bytes / 16, (uint32_t)float_keys, c.nchains,
-                       (size_t)c.tpc * c.tile);
+    hipLaunchKernelGGL(k_hist_digits<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, counts,
+                       reinterpret_cast<uint4*>(lb), lb_bytes / 16, (uint32_t)float_keys);
   }
   {
-    ProfScope ps("k_chain_bases", s);
-    hipLaunchKernelGGL(k_chain_bases, dim3(D), dim3(256), 0, s, counts, c.nchains, bases);
+    ProfScope ps("k_digit_bases", s);
+    hipLaunchKernelGGL(k_digit_bases, dim3(D), dim3(256), 0, s, counts, bases);
   }
   K* kin = keys;
   K* kout = ktmp;
@@ -202,10 +201,10 @@ int enqueue_sort(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, cha
     uint32_t fmode = 0;
     if (float_keys && p == 0) fmode |= kFloatIn;
     if (float_keys && p == D - 1) fmode |= kFloatOut;
-    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * c.nchains * 256, lb + (size_t)p * c.tiles * 256,
-               ctr + p, (uint32_t)(8 * p), fmode, c.nchains, c.tpc};
+    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * 256, lb + (size_t)p * tiles * 256,
+               ctr + p, (uint32_t)tiles, (uint32_t)(8 * p), fmode};
     ProfScope ps("k_onesweep", s);
-    var.launch((unsigned)(c.nchains * c.tpc), s, a);
+    PA_TRY(var.launch(s, a));
     std::swap(kin, kout);
     std::swap(vin, vout);
   }

@@ -17,35 +17,24 @@ constexpr size_t kMaxSortN = (size_t(1) << 30) - 1;
 enum : uint32_t { kFloatIn = 1u, kFloatOut = 2u };
 
 
-// One read of the keys -> D digit histograms (D = sizeof(K)).
+// One read of the keys -> D digit histograms (D = sizeof(K)), counts laid out
+// [D][256].  Also clears the look-back table of the onesweep passes that
+// follow (zero_vec 16-byte words at zero_ptr).
 template <typename K>
-// With nchains > 1 the keys are split into nchains contiguous chains of
-// chain_keys keys (the last may be shorter); block b counts part of chain
-// b % nchains, and counts are laid out [D][nchains][256].  gridDim.x must be
-// a multiple of nchains.
 __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict__ keys, size_t n,
                                                             uint32_t* __restrict__ counts, uint4* __restrict__ zero_ptr,
-                                                            size_t zero_vec, uint32_t float_keys,
-                                                            uint32_t nchains = 1, size_t chain_keys = 0) {
+                                                            size_t zero_vec, uint32_t float_keys) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
   __shared__ uint32_t s_hist[D * 256];
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < D * 256; i += kSortBlock) s_hist[i] = 0;
 
-  // Clear the look-back table used by the following onesweep passes.
   const size_t gstride = (size_t)gridDim.x * kSortBlock;
   for (size_t i = (size_t)blockIdx.x * kSortBlock + tid; i < zero_vec; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
 
-  // This block's chain and its share of it.
-  const uint32_t chain = blockIdx.x % nchains;
-  const uint32_t part = blockIdx.x / nchains, parts = gridDim.x / nchains;
-  if (nchains > 1) {
-    const size_t lo = (size_t)chain * chain_keys;
-    keys += lo;
-    n = lo < n ? std::min(chain_keys, n - lo) : 0;
-  }
+  const uint32_t part = blockIdx.x, parts = gridDim.x;
   const size_t pstride = (size_t)parts * kSortBlock;
   size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
   if (head > n) head = n;
@@ -114,340 +103,331 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
   __syncthreads();
   for (uint32_t i = tid; i < D * 256; i += kSortBlock) {
     uint32_t c = s_hist[i];
-    if (c) atomicAdd(&counts[((i >> 8) * nchains + chain) * 256 + (i & 255)], c);
+    if (c) atomicAdd(&counts[i], c);
   }
 }
 
 
-// One stable digit pass over a tile of BLOCK x ITEMS keys.  Tiles are claimed
-// in order from a global counter so every tile a block waits on in the
-// look-back is already held by a running block.  The look-back reads LBW
-// predecessor status words per round trip.
+// One stable 8-bit digit pass (onesweep).  A block ranks a tile of
+// BLOCK x ITEMS keys in LDS, publishes the tile's per-digit counts to a
+// decoupled look-back table, resolves its global offsets from the
+// predecessors' words and writes the tile out in digit order.
 //
-// Chain mode (!SEG): the keys are split into nchains contiguous chains of
-// tpc tiles, each with its own look-back chain; claimed index i is tile i /
-// nchains of chain i % nchains, so all chains advance together.  gtab holds
-// this pass's base[chain][digit] output offsets.  Independent chains keep the
-// look-back short: each chain sees 1/nchains of the tile throughput.
+// Tiles are claimed in order from a global counter, so every tile a block
+// waits on in the look-back is held by a block that is already running.
 //
-// Segmented mode (SEG): many independent segments, each with its own input
-// and output address (seg_in / seg_out, device addresses as u64) and length
-// (seg_len); kin/kout are unused.  tile_info[t] = (segment << 32) | tile index
-// within the segment, gtab holds per-segment digit counts strided by
-// 256 * gstride per segment, and
-// the look-back stops at the segment's first tile.  Tiles never span segments.
-// MINW: minimum waves per SIMD the register allocation must allow (1 = no
-// constraint); 6 keeps a 512-thread block at 3 resident blocks per CU.
-// COOP > 0: before the per-digit look-back, the whole block loads the status
-// words of the COOP preceding tiles (all 256 digits) into LDS in one round
-// trip; each digit then walks that window in LDS and polls memory only if the
-// window holds no inclusive prefix yet.
-template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool SEG = false, int RANK = 1, bool EARLY = false,
-          int MINW = 1, int COOP = 0>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void k_onesweep(const K* __restrict__ kin, K* __restrict__ kout,
-                                                    const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
-                                                    uint32_t n, const uint32_t* __restrict__ gtab,
-                                                    uint32_t* __restrict__ lb, uint32_t* __restrict__ tile_counter,
-                                                    uint32_t shift, uint32_t fmode, uint32_t nchains, uint32_t tpc,
-                                                    const uint64_t* __restrict__ tile_info = nullptr,
-                                                    const uint64_t* __restrict__ seg_in = nullptr,
-                                                    const uint64_t* __restrict__ seg_out = nullptr,
-                                                    const uint32_t* __restrict__ seg_len = nullptr,
-                                                    uint32_t gstride = 0) {
-  static_assert(BLOCK >= 256 && BLOCK % 64 == 0, "one thread per digit is needed");
-  constexpr int WAVES = BLOCK / 64;
+// Ranking units are half-waves (32 lanes).  Unit u owns the USPAN = 32 x
+// ITEMS consecutive keys [u*USPAN, (u+1)*USPAN) of the tile; item i of lane l
+// is key u*USPAN + i*32 + l, so (unit, item, lane) order is input order and a
+// load instruction reads two 128-byte runs.  Each unit has 256 LDS slots of
+// 64 bits: the low word collects the lanes holding a digit in the current
+// item (each lane ORs its bit), the high word counts that digit's keys seen so
+// far in the unit.  One read returns both, so a key's stable rank in its unit
+// is count + (peer lanes below it); the lowest peer then writes back
+// (count + peers, 0), which also clears the lane mask.  LDS operations of
+// one wave execute in program order and OR commutes, so the result does not
+// depend on the order in which the LDS applies the lanes.
+//
+// Partial tiles are padded with all-ones keys: their digit is 255 in every
+// pass, so they rank after every real key of the tile and occupy the staging
+// slots [valid, TILE), which are never written out.  Only the last tile of an
+// array (or segment) is partial, and no tile looks back past the first tile
+// of its own array, so the padded count it publishes is never consumed.
+//
+// Segmented mode (SEG): tile_info[t] = (segment << 32) | tile index within
+// the segment; the segment's input/output addresses and length come from
+// seg_in / seg_out / seg_len and its digit counts from gtab + segment *
+// gstride * 256.  Plain mode: gtab holds the pass's exclusive digit bases.
+//
+// PIPE: persistent blocks.  After ranking tile t the block claims the next
+// tile and issues its loads, so they are in flight while tile t waits in the
+// look-back, is staged and is written out.
+//
+// MINW: minimum waves per SIMD the register allocation must allow.
+template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool SEG = false, bool PIPE = false, int MINW = 1>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void k_onesweep(
+    const K* __restrict__ kin, K* __restrict__ kout, const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
+    uint32_t n, const uint32_t* __restrict__ gtab, uint32_t* __restrict__ lb, uint32_t* __restrict__ tile_counter,
+    uint32_t ntiles, uint32_t shift, uint32_t fmode, const uint64_t* __restrict__ tile_info = nullptr,
+    const uint64_t* __restrict__ seg_in = nullptr, const uint64_t* __restrict__ seg_out = nullptr,
+    const uint32_t* __restrict__ seg_len = nullptr, uint32_t gstride = 0) {
+  static_assert(BLOCK >= 256 && BLOCK <= 1024 && BLOCK % 64 == 0, "one thread per digit is needed");
+  static_assert(!(SEG && KV), "segmented mode sorts keys only");
+  constexpr int UNITS = BLOCK / 32;
+  constexpr int USPAN = 32 * ITEMS;
   constexpr int TILE = BLOCK * ITEMS;
-  constexpr int WAVE_SPAN = 64 * ITEMS;
+  static_assert(TILE * sizeof(K) >= UNITS * 256 * sizeof(uint64_t), "rank slots must fit the key staging area");
   __shared__ __align__(16) K s_keys[TILE];
-  __shared__ uint32_t s_whist[WAVES * 256];
   __shared__ uint32_t s_goff[256];
-  __shared__ uint32_t s_cnt[EARLY ? 256 : 1];
   __shared__ uint32_t s_wsum[2][4];
-  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_claim[2];
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
   const uint32_t wave = tid >> 6;
+  const uint32_t unit = tid >> 5;
+  const uint32_t ul = tid & 31;
+  const uint32_t lanebit = 1u << ul;
+  // Rank slots of this thread's unit (they alias the key staging area, which
+  // is idle until the ranking is over).
+  uint64_t* const uslot = reinterpret_cast<uint64_t*>(s_keys) + unit * 256;
+  uint32_t* const uword = reinterpret_cast<uint32_t*>(uslot);
+  // High words of every unit's slots, for the per-digit pass over units.
+  uint32_t* const hiword = reinterpret_cast<uint32_t*>(s_keys) + 1;
+  const bool float_in = sizeof(K) == 4 && (fmode & kFloatIn);
+  const bool float_out = sizeof(K) == 4 && (fmode & kFloatOut);
 
-  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-  for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_whist[i] = 0;
-  if constexpr (EARLY) {
-    if (tid < 256) s_cnt[tid] = 0;
-  }
-  if constexpr (RANK != 0) {
-    static_assert(TILE * sizeof(K) >= WAVES * 256 * sizeof(uint64_t), "match slots must fit the key staging area");
-    uint64_t* m = reinterpret_cast<uint64_t*>(s_keys);
-    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) m[i] = 0ull;
-  }
-  __syncthreads();
-  uint32_t tile = s_tile;
-  uint32_t local;  // tile index within its chain or segment
-  uint32_t chain = 0;
-  if constexpr (!SEG) {
-    chain = tile % nchains;
-    local = tile / nchains;
-    tile = chain * tpc + local;  // position of the tile in the key array
-    if (tile * (uint32_t)TILE >= n) return;  // past the end of a short last chain
-    gtab += chain * 256;
-  } else {
-    const uint64_t info = tile_info[tile];
-    const uint32_t seg = (uint32_t)(info >> 32);
-    local = (uint32_t)info;
-    static_assert(!(SEG && KV), "segmented mode sorts keys only");
-    kin = reinterpret_cast<const K*>(seg_in[seg]);
-    kout = reinterpret_cast<K*>(seg_out[seg]);
-    n = seg_len[seg];
-    gtab += (size_t)seg * gstride * 256;
-  }
-  const uint32_t first_tile = tile - local;  // look-back stops here
-  const uint32_t tile_base = (SEG ? local : tile) * (uint32_t)TILE;
-  const uint32_t valid = min((uint32_t)TILE, n - tile_base);
-  const bool full = valid == (uint32_t)TILE;
-
-  // Wave-striped load: wave w owns [tile_base + w*WAVE_SPAN, +WAVE_SPAN); item
-  // i of lane l is element i*64 + l of that span, so (item, lane) order is the
-  // input order and every load instruction reads 64 consecutive keys.
-  const uint32_t wbase = tile_base + wave * WAVE_SPAN + lane;
-  K key[ITEMS];
-  uint32_t val[KV ? ITEMS : 1];
-  if (full) {
-    // One base address per lane; the item offsets are compile-time constants.
-    const K* src = kin + wbase;
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) key[i] = src[i * 64];
-    if constexpr (KV) {
-      const uint32_t* vsrc = vin + wbase;
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) val[i] = vsrc[i * 64];
+  // Per-tile geometry.
+  struct Tile {
+    const K* in;
+    K* out;
+    const uint32_t* vin;
+    uint32_t* vout;
+    const uint32_t* g;  // digit bases (plain) or segment digit counts (SEG)
+    uint32_t len;       // keys in the tile's array
+    uint32_t row;       // look-back row of the tile
+    uint32_t local;     // tile index within its array
+  };
+  auto make_tile = [&](uint32_t claimed) {
+    Tile t;
+    t.row = claimed;
+    if constexpr (SEG) {
+      const uint64_t info = tile_info[claimed];
+      const uint32_t seg = (uint32_t)(info >> 32);
+      t.local = (uint32_t)info;
+      t.in = reinterpret_cast<const K*>(seg_in[seg]);
+      t.out = reinterpret_cast<K*>(seg_out[seg]);
+      t.vin = nullptr;
+      t.vout = nullptr;
+      t.len = seg_len[seg];
+      t.g = gtab + (size_t)seg * gstride * 256;
+    } else {
+      t.local = claimed;
+      t.in = kin;
+      t.out = kout;
+      t.vin = vin;
+      t.vout = vout;
+      t.len = n;
+      t.g = gtab;
     }
-  } else {
+    return t;
+  };
+  // Keys of a tile into registers; all-ones padding past the array's end.
+  auto load_keys = [&](const Tile& t, K (&k)[ITEMS]) {
+    const uint32_t first = t.local * (uint32_t)TILE + unit * USPAN + ul;
+    if (t.local * (uint32_t)TILE + TILE <= t.len) {
+      const K* src = t.in + first;
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      uint32_t idx = wbase + i * 64;
-      key[i] = idx < n ? kin[idx] : K(0);
-    }
-    if constexpr (KV) {
+      for (int i = 0; i < ITEMS; ++i) k[i] = src[i * 32];
+      if (float_in) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) k[i] = (K)dev::f32_fwd((uint32_t)k[i]);
+      }
+    } else {
+      const K* src = t.in + first;
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) {
-        uint32_t idx = wbase + i * 64;
-        val[i] = idx < n ? vin[idx] : 0u;
+        K v = ~K(0);
+        if (first + i * 32 < t.len) {
+          v = src[i * 32];
+          if (float_in) v = (K)dev::f32_fwd((uint32_t)v);
+        }
+        k[i] = v;
       }
     }
-  }
-  if (sizeof(K) == 4 && (fmode & kFloatIn)) {
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) key[i] = (K)dev::f32_fwd((uint32_t)key[i]);
-  }
+  };
 
-  // EARLY: count the tile's digits first, publish the aggregate and put the
-  // predecessor's status word in flight, so the look-back latency overlaps
-  // the ranking instead of following it.
-  uint32_t cnt = 0;  // this thread's digit count for the tile (threads < 256)
-  uint32_t prefetched = 0;
-  if constexpr (EARLY) {
+  if (tid == 0) s_claim[0] = atomicAdd(tile_counter, 1u);
+  __syncthreads();
+  uint32_t claimed = s_claim[0];
+  if (claimed >= ntiles) return;  // the same for every thread of the block
+  Tile cur = make_tile(claimed);
+  K key[ITEMS];
+  load_keys(cur, key);
+
+  for (;;) {
+    // An opaque copy of the thread index: the per-item indices derived from it
+    // below are recomputed each iteration instead of being hoisted out of the
+    // tile loop (and spilled).
+    uint32_t tix = tid;
+    asm volatile("" : "+v"(tix));
+    const uint32_t tile_first = cur.local * (uint32_t)TILE;
+    const uint32_t valid = min((uint32_t)TILE, cur.len - tile_first);
+    uint32_t val[KV ? ITEMS : 1];
+    if constexpr (KV) {
+      const uint32_t first = tile_first + unit * USPAN + ul;
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t idx = first + i * 32;
+        val[i] = idx < cur.len ? cur.vin[idx] : 0u;
+      }
+    }
+    // Clear the rank slots.
+    {
+      uint4* z = reinterpret_cast<uint4*>(s_keys);
+#pragma unroll
+      for (int r = 0; r < UNITS * 256 * 8 / 16 / BLOCK; ++r) z[tid + r * BLOCK] = make_uint4(0, 0, 0, 0);
+    }
+    if (PIPE && tid == 0) s_claim[1] = atomicAdd(tile_counter, 1u);
+    __syncthreads();
+
+    // Stable rank of every key within its unit.
+    uint32_t rank[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      if (full || (wbase + i * 64 < n)) {
-        const uint32_t dg[1] = {dev::digit8<K>(key[i], shift)};
-        dev::lds_count<1>(s_cnt, dg);
+      const uint32_t d = dev::digit8<K>(key[i], shift);
+      atomicOr(&uword[2 * d], lanebit);
+      const uint64_t sv = uslot[d];
+      const uint32_t peers = (uint32_t)sv;
+      const uint32_t seen = (uint32_t)(sv >> 32);
+      const uint32_t below = __popc(peers & (lanebit - 1u));
+      rank[i] = seen + below;
+      // Keep the rank in one register instead of its two operands.
+      asm volatile("" : "+v"(rank[i]));
+      if (below == 0) uslot[d] = (uint64_t)(seen + __popc(peers)) << 32;
+    }
+    __syncthreads();
+
+    // Next tile's keys go in flight now (persistent blocks).
+    Tile nxt = cur;
+    bool more = false;
+    K knext[PIPE ? ITEMS : 1];
+    if constexpr (PIPE) {
+      const uint32_t c = s_claim[1];
+      more = c < ntiles;
+      if (more) {
+        nxt = make_tile(c);
+        load_keys(nxt, knext);
+      }
+    }
+
+    // Per digit: prefix over units, tile count, publish the aggregate, scans
+    // over the 256 digits of the tile counts (and, in SEG mode, of the
+    // segment's digit counts).
+    uint32_t cnt = 0, gval = 0, cinc = 0, ginc = 0;
+    if (tid < 256) {
+#pragma unroll
+      for (int u = 0; u < UNITS; ++u) {
+        const uint32_t c = hiword[2 * (u * 256 + tid)];
+        hiword[2 * (u * 256 + tid)] = cnt;
+        cnt += c;
+      }
+      dev::atomic_store_agent(&lb[(size_t)cur.row * 256 + tid], (cur.local == 0 ? kLbInc : kLbAgg) | cnt);
+      if constexpr (SEG) gval = cur.g[tid];
+      cinc = dev::wave_incl_scan(cnt, lane);
+      if constexpr (SEG) ginc = dev::wave_incl_scan(gval, lane);
+      if (lane == 63) {
+        s_wsum[0][wave] = cinc;
+        s_wsum[1][wave] = ginc;
       }
     }
     __syncthreads();
     if (tid < 256) {
-      cnt = s_cnt[tid];
-      dev::atomic_store_agent(&lb[(size_t)tile * 256 + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
-      if (local > 0) prefetched = dev::atomic_load_agent(&lb[(size_t)(tile - 1) * 256 + tid]);
-    }
-  }
-
-  // Stable rank within the wave.  The lanes holding the same digit (the
-  // "peers") are found either by 8 ballots (RANK 0) or by every lane OR-ing
-  // its lane bit into the wave's 64-bit LDS slot for its digit and reading the
-  // slot back (RANK 1: 3 LDS operations instead of ~70 VALU instructions; OR
-  // commutes, so the result does not depend on the order the LDS applies the
-  // lanes).  The lowest peer bumps the wave's private digit counter and clears
-  // the slot.  LDS operations of one wave execute in program order.
-  uint32_t rank[ITEMS];
-  uint32_t* wh = s_whist + wave * 256;
-  // The match slots alias the key staging area, which is not used until the
-  // ranking is over.
-  uint64_t* wm = reinterpret_cast<uint64_t*>(s_keys) + wave * 256;
-#pragma unroll 4
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t d = dev::digit8<K>(key[i], shift);
-    const bool ok = full || (wbase + i * 64 < n);
-    uint64_t peers;
-    if constexpr (RANK == 0) {
-      peers = __ballot(ok);
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+      uint32_t cpre = 0, gpre = 0;
+      for (uint32_t w = 0; w < wave; ++w) {
+        cpre += s_wsum[0][w];
+        gpre += s_wsum[1][w];
       }
-    } else {
-      if (ok) atomicOr(reinterpret_cast<unsigned long long*>(&wm[d]), 1ull << lane);
-      peers = ok ? wm[d] : 0ull;
-    }
-    const uint32_t below = dev::count_below(peers);
-    const uint32_t base = wh[d];
-    rank[i] = base + below;
-    if (ok && below == 0) {
-      wh[d] = base + (uint32_t)__popcll(peers);
-      if constexpr (RANK != 0) wm[d] = 0ull;
-    }
-  }
-  __syncthreads();
+      const uint32_t tstart = cpre + cinc - cnt;  // first staging slot of this digit
+      const uint32_t gstart = SEG ? gpre + ginc - gval : cur.g[tid];
+#pragma unroll
+      for (int u = 0; u < UNITS; ++u) hiword[2 * (u * 256 + tid)] += tstart;
 
-  // Per digit: exclusive prefix over waves, tile total, publish the aggregate
-  // (unless EARLY already did).
-  uint32_t gval = 0;
-  if (tid < 256) {
-    uint32_t tot = 0;
+      // Decoupled look-back: LBW status words per round trip; rows before the
+      // array's first tile read as "inclusive 0".
+      uint32_t excl = 0;
+      if (cur.local > 0) {
+        const uint32_t first_row = cur.row - cur.local;
+        uint32_t p = cur.row - 1;
+        for (;;) {
+          const uint32_t avail = p - first_row + 1;
+          uint32_t w[LBW];
 #pragma unroll
-    for (int w = 0; w < WAVES; ++w) {
-      uint32_t c = s_whist[w * 256 + tid];
-      s_whist[w * 256 + tid] = tot;
-      tot += c;
-    }
-    if constexpr (!EARLY) {
-      cnt = tot;
-      dev::atomic_store_agent(&lb[(size_t)tile * 256 + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
-    }
-    if constexpr (SEG) gval = gtab[tid];
-  }
-  // Exclusive scans over the 256 digits of (tile count, global count).
-  uint32_t cinc = 0, ginc = 0;
-  if (tid < 256) {
-    cinc = dev::wave_incl_scan(cnt, lane);
-    ginc = dev::wave_incl_scan(gval, lane);
-    if (lane == 63) {
-      s_wsum[0][wave] = cinc;
-      s_wsum[1][wave] = ginc;
-    }
-  }
-  // Cooperative look-back window: status words of tiles tile-1 .. tile-COOP
-  // for every digit, win[j][d] (the key staging area is idle until the
-  // scatter below).  Words before the chain's first tile read as "inclusive 0".
-  uint32_t* const win = reinterpret_cast<uint32_t*>(s_keys);
-  if constexpr (COOP > 0) {
-    static_assert(COOP * 256 * 4 <= TILE * (int)sizeof(K), "window must fit the key staging area");
-    static_assert((COOP * 256) % BLOCK == 0, "whole window rows per thread");
-    if (local > 0) {
-      uint32_t w[COOP * 256 / BLOCK];
+          for (int j = 0; j < LBW; ++j)
+            w[j] = (uint32_t)j < avail ? dev::atomic_load_agent(&lb[(size_t)(p - j) * 256 + tid]) : kLbInc;
+          uint32_t used = 0;
+          bool stop = false, done = false;
 #pragma unroll
-      for (int r = 0; r < COOP * 256 / BLOCK; ++r) {
-        const uint32_t q = tid + r * BLOCK;
-        const int64_t pt = (int64_t)tile - 1 - (int64_t)(q >> 8);
-        w[r] = pt >= (int64_t)first_tile ? dev::atomic_load_agent(&lb[(size_t)pt * 256 + (q & 255)]) : kLbInc;
-      }
-#pragma unroll
-      for (int r = 0; r < COOP * 256 / BLOCK; ++r) win[tid + r * BLOCK] = w[r];
-    }
-  }
-  __syncthreads();
-  if (tid < 256) {
-    uint32_t cpre = 0, gpre = 0;
-    for (uint32_t w = 0; w < wave; ++w) {
-      cpre += s_wsum[0][w];
-      gpre += s_wsum[1][w];
-    }
-    const uint32_t tstart = cpre + cinc - cnt;  // tile-local start of this digit
-    // First output slot of this digit for the chain (or segment).
-    const uint32_t gstart = SEG ? gpre + ginc - gval : gtab[tid];
-
-    // Decoupled look-back over earlier tiles of the segment for this digit, LBW
-    // status words per round trip; words before the first tile read as
-    // "inclusive 0".
-    uint32_t excl = 0;
-    if (local > 0) {
-      int64_t p = (int64_t)tile - 1;
-      bool resolved = false;
-      if constexpr (COOP > 0) {
-        for (int j = 0; j < COOP && !resolved; ++j) {
-          const uint32_t wv = win[j * 256 + tid];
-          const uint32_t flag = wv & ~kLbMask;
-          if (flag == 0) break;  // not published when the window was read
-          excl += wv & kLbMask;
-          resolved = flag == kLbInc;
-          --p;
-        }
-      } else if constexpr (EARLY) {
-        const uint32_t flag = prefetched & ~kLbMask;
-        if (flag != 0) {
-          excl += prefetched & kLbMask;
-          resolved = flag == kLbInc;
-          --p;
-        }
-      }
-      while (!resolved) {
-        uint32_t w[LBW];
-#pragma unroll
-        for (int j = 0; j < LBW; ++j)
-          w[j] = (p - j >= (int64_t)first_tile) ? dev::atomic_load_agent(&lb[(size_t)(p - j) * 256 + tid]) : kLbInc;
-        int used = 0;
-        bool stop = false, done = false;
-#pragma unroll
-        for (int j = 0; j < LBW; ++j) {
-          if (!stop) {
-            const uint32_t flag = w[j] & ~kLbMask;
-            if (flag == 0) {
-              stop = true;  // not published yet: poll again from here
-            } else {
-              excl += w[j] & kLbMask;
-              ++used;
-              if (flag == kLbInc) stop = done = true;
+          for (int j = 0; j < LBW; ++j) {
+            if (!stop) {
+              const uint32_t flag = w[j] & ~kLbMask;
+              if (flag == 0) {
+                stop = true;  // not published yet: poll again from here
+              } else {
+                excl += w[j] & kLbMask;
+                ++used;
+                if (flag == kLbInc) stop = done = true;
+              }
             }
           }
+          if (done) break;
+          p -= used;
+          if (used == 0) __builtin_amdgcn_s_sleep(1);
         }
-        if (done) break;
-        p -= used;
-        if (used == 0) __builtin_amdgcn_s_sleep(1);
+        dev::atomic_store_agent(&lb[(size_t)cur.row * 256 + tid], kLbInc | (excl + cnt));
       }
-      dev::atomic_store_agent(&lb[(size_t)tile * 256 + tid], kLbInc | (excl + cnt));
+      s_goff[tid] = gstart + excl - tstart;
     }
-    s_goff[tid] = gstart + excl - tstart;
-#pragma unroll
-    for (int w = 0; w < WAVES; ++w) s_whist[w * 256 + tid] += tstart;
-  }
-  __syncthreads();
-
-  // Stage the tile in LDS in digit order.
-  uint32_t lpos[ITEMS];
-#pragma unroll 4
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t d = dev::digit8<K>(key[i], shift);
-    lpos[i] = wh[d] + rank[i];
-    if (full || (wbase + i * 64 < n)) s_keys[lpos[i]] = key[i];
-  }
-  __syncthreads();
-
-  // Coalesced write-out: consecutive slots of one digit go to consecutive
-  // addresses of that digit's bin.
-  uint32_t gpos[KV ? ITEMS : 1];
-#pragma unroll 4
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t j = tid + i * BLOCK;
-    if (j < valid) {
-      K k = s_keys[j];
-      const uint32_t p = s_goff[dev::digit8<K>(k, shift)] + j;
-      if (sizeof(K) == 4 && (fmode & kFloatOut)) k = (K)dev::f32_inv((uint32_t)k);
-      kout[p] = k;
-      if constexpr (KV) gpos[i] = p;
-    }
-  }
-  if constexpr (KV) {
     __syncthreads();
-    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
+
+    // Staging slot of every key: its unit's start for the digit + its rank.
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i)
-      if (full || (wbase + i * 64 < n)) s_vals[lpos[i]] = val[i];
+    for (int i = 0; i < ITEMS; ++i) rank[i] += hiword[2 * (unit * 256 + dev::digit8<K>(key[i], shift))];
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t j = tid + i * BLOCK;
-      if (j < valid) vout[gpos[i]] = s_vals[j];
+    for (int i = 0; i < ITEMS; ++i) s_keys[rank[i]] = key[i];
+    __syncthreads();
+
+    // Coalesced write-out: consecutive slots of one digit go to consecutive
+    // addresses of that digit's bin.  Groups of G items keep G LDS reads in
+    // flight without holding every item's address in registers.
+    constexpr int G = ITEMS % 8 == 0 ? 8 : (ITEMS % 4 == 0 ? 4 : 1);
+    uint32_t gpos[KV ? ITEMS : 1];
+    auto write_out = [&](auto fo) {
+#pragma unroll
+      for (int g0 = 0; g0 < ITEMS; g0 += G) {
+        K wk[G];
+        uint32_t gp[G];
+#pragma unroll
+        for (int i = 0; i < G; ++i) wk[i] = s_keys[tix + (g0 + i) * BLOCK];
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+          gp[i] = s_goff[dev::digit8<K>(wk[i], shift)] + tix + (g0 + i) * BLOCK;
+          if constexpr (KV) gpos[g0 + i] = gp[i];
+          if constexpr (decltype(fo)::value) wk[i] = (K)dev::f32_inv((uint32_t)wk[i]);
+        }
+        if (valid == (uint32_t)TILE) {
+#pragma unroll
+          for (int i = 0; i < G; ++i) cur.out[gp[i]] = wk[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < G; ++i)
+            if (tix + (g0 + i) * BLOCK < valid) cur.out[gp[i]] = wk[i];
+        }
+      }
+    };
+    if (float_out)
+      write_out(std::true_type{});
+    else
+      write_out(std::false_type{});
+    if constexpr (KV) {
+      __syncthreads();
+      uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) s_vals[rank[i]] = val[i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        if (tix + i * BLOCK < valid) cur.vout[gpos[i]] = s_vals[tix + i * BLOCK];
     }
+
+    if (!more) break;
+    cur = nxt;
+    if constexpr (PIPE) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) key[i] = knext[i];
+    }
+    __syncthreads();  // the staging area is read before the slots are cleared
   }
 }
 
