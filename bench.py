@@ -126,7 +126,8 @@ def bench_segmented(pa, torch, count=10_000, seed=1239):
 
 def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
     """Served stream: jobs 70% sort / 20% select (k log-uniform 1..n/10) / 10%
-    256-bin histogram, sizes log-uniform 4K..64M, sort keys u32 or u64 (50/50),
+    256-bin histogram of the top byte, sizes log-uniform 4K..64M, keys u32 or
+    u64 (50/50),
     Poisson arrivals at fractions of the measured saturation rate, 64 GB
     admission budget.  Latency = completion - submission."""
     import time
@@ -138,12 +139,8 @@ def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
     spec = []
     for i in range(jobs):
         n = int(sizes[i])
-        if kind_u[i] < 70:
-            op, kt = pa.OP_SORT, (pa.KEY_U64 if wide[i] else pa.KEY_U32)
-        elif kind_u[i] < 90:
-            op, kt = pa.OP_SELECT, pa.KEY_U32
-        else:
-            op, kt = pa.OP_HIST, pa.KEY_U32
+        kt = pa.KEY_U64 if wide[i] else pa.KEY_U32
+        op = pa.OP_SORT if kind_u[i] < 70 else (pa.OP_SELECT if kind_u[i] < 90 else pa.OP_HIST)
         u = float(r[3 * jobs + i] >> np.uint64(11)) / float(1 << 53)
         k = int(np.floor(np.exp(u * np.log(max(2.0, n / 10.0))))) - 1
         spec.append((op, kt, n, min(max(k, 0), n - 1)))
@@ -171,7 +168,8 @@ def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
                         time.sleep(due - time.perf_counter() - 1e-3)
                     while time.perf_counter() < due:
                         pass
-                tickets.append(srv.submit(op, arrs[i], kt, k=k, shift=24, bits=8, out=outs[i]))
+                tickets.append(srv.submit(op, arrs[i], kt, k=k, shift=56 if kt == pa.KEY_U64 else 24, bits=8,
+                                          out=outs[i]))
             lat = [srv.wait(t) for t in tickets]
             span = time.perf_counter() - t0
             st = srv.stats()
@@ -194,9 +192,8 @@ def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
         levels.append({"load": load, "offered_jobs_per_s": round(rate, 1), "achieved_jobs_per_s": round(jobs / span, 1),
                        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                        "gkeys_per_s": round(keys_total / span / 1e9, 2), "coalesced_jobs": st["coalesced_jobs"]})
-    return {"workload": f"{jobs} jobs: 70% sort (u32/u64), 20% select, 10% hist; sizes log-uniform 4K..64M",
-            "saturation_jobs_per_s": round(sat, 1), "levels": levels,
-            "note": "select and histogram jobs use u32 keys"}
+    return {"workload": f"{jobs} jobs: 70% sort, 20% select, 10% hist; u32/u64 keys 50/50; sizes log-uniform 4K..64M",
+            "saturation_jobs_per_s": round(sat, 1), "levels": levels}
 
 
 def load_traffic(kernel, workload):

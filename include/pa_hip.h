@@ -97,6 +97,11 @@ int pa_hist_digits_u32(const uint32_t* keys, size_t n, uint64_t* counts, pa_stre
  * shift, thr) (HistogramParallel.h:535-576) and the _2 variant (:466-533). */
 int pa_hist_u32(const uint32_t* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, pa_stream_t stream);
 
+/* As pa_hist_u32 over uint64 keys, shift in 0..63.  Extension of
+ * HistogramOneByteComponentParallel (HistogramParallel.h:535-576), which is
+ * 32-bit only, for the served uint64 jobs. */
+int pa_hist_u64(const uint64_t* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, pa_stream_t stream);
+
 /* Per-work-quantum histogram of byte `which_byte` over keys[first, last):
  * counts[q][b] (nq rows of 256 uint64) with q = index / quantum on the
  * absolute index; rows outside [first/quantum, (last-1)/quantum] stay zero.
@@ -126,6 +131,10 @@ int pa_hist_u8(const uint8_t* bytes, size_t n, uint64_t* counts, pa_stream_t str
  * SelectRadix(unsigned[], size_t start, size_t len, size_t k)
  * (RadixSelect.h:139-168).  result may be host or device memory. */
 int pa_select_u32(const uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream);
+
+/* As pa_select_u32 over uint64 keys (digits 56 .. 0).  Extension of SelectRadix
+ * (RadixSelect.h:139-168), which is 32-bit only, for the served uint64 jobs. */
+int pa_select_u64(const uint64_t* keys, size_t n, size_t k, uint64_t* result, pa_stream_t stream);
 
 /* As pa_select_u32, and also stores the selected value at keys[k] -- the
  * post-condition SelectRadix's callers rely on (they read a[k] after the call,
@@ -177,12 +186,12 @@ enum pa_key_type { PA_KEY_U32 = 0, PA_KEY_U64 = 1 };
 
 typedef struct pa_job {
   int op;              /* pa_op */
-  int key_type;        /* pa_key_type; select and histogram take PA_KEY_U32 */
+  int key_type;        /* pa_key_type */
   void* keys;          /* device memory, n keys; sorted in place for PA_OP_SORT */
   size_t n;
   size_t k;            /* PA_OP_SELECT: rank */
   unsigned shift, bits; /* PA_OP_HIST: digit (key >> shift) & (2^bits - 1) */
-  void* out;           /* device memory: uint32 (select) or 2^bits uint64 (hist) */
+  void* out;           /* device memory: one key (select) or 2^bits uint64 (hist) */
 } pa_job;
 
 typedef struct pa_server_stats {

@@ -39,10 +39,12 @@ _SIGS = {
     "pa_sort_kv_u64_u32": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
     "pa_hist_digits_u32": (_i, [_vp, _sz, _vp, _vp]),
     "pa_hist_u32": (_i, [_vp, _sz, _u, _u, _vp, _vp]),
+    "pa_hist_u64": (_i, [_vp, _sz, _u, _u, _vp, _vp]),
     "pa_hist_quanta_u32": (_i, [_vp, _sz, _sz, _sz, _sz, _u, _vp, _vp]),
     "pa_start_of_bins_u32": (_i, [_vp, _sz, _sz, _sz, _u, _vp, _vp]),
     "pa_hist_u8": (_i, [_vp, _sz, _vp, _vp]),
     "pa_select_u32": (_i, [_vp, _sz, _sz, _vp, _vp]),
+    "pa_select_u64": (_i, [_vp, _sz, _sz, _vp, _vp]),
     "pa_select_radix_u32": (_i, [_vp, _sz, _sz, _vp, _vp]),
     "pa_merge_u32": (_i, [_vp, _sz, _vp, _sz, _vp, _vp]),
     "pa_merge_u64": (_i, [_vp, _sz, _vp, _sz, _vp, _vp]),
@@ -176,6 +178,10 @@ def hist_u32(keys, shift, bits, counts, stream=None):
     call("pa_hist_u32", _ptr(keys), numel(keys), shift, bits, _ptr(counts), _stream(stream))
 
 
+def hist_u64(keys, shift, bits, counts, stream=None):
+    call("pa_hist_u64", _ptr(keys), numel(keys), shift, bits, _ptr(counts), _stream(stream))
+
+
 def hist_quanta_u32(keys, first, last, quantum, nq, which_byte, counts, stream=None):
     call("pa_hist_quanta_u32", _ptr(keys), first, last, quantum, nq, which_byte, _ptr(counts), _stream(stream))
 
@@ -190,6 +196,10 @@ def hist_u8(data, counts, stream=None):
 
 def select_u32(keys, k, result, stream=None):
     call("pa_select_u32", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
+
+
+def select_u64(keys, k, result, stream=None):
+    call("pa_select_u64", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
 
 
 def select_radix_u32(keys, k, result, stream=None):

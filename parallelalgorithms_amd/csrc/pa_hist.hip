@@ -24,30 +24,37 @@ constexpr unsigned kHistMaxGrid = 2048;
 
 using dev::lds_count;
 
-// Histogram of (key >> shift) & mask over u32 keys, nbins = mask + 1 <= 4096.
-__global__ __launch_bounds__(kHistBlock) void k_hist_u32(const uint32_t* __restrict__ keys, size_t n, uint32_t shift,
+// Histogram of (key >> shift) & mask over u32 or u64 keys, nbins = mask + 1
+// <= 4096.
+template <typename K>
+__global__ __launch_bounds__(kHistBlock) void k_hist_key(const K* __restrict__ keys, size_t n, uint32_t shift,
                                                          uint32_t mask, uint32_t* __restrict__ counts) {
+  constexpr int PER = 16 / sizeof(K);
   extern __shared__ uint32_t s_bins[];
   const uint32_t nb = mask + 1;
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < nb; i += kHistBlock) s_bins[i] = 0;
   __syncthreads();
-  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / 4;
+  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
   if (head > n) head = n;
-  const size_t nvec = (n - head) / 4;
+  const size_t nvec = (n - head) / PER;
   const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
   const size_t stride = (size_t)gridDim.x * kHistBlock;
   for (size_t v = (size_t)blockIdx.x * kHistBlock + tid; v < nvec; v += stride) {
-    uint4 x = vk[v];
-    uint32_t dg[4] = {(x.x >> shift) & mask, (x.y >> shift) & mask, (x.z >> shift) & mask, (x.w >> shift) & mask};
-    lds_count<4>(s_bins, dg);
+    const uint4 x = vk[v];
+    K kk[PER];
+    __builtin_memcpy(kk, &x, 16);
+    uint32_t dg[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) dg[j] = (uint32_t)(kk[j] >> shift) & mask;
+    lds_count<PER>(s_bins, dg);
   }
   if (blockIdx.x == 0) {
-    const size_t tail = head + nvec * 4;
+    const size_t tail = head + nvec * PER;
     const size_t extra = head + (n - tail);
     for (size_t e = tid; e < extra; e += kHistBlock) {
       size_t idx = e < head ? e : tail + (e - head);
-      atomicAdd(&s_bins[(keys[idx] >> shift) & mask], 1u);
+      atomicAdd(&s_bins[(uint32_t)(keys[idx] >> shift) & mask], 1u);
     }
   }
   __syncthreads();
@@ -213,34 +220,52 @@ int pa_hist_digits_u32(const uint32_t* keys, size_t n, uint64_t* counts, pa_stre
   return deliver_u64(c64, counts, 1024, dout, s);
 }
 
-int pa_hist_u32(const uint32_t* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, pa_stream_t stream) {
-  if (!counts || (n && !keys)) return arg_error("pa_hist_u32: null pointer");
-  if (bits < 1 || bits > 12 || shift > 31) return arg_error("pa_hist_u32: need 1 <= bits <= 12 and shift <= 31");
-  if (n > 0xFFFFFFFFull) return arg_error("pa_hist_u32: n must be < 2^32");
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+namespace pa {
+
+template <typename K>
+static int hist_key_entry(const K* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, hipStream_t s,
+                          const char* who) {
+  if (!counts || (n && !keys)) return arg_error((std::string(who) + ": null pointer").c_str());
+  if (bits < 1 || bits > 12 || shift + 1 > 8 * sizeof(K))
+    return arg_error((std::string(who) + ": need 1 <= bits <= 12 and shift < key bits").c_str());
+  if (n > 0xFFFFFFFFull) return arg_error((std::string(who) + ": n must be < 2^32").c_str());
   StreamCtx& ctx = stream_ctx(s);
   std::lock_guard<std::mutex> lk(ctx.mu);
   const uint32_t nb = 1u << bits;
   const bool dk = n == 0 || is_device_ptr(keys);
   const bool dout = is_device_ptr(counts);
   Layout L;
-  size_t o_c32 = L.take(nb * 4), o_c64 = L.take(nb * 8), o_in = dk ? 0 : L.take(n * 4);
+  size_t o_c32 = L.take(nb * 4), o_c64 = L.take(nb * 8), o_in = dk ? 0 : L.take(n * sizeof(K));
   void* base;
   PA_TRY(ensure_scratch(ctx, L.off, &base));
   char* b = (char*)base;
   uint32_t* c32 = (uint32_t*)(b + o_c32);
   uint64_t* c64 = dout ? counts : (uint64_t*)(b + o_c64);
   const void* din;
-  PA_TRY(stage_in(keys, n * 4, b + o_in, dk, s, &din));
+  PA_TRY(stage_in(keys, n * sizeof(K), b + o_in, dk, s, &din));
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, c32, (size_t)nb);
   if (n) {
-    ProfScope ps("k_hist_u32", s);
-    hipLaunchKernelGGL(k_hist_u32, dim3(hist_grid(n / 4 + 1)), dim3(kHistBlock), nb * 4, s, (const uint32_t*)din, n,
-                       (uint32_t)shift, nb - 1, c32);
+    ProfScope ps(sizeof(K) == 4 ? "k_hist_u32" : "k_hist_u64", s);
+    hipLaunchKernelGGL(k_hist_key<K>, dim3(hist_grid(n / (16 / sizeof(K)) + 1)), dim3(kHistBlock), nb * 4, s,
+                       (const K*)din, n, (uint32_t)shift, nb - 1, c32);
   }
   hipLaunchKernelGGL(k_widen, dim3(std::max<unsigned>(1, nb / 256)), dim3(256), 0, s, c32, c64, (size_t)nb);
   PA_HIP(hipGetLastError());
   return deliver_u64(c64, counts, nb, dout, s);
+}
+
+}  // namespace pa
+
+extern "C" {
+
+int pa_hist_u32(const uint32_t* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, pa_stream_t stream) {
+  return hist_key_entry<uint32_t>(keys, n, shift, bits, counts, (hipStream_t)stream, "pa_hist_u32");
+}
+
+int pa_hist_u64(const uint64_t* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, pa_stream_t stream) {
+  return hist_key_entry<uint64_t>(keys, n, shift, bits, counts, (hipStream_t)stream, "pa_hist_u64");
 }
 
 int pa_hist_u8(const uint8_t* bytes, size_t n, uint64_t* counts, pa_stream_t stream) {

@@ -1,11 +1,12 @@
-// pa_select.hip -- radix select (k-th smallest u32) for gfx950.
+// pa_select.hip -- radix select (k-th smallest u32 or u64 key) for gfx950.
 //
 // Reference: SelectRadix / RadixSelectiontNonRecursiveInner (RadixSelect.h:83-168).
 // The reference histograms the current digit over the candidate range
 // (HistogramOneComponentOpt, Histogram.h:103-133), finds the bin holding rank
 // k, moves that bin's keys together (MoveOutsideOfKthBinIn, :36-55), narrows
 // the range to the bin and repeats for digits 24, 16, 8, 0 (same digit order
-// here, so the bin chosen at every pass is the reference's).
+// here, so the bin chosen at every pass is the reference's).  u64 keys use the
+// same scheme over digits 56 .. 0 (an extension; the reference is 32-bit).
 //
 // GPU form (the caller's array is never modified):
 //   k_select_first  one read of all keys builds the 4 digit histograms.  The
@@ -34,23 +35,23 @@ struct SelState {
   uint64_t cand;       // number of keys matching (prefix, pmask)
   uint64_t buf_count;  // keys in the compaction buffer
   uint64_t cap;        // capacity of the compaction buffer
-  uint32_t prefix;     // fixed high digits of the answer
-  uint32_t pmask;      // bits fixed so far
+  uint64_t prefix;     // fixed high digits of the answer
+  uint64_t pmask;      // bits fixed so far
+  uint64_t result;
   int32_t shift;       // digit of the next pass (-8 when done)
   uint32_t done;
   uint32_t src;          // 0: read the caller's keys, 1: read the buffer
   uint32_t compact_now;  // the next pass reading the keys compacts candidates
-  uint32_t result;
-  uint32_t pad;
 };
 
-__global__ void k_select_init(SelState* st, uint64_t k, uint64_t n, uint64_t cap, uint32_t* zero, uint32_t nzero) {
+__global__ void k_select_init(SelState* st, uint64_t k, uint64_t n, uint64_t cap, int32_t top_shift, uint32_t* zero,
+                              uint32_t nzero) {
   if (threadIdx.x == 0) {
     SelState s{};
     s.k = k;
     s.cand = n;
     s.cap = cap;
-    s.shift = 24;
+    s.shift = top_shift;
     *st = s;
   }
   for (uint32_t i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
@@ -89,61 +90,67 @@ __device__ bool last_block(uint32_t* done_ctr, uint32_t* s_flag) {
   return *s_flag;
 }
 
-__global__ __launch_bounds__(kSelBlock) void k_select_first(const uint32_t* __restrict__ keys, size_t n,
+template <typename K>
+__global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict__ keys, size_t n,
                                                             SelState* __restrict__ st, uint32_t* __restrict__ ghist,
                                                             uint32_t* __restrict__ done_ctr) {
-  __shared__ uint32_t s_hist[4 * 256];
+  constexpr int D = sizeof(K);
+  constexpr int PER = 16 / sizeof(K);
+  __shared__ uint32_t s_hist[D * 256];
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_pick[1];
   __shared__ uint64_t s_below;
   __shared__ uint32_t s_flag;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < 4 * 256; i += kSelBlock) s_hist[i] = 0;
+  for (uint32_t i = tid; i < D * 256; i += kSelBlock) s_hist[i] = 0;
   __syncthreads();
-  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / 4;
+  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
   if (head > n) head = n;
-  const size_t nvec = (n - head) / 4;
+  const size_t nvec = (n - head) / PER;
   const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
   const size_t stride = (size_t)gridDim.x * kSelBlock;
   for (size_t v = (size_t)blockIdx.x * kSelBlock + tid; v < nvec; v += stride) {
-    uint4 x = vk[v];
+    const uint4 x = vk[v];
+    K kk[PER];
+    __builtin_memcpy(kk, &x, 16);
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      const uint32_t sh = 8 * d;
-      uint32_t dg[4] = {(x.x >> sh) & 0xFFu, (x.y >> sh) & 0xFFu, (x.z >> sh) & 0xFFu, (x.w >> sh) & 0xFFu};
-      dev::lds_count<4>(s_hist + d * 256, dg);
+    for (int d = 0; d < D; ++d) {
+      uint32_t dg[PER];
+#pragma unroll
+      for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(kk[j], 8 * d);
+      dev::lds_count<PER>(s_hist + d * 256, dg);
     }
   }
   if (blockIdx.x == 0) {
-    const size_t tail = head + nvec * 4;
+    const size_t tail = head + nvec * PER;
     const size_t extra = head + (n - tail);
     for (size_t e = tid; e < extra; e += kSelBlock) {
-      const uint32_t key = keys[e < head ? e : tail + (e - head)];
+      const K key = keys[e < head ? e : tail + (e - head)];
 #pragma unroll
-      for (int d = 0; d < 4; ++d) atomicAdd(&s_hist[d * 256 + ((key >> (8 * d)) & 0xFFu)], 1u);
+      for (int d = 0; d < D; ++d) atomicAdd(&s_hist[d * 256 + dev::digit8<K>(key, 8 * d)], 1u);
     }
   }
   __syncthreads();
-  for (uint32_t i = tid; i < 4 * 256; i += kSelBlock)
+  for (uint32_t i = tid; i < D * 256; i += kSelBlock)
     if (s_hist[i]) atomicAdd(&ghist[i], s_hist[i]);
   if (!last_block(done_ctr, &s_flag)) return;
 
   // Last block: walk the digits from the top while the set is not narrowed.
-  for (uint32_t i = tid; i < 4 * 256; i += kSelBlock) {
+  for (uint32_t i = tid; i < D * 256; i += kSelBlock) {
     s_hist[i] = dev::atomic_load_agent(&ghist[i]);
     ghist[i] = 0;
   }
   __syncthreads();
   uint64_t k = st->k, cand = st->cand;
-  uint32_t prefix = 0, pmask = 0;
-  int shift = 24;
+  uint64_t prefix = 0, pmask = 0;
+  int shift = 8 * (D - 1);
   while (shift >= 0) {
     const int d = shift / 8;
     pick_bin(s_hist + d * 256, k, s_wsum, s_pick, &s_below);
     const uint32_t b = s_pick[0];
     const uint64_t c = s_hist[d * 256 + b];
-    prefix |= b << shift;
-    pmask |= 0xFFu << shift;
+    prefix |= (uint64_t)b << shift;
+    pmask |= (uint64_t)0xFF << shift;
     k -= s_below;
     shift -= 8;
     if (c < cand) {
@@ -167,12 +174,14 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const uint32_t* __re
 
 // One filtered histogram pass at st->shift over keys matching the fixed
 // digits; optionally compacts the matching keys into buf.
-__global__ __launch_bounds__(kSelBlock) void k_select_pass(const uint32_t* __restrict__ keys, size_t n,
-                                                           uint32_t* __restrict__ buf, SelState* __restrict__ st,
+template <typename K>
+__global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__ keys, size_t n,
+                                                           K* __restrict__ buf, SelState* __restrict__ st,
                                                            uint32_t* __restrict__ ghist, uint32_t* __restrict__ done_ctr,
                                                            uint32_t* __restrict__ buf_ctr) {
   __shared__ uint32_t s_hist[256];
-  __shared__ uint32_t s_stage[kStage];
+  constexpr int PER = 16 / sizeof(K);
+  __shared__ K s_stage[kStage];
   __shared__ uint32_t s_stage_n;
   __shared__ uint64_t s_stage_base;
   __shared__ uint32_t s_wsum[4];
@@ -181,11 +190,11 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const uint32_t* __res
   __shared__ uint32_t s_flag;
   if (st->done) return;  // uniform for the whole grid: nobody counts in
   const uint32_t tid = threadIdx.x, lane = tid & 63;
-  const uint32_t prefix = st->prefix, pmask = st->pmask;
+  const K prefix = (K)st->prefix, pmask = (K)st->pmask;
   const uint32_t shift = (uint32_t)st->shift;
   const bool from_buf = st->src != 0;
   const bool compact = !from_buf && st->compact_now;
-  const uint32_t* src = from_buf ? buf : keys;
+  const K* src = from_buf ? buf : keys;
   const size_t m = from_buf ? (size_t)st->buf_count : n;
   if (tid < 256) s_hist[tid] = 0;
   if (tid == 0) s_stage_n = 0;
@@ -193,32 +202,34 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const uint32_t* __res
 
   // Vector part: every thread runs the same number of iterations so the
   // compaction flush can synchronise the block.
-  size_t head = ((16 - ((uintptr_t)src & 15)) & 15) / 4;
+  size_t head = ((16 - ((uintptr_t)src & 15)) & 15) / sizeof(K);
   if (head > m) head = m;
-  const size_t nvec = (m - head) / 4;
+  const size_t nvec = (m - head) / PER;
   const uint4* vk = reinterpret_cast<const uint4*>(src + head);
   const size_t stride = (size_t)gridDim.x * kSelBlock;
   const size_t iters = (nvec + stride - 1) / stride;
   for (size_t it = 0; it < iters; ++it) {
     const size_t v = it * stride + (size_t)blockIdx.x * kSelBlock + tid;
-    uint32_t kk[4] = {0, 0, 0, 0};
+    K kk[PER] = {};
     uint32_t nm = 0;
     if (v < nvec) {
-      uint4 x = vk[v];
-      const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
+      const uint4 x = vk[v];
+      K xs[PER];
+      __builtin_memcpy(xs, &x, 16);
       bool all = true;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < PER; ++j) {
         const bool hit = (xs[j] & pmask) == prefix;
         all = all && hit;
         if (hit) kk[nm++] = xs[j];
       }
       if (all) {  // wave-aggregated counting for runs (presorted, shared digits)
-        uint32_t dg[4] = {(xs[0] >> shift) & 0xFFu, (xs[1] >> shift) & 0xFFu, (xs[2] >> shift) & 0xFFu,
-                          (xs[3] >> shift) & 0xFFu};
-        dev::lds_count<4>(s_hist, dg);
+        uint32_t dg[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(xs[j], shift);
+        dev::lds_count<PER>(s_hist, dg);
       } else {
-        for (uint32_t j = 0; j < nm; ++j) atomicAdd(&s_hist[(kk[j] >> shift) & 0xFFu], 1u);
+        for (uint32_t j = 0; j < nm; ++j) atomicAdd(&s_hist[dev::digit8<K>(kk[j], shift)], 1u);
       }
     }
     if (compact) {
@@ -230,7 +241,7 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const uint32_t* __res
       wbase = __shfl(wbase, 63, 64);
       for (uint32_t j = 0; j < nm; ++j) s_stage[wbase + inc - nm + j] = kk[j];
       __syncthreads();
-      if (s_stage_n > kStage - 4 * kSelBlock || it + 1 == iters) {
+      if (s_stage_n > kStage - PER * kSelBlock || it + 1 == iters) {
         if (tid == 0) s_stage_base = atomicAdd(buf_ctr, s_stage_n);
         __syncthreads();
         const uint32_t cnt = s_stage_n;
@@ -243,12 +254,12 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const uint32_t* __res
   }
   // Head and tail elements outside whole vectors (at most 6): block 0.
   if (blockIdx.x == 0 && tid == 0) {
-    const size_t tail = head + nvec * 4;
+    const size_t tail = head + nvec * PER;
     const size_t extra = head + (m - tail);
     for (size_t e = 0; e < extra; ++e) {
-      const uint32_t key = src[e < head ? e : tail + (e - head)];
+      const K key = src[e < head ? e : tail + (e - head)];
       if ((key & pmask) == prefix) {
-        atomicAdd(&s_hist[(key >> shift) & 0xFFu], 1u);
+        atomicAdd(&s_hist[dev::digit8<K>(key, shift)], 1u);
         if (compact) buf[atomicAdd(buf_ctr, 1u)] = key;
       }
     }
@@ -268,11 +279,11 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const uint32_t* __res
     const uint32_t b = s_pick[0];
     st->k = k - s_below;
     st->cand = s_hist[b];
-    st->prefix = prefix | (b << shift);
-    st->pmask = pmask | (0xFFu << shift);
+    st->prefix = (uint64_t)prefix | ((uint64_t)b << shift);
+    st->pmask = (uint64_t)pmask | ((uint64_t)0xFF << shift);
     st->shift = (int32_t)shift - 8;
     st->done = shift == 0;
-    st->result = prefix | (b << shift);
+    st->result = (uint64_t)prefix | ((uint64_t)b << shift);
     if (compact) {
       st->src = 1;
       st->buf_count = dev::atomic_load_agent(buf_ctr);
@@ -284,10 +295,15 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const uint32_t* __res
 }
 
 // a[k] = the selected value (the reference's post-condition, RadixSelect.h:149).
-__global__ void k_select_place(const SelState* st, uint32_t* dst) { *dst = st->result; }
+template <typename K>
+__global__ void k_select_place(const SelState* st, K* dst) { *dst = (K)st->result; }
 
-static int select_impl(const uint32_t* keys, size_t n, size_t k, uint32_t* result, uint32_t* place, hipStream_t s,
-                       const char* who) {
+template <typename K>
+__global__ void k_select_result(const SelState* st, K* dst) { *dst = (K)st->result; }
+
+template <typename K>
+static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, hipStream_t s, const char* who) {
+  constexpr int D = sizeof(K);
   if (!keys || !result) return arg_error(who);
   if (n == 0 || k >= n) return arg_error("select: need 0 <= k < n");
   if (n > 0xFFFFFFFFull) return arg_error("select: n must be < 2^32");
@@ -298,41 +314,47 @@ static int select_impl(const uint32_t* keys, size_t n, size_t k, uint32_t* resul
   const size_t cap = std::max<size_t>(n / 2, 1);
   Layout L;
   const size_t o_st = L.take(sizeof(SelState));
-  const size_t o_z = L.take((1024 + 256 + 2) * 4);  // ghist4, ghist, done counter, buffer counter
-  const size_t o_buf = L.take(cap * 4);
-  const size_t o_in = dk ? 0 : L.take(n * 4);
+  const size_t nz = D * 256 + 256 + 2;  // ghistD, ghist, done counter, buffer counter
+  const size_t o_z = L.take(nz * 4);
+  const size_t o_res = L.take(sizeof(K));
+  const size_t o_buf = L.take(cap * sizeof(K));
+  const size_t o_in = dk ? 0 : L.take(n * sizeof(K));
   void* base;
   PA_TRY(ensure_scratch(ctx, L.off, &base));
   char* b = (char*)base;
   SelState* st = (SelState*)(b + o_st);
   uint32_t* z = (uint32_t*)(b + o_z);
-  uint32_t* ghist4 = z;
-  uint32_t* ghist = z + 1024;
-  uint32_t* done_ctr = z + 1280;
-  uint32_t* buf_ctr = z + 1281;
-  uint32_t* buf = (uint32_t*)(b + o_buf);
-  const uint32_t* din = keys;
+  uint32_t* ghistD = z;
+  uint32_t* ghist = z + D * 256;
+  uint32_t* done_ctr = ghist + 256;
+  uint32_t* buf_ctr = done_ctr + 1;
+  K* dres_tmp = (K*)(b + o_res);
+  K* buf = (K*)(b + o_buf);
+  const K* din = keys;
   if (!dk) {
-    PA_HIP(hipMemcpyAsync(b + o_in, keys, n * 4, hipMemcpyHostToDevice, s));
-    din = (const uint32_t*)(b + o_in);
+    PA_HIP(hipMemcpyAsync(b + o_in, keys, n * sizeof(K), hipMemcpyHostToDevice, s));
+    din = (const K*)(b + o_in);
   }
-  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, (uint64_t)k, (uint64_t)n, (uint64_t)cap, z, 1282u);
-  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((n / 4 + kSelBlock) / kSelBlock, 1024));
+  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
+                     (int32_t)(8 * (D - 1)), z, (uint32_t)nz);
+  const size_t nvec = n / (16 / sizeof(K));
+  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + kSelBlock) / kSelBlock, 1024));
   {
     ProfScope ps("k_select_first", s);
-    hipLaunchKernelGGL(k_select_first, dim3(grid), dim3(kSelBlock), 0, s, din, n, st, ghist4, done_ctr);
+    hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, st, ghistD, done_ctr);
   }
-  for (int pass = 0; pass < 3; ++pass) {
+  for (int pass = 0; pass < D - 1; ++pass) {
     ProfScope ps("k_select_pass", s);
-    hipLaunchKernelGGL(k_select_pass, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, st, ghist, done_ctr, buf_ctr);
+    hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, st, ghist, done_ctr, buf_ctr);
   }
-  if (place && dk) hipLaunchKernelGGL(k_select_place, dim3(1), dim3(1), 0, s, st, place);
+  if (place && dk) hipLaunchKernelGGL(k_select_place<K>, dim3(1), dim3(1), 0, s, st, place);
+  hipLaunchKernelGGL(k_select_result<K>, dim3(1), dim3(1), 0, s, st, dres_tmp);
   PA_HIP(hipGetLastError());
-  PA_HIP(hipMemcpyAsync(result, &st->result, 4, dres ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  PA_HIP(hipMemcpyAsync(result, dres_tmp, sizeof(K), dres ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   if (!dres || !dk) PA_HIP(hipStreamSynchronize(s));
   if (place && !dk) {  // host keys: store the value into the caller's array
     if (dres)
-      PA_HIP(hipMemcpy(place, &st->result, 4, hipMemcpyDeviceToHost));
+      PA_HIP(hipMemcpy(place, dres_tmp, sizeof(K), hipMemcpyDeviceToHost));
     else
       *place = *result;
   }
@@ -346,12 +368,16 @@ using namespace pa;
 extern "C" {
 
 int pa_select_u32(const uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream) {
-  return select_impl(keys, n, k, result, nullptr, (hipStream_t)stream, "pa_select_u32: null pointer");
+  return select_impl<uint32_t>(keys, n, k, result, nullptr, (hipStream_t)stream, "pa_select_u32: null pointer");
+}
+
+int pa_select_u64(const uint64_t* keys, size_t n, size_t k, uint64_t* result, pa_stream_t stream) {
+  return select_impl<uint64_t>(keys, n, k, result, nullptr, (hipStream_t)stream, "pa_select_u64: null pointer");
 }
 
 int pa_select_radix_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream) {
-  return select_impl(keys, n, k, result, keys ? keys + k : nullptr, (hipStream_t)stream,
-                     "pa_select_radix_u32: null pointer");
+  return select_impl<uint32_t>(keys, n, k, result, keys ? keys + k : nullptr, (hipStream_t)stream,
+                               "pa_select_radix_u32: null pointer");
 }
 
 }  // extern "C"

@@ -71,7 +71,7 @@ static size_t job_bytes(const pa_job& j) {
   const size_t kb = j.key_type == PA_KEY_U64 ? 8 : 4;
   switch (j.op) {
     case PA_OP_SORT: return j.n * kb + j.n / 8192 * 8 * 1024 + (1 << 20);
-    case PA_OP_SELECT: return j.n / 2 * 4 + (1 << 20);
+    case PA_OP_SELECT: return j.n / 2 * kb + (1 << 20);
     default: return 1 << 20;
   }
 }
@@ -86,10 +86,10 @@ static int launch_one(const pa_job& j, hipStream_t s) {
       if (j.key_type == PA_KEY_U64) return pa_sort_u64((uint64_t*)j.keys, nullptr, j.n, s);
       return pa_sort_u32((uint32_t*)j.keys, nullptr, j.n, s);
     case PA_OP_SELECT:
-      if (j.key_type != PA_KEY_U32) return arg_error("server: select supports u32 keys");
+      if (j.key_type == PA_KEY_U64) return pa_select_u64((const uint64_t*)j.keys, j.n, j.k, (uint64_t*)j.out, s);
       return pa_select_u32((const uint32_t*)j.keys, j.n, j.k, (uint32_t*)j.out, s);
     case PA_OP_HIST:
-      if (j.key_type != PA_KEY_U32) return arg_error("server: histogram supports u32 keys");
+      if (j.key_type == PA_KEY_U64) return pa_hist_u64((const uint64_t*)j.keys, j.n, j.shift, j.bits, (uint64_t*)j.out, s);
       return pa_hist_u32((const uint32_t*)j.keys, j.n, j.shift, j.bits, (uint64_t*)j.out, s);
     default:
       return arg_error("server: unknown op");

@@ -51,6 +51,24 @@ def test_hist_one_digit(pa, orc, dist, shift, bits):
     np.testing.assert_array_equal(_host(out), expect)
 
 
+@pytest.mark.parametrize("shift,bits", [(56, 8), (0, 8), (32, 8), (28, 12), (63, 1)])
+@pytest.mark.parametrize("n", [1, 3, 4097, 1_000_003])
+def test_hist_u64(pa, shift, bits, n):
+    """u64 one-digit histogram (an extension: the reference's is 32-bit);
+    the counts are fixed by np.bincount of the digit."""
+    import parallelalgorithms_amd as P
+    a = P.gen_u64(91 + n, n)
+    a[: n // 3] = a[0]  # a constant run for the wave-uniform path
+    expect = np.bincount(((a >> np.uint64(shift)) & np.uint64((1 << bits) - 1)).astype(np.int64),
+                         minlength=1 << bits).astype(np.uint64)
+    out = _dev_u64(1 << bits)
+    pa.hist_u64(to_dev(a), shift, bits, out)
+    np.testing.assert_array_equal(_host(out), expect)
+    host_out = np.zeros(1 << bits, dtype=np.uint64)
+    pa.hist_u64(a, shift, bits, host_out)
+    np.testing.assert_array_equal(host_out, expect)
+
+
 def test_hist_one_digit_host_pointers(pa, orc):
     n = 77_777
     a = make_u32("uniform", n, seed=3)

@@ -47,6 +47,37 @@ def test_select_rejects_bad_k(pa):
         pa.select_u32(a, 10, res)
 
 
+def _make_u64(dist, n, seed=77):
+    import parallelalgorithms_amd as P
+    if dist == "uniform":
+        return P.gen_u64(seed, n)
+    if dist == "presorted":
+        return np.arange(n, dtype=np.uint64) * np.uint64(0x100000001)
+    if dist == "constant":
+        return np.full(n, 0x5A5A5A5A_A5A5A5A5, dtype=np.uint64)
+    if dist == "high":  # only the top bytes vary: every low digit is one bin
+        return (P.gen_u64(seed, n) % np.uint64(5)) << np.uint64(56)
+    raise ValueError(dist)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "presorted", "constant", "high"])
+@pytest.mark.parametrize("n", [1, 5, 4097, 1_000_003])
+def test_select_u64(pa, dist, n):
+    """u64 select has no reference counterpart (SelectRadix is 32-bit): the
+    answer is fixed by the order statistic itself, np.sort(a)[k]."""
+    a = _make_u64(dist, n)
+    srt = np.sort(a)
+    d = to_dev(a)
+    for k in sorted({0, n // 100, n // 2, max(0, n - 21), n - 1}):
+        res = np.zeros(1, dtype=np.uint64)
+        pa.select_u64(d, k, res)
+        assert int(res[0]) == int(srt[k]), (dist, n, k)
+    r = dev_empty(1, np.uint64)
+    pa.select_u64(d, n // 2, r)
+    assert int(to_host(r, np.uint64)[0]) == int(srt[n // 2])
+    np.testing.assert_array_equal(to_host(d, np.uint64), a)
+
+
 def _sorted_u32(seed, n, mod=None):
     import parallelalgorithms_amd as P
     x = P.gen_u32(seed, n)

@@ -19,12 +19,12 @@ def _jobs(seed, count):
         if u < 700:
             kind = "sort64" if (int(r[2 * count + i]) & 1) else "sort32"
         elif u < 900:
-            kind = "select"
+            kind = "select64" if (int(r[2 * count + i]) & 1) else "select"
         else:
-            kind = "hist"
+            kind = "hist64" if (int(r[2 * count + i]) & 1) else "hist"
         if kind == "sort32" and i % 3 == 0:
             n = int(n % 50_000) + 2  # small ones: coalesced into batches
-        keys = P.gen_u64(seed + 7 * i, n) if kind == "sort64" else P.gen_u32(seed + 7 * i, n)
+        keys = P.gen_u64(seed + 7 * i, n) if kind.endswith("64") else P.gen_u32(seed + 7 * i, n)
         k = int(r[3 * count + i] % max(1, n // 10))
         jobs.append((kind, keys, k))
     return jobs
@@ -42,9 +42,15 @@ def test_server_mix_equals_alone(pa, orc):
             if kind == "select":
                 out = dev_empty(1, np.uint32)
                 t = srv.submit(P.OP_SELECT, d, P.KEY_U32, k=k, out=out)
+            elif kind == "select64":
+                out = dev_empty(1, np.uint64)
+                t = srv.submit(P.OP_SELECT, d, P.KEY_U64, k=k, out=out)
             elif kind == "hist":
                 out = dev_empty(256, np.uint64)
                 t = srv.submit(P.OP_HIST, d, P.KEY_U32, shift=24, bits=8, out=out)
+            elif kind == "hist64":
+                out = dev_empty(256, np.uint64)
+                t = srv.submit(P.OP_HIST, d, P.KEY_U64, shift=56, bits=8, out=out)
             else:
                 t = srv.submit(P.OP_SORT, d, P.KEY_U64 if kind == "sort64" else P.KEY_U32)
             dev.append(d)
@@ -70,8 +76,13 @@ def test_server_mix_equals_alone(pa, orc):
         elif kind == "select":
             expect, _ = orc.select_radix_u32(keys, k)
             assert int(to_host(out, np.uint32)[0]) == expect
-        else:
+        elif kind == "select64":
+            assert int(to_host(out, np.uint64)[0]) == int(np.sort(keys)[k])
+        elif kind == "hist":
             np.testing.assert_array_equal(to_host(out, np.uint64), orc.hist_one_byte_u32(keys, 0, keys.size - 1, 24))
+        else:
+            np.testing.assert_array_equal(to_host(out, np.uint64),
+                                          np.bincount((keys >> np.uint64(56)).astype(np.int64), minlength=256))
 
 
 def test_server_rejects_host_memory(pa):
