@@ -210,7 +210,7 @@ static int run_plan(StreamCtx& ctx, const SegPlan& P, char* dev_base, size_t o_t
       const uint64_t* tin = (p & 1) ? d_lt : d_li;
       const uint64_t* tout = (p & 1) ? d_li : d_lt;
       ProfScope ps("k_onesweep_seg", s);
-      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, true, false, 8>), dim3((unsigned)nt),
+      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, kModeSeg, false, 8>), dim3((unsigned)nt),
                          dim3(kSegBlock), 0, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)nullptr,
                          (uint32_t*)nullptr, 0u, counts + p * 256, lbp + (size_t)p * nt * 256, ctr + p,
                          (uint32_t)nt, (uint32_t)(8 * p), 0u, d_ti, tin, tout, d_ll, 4u);

@@ -53,6 +53,205 @@ __global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Chained passes: per-chain digit counts, bases and the interleaved claim order
+// ---------------------------------------------------------------------------
+// Counts row p (p < D) is [kChains][256]: chain-major counts of digit p, where
+// the chain of a key is its position >> log2s for p = 0 (contiguous position
+// ranges of the input) and digit p-1 >> kChainShift for p >= 1.  Pass p's input
+// is sorted by digit p-1, so the keys of value chain c are the contiguous range
+// of that input holding digits [32c, 32c + 32) of digit p-1.  Every row depends
+// only on the multiset of keys, so one read of the input yields all of them.
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_hist_chains(const K* __restrict__ keys, size_t n, uint32_t log2s,
+                                                            uint32_t* __restrict__ partial,
+                                                            uint4* __restrict__ zero_ptr, size_t zero_vec,
+                                                            uint32_t float_keys) {
+  constexpr int D = sizeof(K);
+  constexpr int PER = 16 / sizeof(K);
+  constexpr int ROW = kChains * 256;
+  __shared__ uint32_t s_hist[D * ROW];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < D * ROW; i += kSortBlock) s_hist[i] = 0;
+  const size_t gstride = (size_t)gridDim.x * kSortBlock;
+  for (size_t i = (size_t)blockIdx.x * kSortBlock + tid; i < zero_vec; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+
+  auto count = [&](const K (&k)[PER], const size_t (&pos)[PER]) {
+    uint32_t idx[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) idx[j] = (uint32_t)(pos[j] >> log2s) * 256 + dev::digit8<K>(k[j], 0);
+    dev::lds_count<PER>(s_hist, idx);
+#pragma unroll
+    for (int d = 1; d < D; ++d) {
+#pragma unroll
+      for (int j = 0; j < PER; ++j)
+        idx[j] = (dev::digit8<K>(k[j], 8 * (d - 1)) >> kChainShift) * 256 + dev::digit8<K>(k[j], 8 * d);
+      dev::lds_count<PER>(s_hist + d * ROW, idx);
+    }
+  };
+
+  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
+  if (head > n) head = n;
+  const size_t nvec = (n - head) / PER;
+  const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
+  constexpr int U = 4;  // 16-byte loads in flight per thread
+  const size_t v0 = (size_t)blockIdx.x * kSortBlock + tid;
+  for (size_t base = 0; base < nvec; base += gstride * U) {
+    uint4 x[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t v = base + u * gstride + v0;
+      ok[u] = v < nvec;
+      if (ok[u]) x[u] = vk[v];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      K k[PER];
+      size_t pos[PER];
+      __builtin_memcpy(k, &x[u], 16);
+      const size_t p0 = head + (base + u * gstride + v0) * PER;
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        pos[j] = p0 + j;
+        if (sizeof(K) == 4 && float_keys) k[j] = (K)dev::f32_fwd((uint32_t)k[j]);
+      }
+      count(k, pos);
+    }
+  }
+  if (blockIdx.x == 0) {  // head and tail elements outside whole vectors
+    const size_t tail_start = head + nvec * PER;
+    const size_t extra = head + (n - tail_start);
+    for (size_t e = tid; e < extra; e += kSortBlock) {
+      const size_t idx = e < head ? e : tail_start + (e - head);
+      K kk = keys[idx];
+      if (sizeof(K) == 4 && float_keys) kk = (K)dev::f32_fwd((uint32_t)kk);
+      atomicAdd(&s_hist[(uint32_t)(idx >> log2s) * 256 + dev::digit8<K>(kk, 0)], 1u);
+#pragma unroll
+      for (int d = 1; d < D; ++d)
+        atomicAdd(&s_hist[d * ROW + (dev::digit8<K>(kk, 8 * (d - 1)) >> kChainShift) * 256 + dev::digit8<K>(kk, 8 * d)],
+                  1u);
+    }
+  }
+  __syncthreads();
+  uint32_t* out = partial + (size_t)blockIdx.x * D * ROW;
+  for (uint32_t i = tid; i < D * ROW; i += kSortBlock) out[i] = s_hist[i];
+}
+
+// counts[i] = sum over the histogram blocks of partial[b][i]; blockIdx.y takes
+// a share of the blocks and adds it atomically.
+__global__ __launch_bounds__(256) void k_reduce_counts(const uint32_t* __restrict__ partial, uint32_t nparts,
+                                                       uint32_t ncounts, uint32_t* __restrict__ counts) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ncounts) return;
+  const uint32_t per = (nparts + gridDim.y - 1) / gridDim.y;
+  const uint32_t b0 = blockIdx.y * per, b1 = min(nparts, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; ++b) sum += partial[(size_t)b * ncounts + i];
+  if (sum) atomicAdd(&counts[i], sum);
+}
+
+// One block per pass p: chain bases, chain ranges and the claim order.
+//   bases[p][c][d] = G_p[d] + sum_{c' < c} counts[p][c'][d], where G_p[d] is the
+//   number of keys whose digit p is below d: chain c's keys of digit d follow
+//   those of every earlier chain (pass p's input holds the chains in order and
+//   the pass is stable).
+//   info[p][4c..4c+2] = (first key, keys, first look-back row) of chain c;
+//   info[p][32] = tiles in the pass.
+//   tiles[p][r] = (c << 24) | k: tile k of chain c is claimed r-th, in the order
+//   of (2k + 1) / (2 T_c) (T_c = tiles of chain c), ties by chain, so every
+//   chain's tiles are spread evenly over the pass.
+__global__ __launch_bounds__(1024) void k_chain_plan(const uint32_t* __restrict__ counts, uint32_t n, uint32_t log2s,
+                                                     uint32_t tile, uint32_t* __restrict__ bases,
+                                                     uint32_t* __restrict__ info, uint32_t* __restrict__ tiles,
+                                                     uint32_t max_tiles) {
+  constexpr uint32_t ROW = kChains * 256;
+  __shared__ uint32_t s_prev[257];  // exclusive digit starts of digit p-1, [256] = n
+  __shared__ uint32_t s_wsum[2][4];
+  __shared__ uint32_t s_T[kChains], s_rb[kChains + 1];
+  const uint32_t p = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t* cnt = counts + (size_t)p * ROW;
+  uint32_t tot = 0, tot_prev = 0, inc = 0, inc_prev = 0;
+  if (tid < 256) {
+    for (int c = 0; c < kChains; ++c) tot += cnt[c * 256 + tid];
+    if (p > 0) {
+      const uint32_t* prev = counts + (size_t)(p - 1) * ROW;
+      for (int c = 0; c < kChains; ++c) tot_prev += prev[c * 256 + tid];
+    }
+    inc = dev::wave_incl_scan(tot, lane);
+    inc_prev = dev::wave_incl_scan(tot_prev, lane);
+    if (lane == 63) {
+      s_wsum[0][wave] = inc;
+      s_wsum[1][wave] = inc_prev;
+    }
+  }
+  __syncthreads();
+  if (tid < 256) {
+    uint32_t pre = 0, pre_prev = 0;
+    for (uint32_t w = 0; w < wave; ++w) {
+      pre += s_wsum[0][w];
+      pre_prev += s_wsum[1][w];
+    }
+    uint32_t b = pre + inc - tot;  // G_p[tid]
+    for (int c = 0; c < kChains; ++c) {
+      bases[(size_t)p * ROW + c * 256 + tid] = b;
+      b += cnt[c * 256 + tid];
+    }
+    s_prev[tid] = pre_prev + inc_prev - tot_prev;
+    if (tid == 0) s_prev[256] = (uint32_t)n;
+  }
+  __syncthreads();
+  if (tid < kChains) {
+    uint32_t first, len;
+    if (p == 0) {
+      const uint64_t f = (uint64_t)tid << log2s;
+      const uint64_t cap = (uint64_t)1 << log2s;
+      first = f < n ? (uint32_t)f : n;
+      len = (uint64_t)n - first < cap ? n - first : (uint32_t)cap;
+    } else {
+      first = s_prev[tid << kChainShift];
+      len = s_prev[(tid + 1) << kChainShift] - first;
+    }
+    info[(size_t)p * 36 + 4 * tid] = first;
+    info[(size_t)p * 36 + 4 * tid + 1] = len;
+    s_T[tid] = (len + tile - 1) / tile;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t r = 0;
+    for (int c = 0; c < kChains; ++c) {
+      s_rb[c] = r;
+      info[(size_t)p * 36 + 4 * c + 2] = r;
+      info[(size_t)p * 36 + 4 * c + 3] = 0;
+      r += s_T[c];
+    }
+    s_rb[kChains] = r;
+    info[(size_t)p * 36 + 32] = r;
+  }
+  __syncthreads();
+  const uint32_t total = s_rb[kChains];
+  uint32_t* order = tiles + (size_t)p * max_tiles;
+  for (uint32_t t = tid; t < total; t += blockDim.x) {
+    uint32_t c = 0;
+    while (t >= s_rb[c + 1]) ++c;
+    const uint32_t k = t - s_rb[c];
+    const uint64_t Tc = s_T[c];
+    uint32_t rank = k;
+    for (uint32_t c2 = 0; c2 < (uint32_t)kChains; ++c2) {
+      if (c2 == c || s_T[c2] == 0) continue;
+      // tiles k2 of chain c2 with (2k2 + 1) * Tc < (2k + 1) * T_c2, plus the
+      // tie (2k2 + 1) * Tc == (2k + 1) * T_c2 when c2 < c
+      const uint64_t X = (uint64_t)(2 * k + 1) * s_T[c2];
+      uint64_t before = ((X + Tc - 1) / Tc) / 2;
+      if (c2 < c && X % Tc == 0 && ((X / Tc) & 1)) ++before;
+      rank += before < s_T[c2] ? (uint32_t)before : s_T[c2];
+    }
+    order[rank] = (c << 24) | k;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Pass variants (tile shape, look-back width, persistent blocks), selectable
 // at run time; PA_SORT_VARIANT overrides the default (entry 0).
 // ---------------------------------------------------------------------------
@@ -65,71 +264,58 @@ struct PassArgs {
   const uint32_t* gtab;
   uint32_t* lb;
   uint32_t* ctr;
-  uint32_t ntiles;
+  uint32_t ntiles;  // plain: tiles; chained: launch bound (tiles + kChains)
   uint32_t shift;
   uint32_t fmode;
+  const uint32_t* chain_info;
+  const uint32_t* chain_tiles;
 };
 
 struct PassVariant {
-  int block, items, lbw, pipe, minw;
+  int block, items, lbw, late, minw, chained;
   int (*launch)(hipStream_t s, const PassArgs& a);
 };
 
-// Resident blocks of a kernel on the current device (persistent launches).
-template <typename F>
-unsigned resident_blocks(F kernel, int block) {
-  int dev = 0, cus = 0, per_cu = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
-  (void)hipGetLastError();
-  return (unsigned)std::max(1, cus * std::max(1, per_cu));
-}
-
-template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool PIPE, int MINW>
+template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool LATE, int MINW, int MODE, bool UNI>
 int launch_pass(hipStream_t s, const PassArgs& a) {
-  auto kern = k_onesweep<K, KV, BLOCK, ITEMS, LBW, false, PIPE, MINW>;
-  unsigned grid = a.ntiles;
-  if (PIPE) {
-    static const unsigned resident = resident_blocks(kern, BLOCK);  // one device per process is assumed
-    grid = std::min(grid, resident);
-  }
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, static_cast<const K*>(a.kin), static_cast<K*>(a.kout), a.vin,
-                     a.vout, a.n, a.gtab, a.lb, a.ctr, a.ntiles, a.shift, a.fmode, nullptr, nullptr, nullptr, nullptr,
-                     0u);
+  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI>), dim3(a.ntiles), dim3(BLOCK), 0, s,
+                     static_cast<const K*>(a.kin), static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr,
+                     a.ntiles, a.shift, a.fmode, nullptr, nullptr, nullptr, nullptr, 0u, a.chain_info, a.chain_tiles);
   PA_HIP(hipGetLastError());
   return PA_OK;
 }
 
-#define PA_VARIANT(K, KV, B, I, W, P, M) \
-  { B, I, W, P, M, &launch_pass<K, KV, B, I, W, P, M> }
+// PA_VARIANT(key, payload, block, items, look-back width, late look-back, min waves/SIMD, chained,
+//            uniform-digit fast path)
+#define PA_VARIANT(K, KV, B, I, W, P, M, C, U) \
+  { B, I, W, P, M, C, &launch_pass<K, KV, B, I, W, P, M, (C) ? kModeChain : kModePlain, U> }
 
 template <typename K, bool KV>
 struct Variants;
 template <>
 struct Variants<uint32_t, false> {
   static constexpr PassVariant table[] = {
-      PA_VARIANT(uint32_t, false, 512, 24, 4, false, 6), PA_VARIANT(uint32_t, false, 512, 24, 8, false, 6),
-      PA_VARIANT(uint32_t, false, 512, 24, 2, false, 6), PA_VARIANT(uint32_t, false, 512, 20, 4, false, 6),
-      PA_VARIANT(uint32_t, false, 256, 32, 4, false, 4), PA_VARIANT(uint32_t, false, 256, 48, 4, false, 3),
-      PA_VARIANT(uint32_t, false, 512, 32, 4, false, 4), PA_VARIANT(uint32_t, false, 512, 24, 4, true, 4),
+      PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 1, true),
+      PA_VARIANT(uint32_t, false, 512, 24, 4, false, 6, 0, true), PA_VARIANT(uint32_t, false, 512, 32, 4, true, 4, 0, true),
+      PA_VARIANT(uint32_t, false, 512, 24, 8, true, 6, 0, true),  PA_VARIANT(uint32_t, false, 512, 24, 2, true, 6, 0, true),
+      PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, false),
   };
 };
 template <>
 struct Variants<uint32_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 8, false, 5),
-                                          PA_VARIANT(uint32_t, true, 512, 16, 8, false, 4)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 5, 0, true),
+                                          PA_VARIANT(uint32_t, true, 512, 16, 4, true, 5, 1, true)};
 };
 template <>
 struct Variants<uint64_t, false> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 8, false, 8),
-                                          PA_VARIANT(uint64_t, false, 512, 12, 8, false, 6),
-                                          PA_VARIANT(uint64_t, false, 512, 8, 8, true, 4)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 0, true),
+                                          PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true),
+                                          PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true)};
 };
 template <>
 struct Variants<uint64_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 8, false, 6),
-                                          PA_VARIANT(uint64_t, true, 512, 8, 8, true, 4)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 0, true),
+                                          PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 1, true)};
 };
 
 template <typename K, bool KV>
@@ -143,23 +329,40 @@ const PassVariant& pick_variant() {
   return Variants<K, KV>::table[idx];
 }
 
+// Look-back rows (and launch bound) of one pass: every chain rounds up to
+// whole tiles.
 template <typename K, bool KV>
-size_t sort_tiles(size_t n) {
+size_t sort_rows(size_t n) {
   const PassVariant& v = pick_variant<K, KV>();
   const size_t tile = (size_t)v.block * v.items;
-  return (n + tile - 1) / tile;
+  return (n + tile - 1) / tile + (v.chained ? kChains : 0);
 }
 
-// Enqueues the full sort on device buffers.  scratch must hold
-// sort_scratch_bytes<K, KV>(n) bytes.
+// Pass-0 chains are position ranges of 2^log2s keys (at least 2^16, at most
+// kChains of them).
+inline uint32_t chain_log2s(size_t n) {
+  uint32_t l = 16;
+  while ((size_t(1) << l) * kChains < n) ++l;
+  return l;
+}
+
+constexpr unsigned kHistChainBlocks = 512;
+
+// Scratch of one sort call (see enqueue_sort for the layout).
 template <typename K, bool KV>
 size_t sort_scratch_bytes(size_t n) {
   constexpr int D = sizeof(K);
+  const size_t rows = sort_rows<K, KV>(n);
   Layout L;
-  L.take((size_t)D * 256 * 4);  // counts
-  L.take((size_t)D * 256 * 4);  // bases
-  L.take(D * 4);                // tile counters
-  L.take((size_t)D * sort_tiles<K, KV>(n) * 256 * 4);
+  L.take((size_t)D * kChains * 256 * 4);  // counts
+  L.take((size_t)D * kChains * 256 * 4);  // bases
+  L.take(D * 4);                          // tile counters
+  L.take((size_t)D * rows * 256 * 4);     // look-back table
+  if (pick_variant<K, KV>().chained) {
+    L.take((size_t)D * 36 * 4);                                  // chain tables
+    L.take((size_t)D * rows * 4);                                // claim orders
+    L.take((size_t)kHistChainBlocks * D * kChains * 256 * 4);    // histogram partials
+  }
   return L.off;
 }
 
@@ -168,29 +371,48 @@ int enqueue_sort(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, cha
                  bool float_keys) {
   constexpr int D = sizeof(K);
   const PassVariant& var = pick_variant<K, KV>();
-  const size_t tiles = sort_tiles<K, KV>(n);
-  const size_t ncount = (size_t)D * 256;
+  const size_t rows = sort_rows<K, KV>(n);
+  const size_t ncount = (size_t)D * kChains * 256;
   Layout L;
   uint32_t* counts = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
   uint32_t* bases = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
   uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch + L.take(D * 4));
-  const size_t lb_bytes = (size_t)D * tiles * 256 * 4;
+  const size_t lb_bytes = (size_t)D * rows * 256 * 4;
   uint32_t* lb = reinterpret_cast<uint32_t*>(scratch + L.take(lb_bytes));
+  uint32_t* chain_info = nullptr;
+  uint32_t* chain_tiles = nullptr;
+  uint32_t* partial = nullptr;
+  if (var.chained) {
+    chain_info = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)D * 36 * 4));
+    chain_tiles = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)D * rows * 4));
+    partial = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)kHistChainBlocks * ncount * 4));
+  }
 
   {
     ProfScope ps("k_sort_prepare", s);
     hipLaunchKernelGGL(k_sort_prepare, dim3(1), dim3(256), 0, s, counts, (uint32_t)ncount, ctr, (uint32_t)D);
   }
-  {
+  const size_t tile = (size_t)var.block * var.items;
+  if (var.chained) {
+    const uint32_t log2s = chain_log2s(n);
+    {
+      ProfScope ps("k_hist_digits", s);
+      hipLaunchKernelGGL(k_hist_chains<K>, dim3(kHistChainBlocks), dim3(kSortBlock), 0, s, keys, n, log2s, partial,
+                         reinterpret_cast<uint4*>(lb), lb_bytes / 16, (uint32_t)float_keys);
+    }
+    hipLaunchKernelGGL(k_reduce_counts, dim3((unsigned)((ncount + 255) / 256), 16), dim3(256), 0, s, partial,
+                       kHistChainBlocks, (uint32_t)ncount, counts);
+    hipLaunchKernelGGL(k_chain_plan, dim3(D), dim3(1024), 0, s, counts, (uint32_t)n, log2s, (uint32_t)tile, bases,
+                       chain_info, chain_tiles, (uint32_t)rows);
+  } else {
     const size_t nvec = n / (16 / sizeof(K)) + 1;
     const size_t grid = std::max<size_t>(std::min<size_t>((nvec + kSortBlock - 1) / kSortBlock, 2048),
                                          std::min<size_t>(lb_bytes / 16 / kSortBlock + 1, 2048));
-    ProfScope ps("k_hist_digits", s);
-    hipLaunchKernelGGL(k_hist_digits<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, counts,
-                       reinterpret_cast<uint4*>(lb), lb_bytes / 16, (uint32_t)float_keys);
-  }
-  {
-    ProfScope ps("k_digit_bases", s);
+    {
+      ProfScope ps("k_hist_digits", s);
+      hipLaunchKernelGGL(k_hist_digits<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, counts,
+                         reinterpret_cast<uint4*>(lb), lb_bytes / 16, (uint32_t)float_keys);
+    }
     hipLaunchKernelGGL(k_digit_bases, dim3(D), dim3(256), 0, s, counts, bases);
   }
   K* kin = keys;
@@ -201,8 +423,10 @@ int enqueue_sort(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, cha
     uint32_t fmode = 0;
     if (float_keys && p == 0) fmode |= kFloatIn;
     if (float_keys && p == D - 1) fmode |= kFloatOut;
-    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * 256, lb + (size_t)p * tiles * 256,
-               ctr + p, (uint32_t)tiles, (uint32_t)(8 * p), fmode};
+    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * (var.chained ? kChains * 256 : 256),
+               lb + (size_t)p * rows * 256, ctr + p, (uint32_t)rows, (uint32_t)(8 * p), fmode,
+               var.chained ? chain_info + (size_t)p * 36 : nullptr,
+               var.chained ? chain_tiles + (size_t)p * rows : nullptr};
     ProfScope ps("k_onesweep", s);
     PA_TRY(var.launch(s, a));
     std::swap(kin, kout);

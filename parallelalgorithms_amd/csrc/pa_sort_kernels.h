@@ -16,6 +16,15 @@ constexpr size_t kMaxSortN = (size_t(1) << 30) - 1;
 
 enum : uint32_t { kFloatIn = 1u, kFloatOut = 2u };
 
+// Tile-to-array modes of the onesweep pass.
+enum : int {
+  kModePlain = 0,  // one array, tiles in order, one look-back chain
+  kModeSeg = 1,    // independent segments (segmented batch sort)
+  kModeChain = 2,  // one array split into interleaved look-back chains
+};
+constexpr int kChains = 8;            // look-back chains of a chained pass
+constexpr uint32_t kChainShift = 5;  // chain of a key in pass p >= 1: digit p-1 >> 5
+
 
 // One read of the keys -> D digit histograms (D = sizeof(K)), counts laid out
 // [D][256].  Also clears the look-back table of the onesweep passes that
@@ -134,23 +143,36 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 // array (or segment) is partial, and no tile looks back past the first tile
 // of its own array, so the padded count it publishes is never consumed.
 //
-// Segmented mode (SEG): tile_info[t] = (segment << 32) | tile index within
-// the segment; the segment's input/output addresses and length come from
-// seg_in / seg_out / seg_len and its digit counts from gtab + segment *
-// gstride * 256.  Plain mode: gtab holds the pass's exclusive digit bases.
+// Segmented mode: tile_info[t] = (segment << 32) | tile index within the
+// segment; the segment's input/output addresses and length come from seg_in /
+// seg_out / seg_len and its digit counts from gtab + segment * gstride * 256.
+// Plain mode: gtab holds the pass's exclusive digit bases.
 //
-// PIPE: persistent blocks.  After ranking tile t the block claims the next
-// tile and issues its loads, so they are in flight while tile t waits in the
-// look-back, is staged and is written out.
+// Chain mode: the array is split into kChains contiguous chains, each with its
+// own look-back rows, and claims interleave the chains (chain_tiles[i] =
+// (chain << 24) | tile within the chain), so consecutive tiles of one chain
+// are claimed several tiles apart and a tile mostly finds its predecessor's
+// inclusive prefix already published.  chain_info[4c + 0..2] = (first key,
+// keys, first look-back row) of chain c, chain_info[32] = tiles in the pass;
+// gtab + 256c holds chain c's output base per digit.
+//
+// LATE: the look-back's first round trip is issued while the block scatters
+// its keys into LDS and read afterwards, so the status loads are in flight
+// during the scatter.
 //
 // MINW: minimum waves per SIMD the register allocation must allow.
-template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool SEG = false, bool PIPE = false, int MINW = 1>
+// UNI: items whose 64 lanes all hold one digit take a path without LDS atomics.
+template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, int MODE = kModePlain, bool LATE = false, int MINW = 1,
+          bool UNI = true>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void k_onesweep(
     const K* __restrict__ kin, K* __restrict__ kout, const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
     uint32_t n, const uint32_t* __restrict__ gtab, uint32_t* __restrict__ lb, uint32_t* __restrict__ tile_counter,
     uint32_t ntiles, uint32_t shift, uint32_t fmode, const uint64_t* __restrict__ tile_info = nullptr,
     const uint64_t* __restrict__ seg_in = nullptr, const uint64_t* __restrict__ seg_out = nullptr,
-    const uint32_t* __restrict__ seg_len = nullptr, uint32_t gstride = 0) {
+    const uint32_t* __restrict__ seg_len = nullptr, uint32_t gstride = 0,
+    const uint32_t* __restrict__ chain_info = nullptr, const uint32_t* __restrict__ chain_tiles = nullptr) {
+  constexpr bool SEG = MODE == kModeSeg;
+  constexpr bool CHAIN = MODE == kModeChain;
   static_assert(BLOCK >= 256 && BLOCK <= 1024 && BLOCK % 64 == 0, "one thread per digit is needed");
   static_assert(!(SEG && KV), "segmented mode sorts keys only");
   constexpr int UNITS = BLOCK / 32;
@@ -160,7 +182,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   __shared__ __align__(16) K s_keys[TILE];
   __shared__ uint32_t s_goff[256];
   __shared__ uint32_t s_wsum[2][4];
-  __shared__ uint32_t s_claim[2];
+  __shared__ uint32_t s_claim;
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
@@ -177,105 +199,110 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   const bool float_in = sizeof(K) == 4 && (fmode & kFloatIn);
   const bool float_out = sizeof(K) == 4 && (fmode & kFloatOut);
 
-  // Per-tile geometry.
-  struct Tile {
-    const K* in;
-    K* out;
-    const uint32_t* vin;
-    uint32_t* vout;
-    const uint32_t* g;  // digit bases (plain) or segment digit counts (SEG)
-    uint32_t len;       // keys in the tile's array
-    uint32_t row;       // look-back row of the tile
-    uint32_t local;     // tile index within its array
-  };
-  auto make_tile = [&](uint32_t claimed) {
-    Tile t;
-    t.row = claimed;
-    if constexpr (SEG) {
-      const uint64_t info = tile_info[claimed];
-      const uint32_t seg = (uint32_t)(info >> 32);
-      t.local = (uint32_t)info;
-      t.in = reinterpret_cast<const K*>(seg_in[seg]);
-      t.out = reinterpret_cast<K*>(seg_out[seg]);
-      t.vin = nullptr;
-      t.vout = nullptr;
-      t.len = seg_len[seg];
-      t.g = gtab + (size_t)seg * gstride * 256;
-    } else {
-      t.local = claimed;
-      t.in = kin;
-      t.out = kout;
-      t.vin = vin;
-      t.vout = vout;
-      t.len = n;
-      t.g = gtab;
-    }
-    return t;
-  };
-  // Keys of a tile into registers; all-ones padding past the array's end.
-  auto load_keys = [&](const Tile& t, K (&k)[ITEMS]) {
-    const uint32_t first = t.local * (uint32_t)TILE + unit * USPAN + ul;
-    if (t.local * (uint32_t)TILE + TILE <= t.len) {
-      const K* src = t.in + first;
+  // Clear the rank slots while the claim is in flight.
+  {
+    uint4* z = reinterpret_cast<uint4*>(s_keys);
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) k[i] = src[i * 32];
+    for (int r = 0; r < UNITS * 256 * 8 / 16 / BLOCK; ++r) z[tid + r * BLOCK] = make_uint4(0, 0, 0, 0);
+  }
+  if (CHAIN) ntiles = chain_info[32];
+  if (tid == 0) s_claim = atomicAdd(tile_counter, 1u);
+  __syncthreads();
+  // Uniform across the block: the tile geometry below lives in scalar registers.
+  const uint32_t claimed = __builtin_amdgcn_readfirstlane(s_claim);
+  if (claimed >= ntiles) return;
+
+  // Geometry of the claimed tile.
+  const K* in;
+  K* out;
+  const uint32_t* vsrc = nullptr;
+  uint32_t* vdst = nullptr;
+  const uint32_t* g;  // digit bases (plain, chain) or segment digit counts (SEG)
+  uint32_t len;       // keys in the tile's array (segment, chain)
+  uint32_t row = claimed;  // look-back row of the tile
+  uint32_t local;          // tile index within its array (segment, chain)
+  if constexpr (CHAIN) {
+    const uint32_t info = chain_tiles[claimed];
+    const uint32_t c = info >> 24;
+    local = info & 0xFFFFFFu;
+    const uint32_t first = chain_info[4 * c];
+    in = kin + first;
+    out = kout;
+    if constexpr (KV) {
+      vsrc = vin + first;
+      vdst = vout;
+    }
+    len = chain_info[4 * c + 1];
+    row = chain_info[4 * c + 2] + local;
+    g = gtab + 256 * c;
+  } else if constexpr (SEG) {
+    const uint64_t info = tile_info[claimed];
+    const uint32_t seg = (uint32_t)(info >> 32);
+    local = (uint32_t)info;
+    in = reinterpret_cast<const K*>(seg_in[seg]);
+    out = reinterpret_cast<K*>(seg_out[seg]);
+    len = seg_len[seg];
+    g = gtab + (size_t)seg * gstride * 256;
+  } else {
+    local = claimed;
+    in = kin;
+    out = kout;
+    vsrc = vin;
+    vdst = vout;
+    len = n;
+    g = gtab;
+  }
+  const uint32_t tile_first = local * (uint32_t)TILE;
+  const uint32_t valid = min((uint32_t)TILE, len - tile_first);
+
+  // Keys into registers, all-ones padding past the array's end.
+  K key[ITEMS];
+  {
+    const uint32_t first = tile_first + unit * USPAN + ul;
+    const K* src = in + first;
+    if (valid == (uint32_t)TILE) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) key[i] = src[i * 32];
       if (float_in) {
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) k[i] = (K)dev::f32_fwd((uint32_t)k[i]);
+        for (int i = 0; i < ITEMS; ++i) key[i] = (K)dev::f32_fwd((uint32_t)key[i]);
       }
     } else {
-      const K* src = t.in + first;
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) {
         K v = ~K(0);
-        if (first + i * 32 < t.len) {
+        if (first + i * 32 < len) {
           v = src[i * 32];
           if (float_in) v = (K)dev::f32_fwd((uint32_t)v);
         }
-        k[i] = v;
+        key[i] = v;
       }
     }
-  };
-
-  if (tid == 0) s_claim[0] = atomicAdd(tile_counter, 1u);
-  __syncthreads();
-  uint32_t claimed = s_claim[0];
-  if (claimed >= ntiles) return;  // the same for every thread of the block
-  Tile cur = make_tile(claimed);
-  K key[ITEMS];
-  load_keys(cur, key);
-
-  for (;;) {
-    // An opaque copy of the thread index: the per-item indices derived from it
-    // below are recomputed each iteration instead of being hoisted out of the
-    // tile loop (and spilled).
-    uint32_t tix = tid;
-    asm volatile("" : "+v"(tix));
-    const uint32_t tile_first = cur.local * (uint32_t)TILE;
-    const uint32_t valid = min((uint32_t)TILE, cur.len - tile_first);
-    uint32_t val[KV ? ITEMS : 1];
-    if constexpr (KV) {
-      const uint32_t first = tile_first + unit * USPAN + ul;
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) {
-        const uint32_t idx = first + i * 32;
-        val[i] = idx < cur.len ? cur.vin[idx] : 0u;
-      }
-    }
-    // Clear the rank slots.
-    {
-      uint4* z = reinterpret_cast<uint4*>(s_keys);
-#pragma unroll
-      for (int r = 0; r < UNITS * 256 * 8 / 16 / BLOCK; ++r) z[tid + r * BLOCK] = make_uint4(0, 0, 0, 0);
-    }
-    if (PIPE && tid == 0) s_claim[1] = atomicAdd(tile_counter, 1u);
-    __syncthreads();
-
-    // Stable rank of every key within its unit.
-    uint32_t rank[ITEMS];
+  }
+  uint32_t val[KV ? ITEMS : 1];
+  if constexpr (KV) {
+    const uint32_t first = tile_first + unit * USPAN + ul;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t d = dev::digit8<K>(key[i], shift);
+      const uint32_t idx = first + i * 32;
+      val[i] = idx < len ? vsrc[idx] : 0u;
+    }
+  }
+
+  // Stable rank of every key within its unit.
+  uint32_t rank[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t d = dev::digit8<K>(key[i], shift);
+    if (UNI && __all(d == __builtin_amdgcn_readfirstlane(d))) {
+      // Every lane holds the same digit (constant or presorted runs): the
+      // peers of each unit are its 32 lanes, so no lane mask is needed and 32
+      // lanes do not serialise on one LDS atomic.
+      const uint32_t seen = uword[2 * d + 1];
+      rank[i] = seen + ul;
+      asm volatile("" : "+v"(rank[i]));
+      if (ul == 0) uword[2 * d + 1] = seen + 32u;
+    } else {
       atomicOr(&uword[2 * d], lanebit);
       const uint64_t sv = uslot[d];
       const uint32_t peers = (uint32_t)sv;
@@ -286,148 +313,145 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       asm volatile("" : "+v"(rank[i]));
       if (below == 0) uslot[d] = (uint64_t)(seen + __popc(peers)) << 32;
     }
-    __syncthreads();
+  }
+  __syncthreads();
 
-    // Next tile's keys go in flight now (persistent blocks).
-    Tile nxt = cur;
-    bool more = false;
-    K knext[PIPE ? ITEMS : 1];
-    if constexpr (PIPE) {
-      const uint32_t c = s_claim[1];
-      more = c < ntiles;
-      if (more) {
-        nxt = make_tile(c);
-        load_keys(nxt, knext);
-      }
+  // Per digit: prefix over units, tile count, publish the aggregate, scans
+  // over the 256 digits of the tile counts (and, in SEG mode, of the
+  // segment's digit counts).
+  uint32_t cnt = 0, gval = 0, cinc = 0, ginc = 0;
+  if (tid < 256) {
+#pragma unroll
+    for (int u = 0; u < UNITS; ++u) {
+      const uint32_t c = hiword[2 * (u * 256 + tid)];
+      hiword[2 * (u * 256 + tid)] = cnt;
+      cnt += c;
     }
-
-    // Per digit: prefix over units, tile count, publish the aggregate, scans
-    // over the 256 digits of the tile counts (and, in SEG mode, of the
-    // segment's digit counts).
-    uint32_t cnt = 0, gval = 0, cinc = 0, ginc = 0;
-    if (tid < 256) {
-#pragma unroll
-      for (int u = 0; u < UNITS; ++u) {
-        const uint32_t c = hiword[2 * (u * 256 + tid)];
-        hiword[2 * (u * 256 + tid)] = cnt;
-        cnt += c;
-      }
-      dev::atomic_store_agent(&lb[(size_t)cur.row * 256 + tid], (cur.local == 0 ? kLbInc : kLbAgg) | cnt);
-      if constexpr (SEG) gval = cur.g[tid];
-      cinc = dev::wave_incl_scan(cnt, lane);
-      if constexpr (SEG) ginc = dev::wave_incl_scan(gval, lane);
-      if (lane == 63) {
-        s_wsum[0][wave] = cinc;
-        s_wsum[1][wave] = ginc;
-      }
+    dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
+    if constexpr (SEG) gval = g[tid];
+    cinc = dev::wave_incl_scan(cnt, lane);
+    if constexpr (SEG) ginc = dev::wave_incl_scan(gval, lane);
+    if (lane == 63) {
+      s_wsum[0][wave] = cinc;
+      s_wsum[1][wave] = ginc;
     }
-    __syncthreads();
-    if (tid < 256) {
-      uint32_t cpre = 0, gpre = 0;
-      for (uint32_t w = 0; w < wave; ++w) {
-        cpre += s_wsum[0][w];
-        gpre += s_wsum[1][w];
-      }
-      const uint32_t tstart = cpre + cinc - cnt;  // first staging slot of this digit
-      const uint32_t gstart = SEG ? gpre + ginc - gval : cur.g[tid];
-#pragma unroll
-      for (int u = 0; u < UNITS; ++u) hiword[2 * (u * 256 + tid)] += tstart;
+  }
+  __syncthreads();
 
-      // Decoupled look-back: LBW status words per round trip; rows before the
-      // array's first tile read as "inclusive 0".
-      uint32_t excl = 0;
-      if (cur.local > 0) {
-        const uint32_t first_row = cur.row - cur.local;
-        uint32_t p = cur.row - 1;
-        for (;;) {
-          const uint32_t avail = p - first_row + 1;
-          uint32_t w[LBW];
+  // Decoupled look-back over earlier tiles of the array, LBW status words per
+  // round trip; rows before the array's first tile read as "inclusive 0".
+  uint32_t tstart = 0, gstart = 0, excl = 0;
+  const uint32_t first_row = row - local;
+  uint32_t p = row - 1;
+  uint32_t w[LBW];
+  auto poll = [&]() {
+    const uint32_t avail = p - first_row + 1;
+    const uint32_t* at = lb + (size_t)p * 256 + tid;  // one address, immediate offsets
 #pragma unroll
-          for (int j = 0; j < LBW; ++j)
-            w[j] = (uint32_t)j < avail ? dev::atomic_load_agent(&lb[(size_t)(p - j) * 256 + tid]) : kLbInc;
-          uint32_t used = 0;
-          bool stop = false, done = false;
+    for (int j = 0; j < LBW; ++j) w[j] = (uint32_t)j < avail ? dev::atomic_load_agent(at - j * 256) : kLbInc;
+  };
+  auto consume = [&]() {  // true once an inclusive prefix is reached
+    uint32_t used = 0;
+    bool stop = false, done = false;
 #pragma unroll
-          for (int j = 0; j < LBW; ++j) {
-            if (!stop) {
-              const uint32_t flag = w[j] & ~kLbMask;
-              if (flag == 0) {
-                stop = true;  // not published yet: poll again from here
-              } else {
-                excl += w[j] & kLbMask;
-                ++used;
-                if (flag == kLbInc) stop = done = true;
-              }
-            }
-          }
-          if (done) break;
-          p -= used;
-          if (used == 0) __builtin_amdgcn_s_sleep(1);
-        }
-        dev::atomic_store_agent(&lb[(size_t)cur.row * 256 + tid], kLbInc | (excl + cnt));
-      }
-      s_goff[tid] = gstart + excl - tstart;
-    }
-    __syncthreads();
-
-    // Staging slot of every key: its unit's start for the digit + its rank.
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) rank[i] += hiword[2 * (unit * 256 + dev::digit8<K>(key[i], shift))];
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) s_keys[rank[i]] = key[i];
-    __syncthreads();
-
-    // Coalesced write-out: consecutive slots of one digit go to consecutive
-    // addresses of that digit's bin.  Groups of G items keep G LDS reads in
-    // flight without holding every item's address in registers.
-    constexpr int G = ITEMS % 8 == 0 ? 8 : (ITEMS % 4 == 0 ? 4 : 1);
-    uint32_t gpos[KV ? ITEMS : 1];
-    auto write_out = [&](auto fo) {
-#pragma unroll
-      for (int g0 = 0; g0 < ITEMS; g0 += G) {
-        K wk[G];
-        uint32_t gp[G];
-#pragma unroll
-        for (int i = 0; i < G; ++i) wk[i] = s_keys[tix + (g0 + i) * BLOCK];
-#pragma unroll
-        for (int i = 0; i < G; ++i) {
-          gp[i] = s_goff[dev::digit8<K>(wk[i], shift)] + tix + (g0 + i) * BLOCK;
-          if constexpr (KV) gpos[g0 + i] = gp[i];
-          if constexpr (decltype(fo)::value) wk[i] = (K)dev::f32_inv((uint32_t)wk[i]);
-        }
-        if (valid == (uint32_t)TILE) {
-#pragma unroll
-          for (int i = 0; i < G; ++i) cur.out[gp[i]] = wk[i];
+    for (int j = 0; j < LBW; ++j) {
+      if (!stop) {
+        const uint32_t flag = w[j] & ~kLbMask;
+        if (flag == 0) {
+          stop = true;  // not published yet: poll again from here
         } else {
-#pragma unroll
-          for (int i = 0; i < G; ++i)
-            if (tix + (g0 + i) * BLOCK < valid) cur.out[gp[i]] = wk[i];
+          excl += w[j] & kLbMask;
+          ++used;
+          if (flag == kLbInc) stop = done = true;
         }
       }
-    };
-    if (float_out)
-      write_out(std::true_type{});
-    else
-      write_out(std::false_type{});
-    if constexpr (KV) {
-      __syncthreads();
-      uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) s_vals[rank[i]] = val[i];
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i)
-        if (tix + i * BLOCK < valid) cur.vout[gpos[i]] = s_vals[tix + i * BLOCK];
     }
+    p -= used;
+    if (!done && used == 0) __builtin_amdgcn_s_sleep(1);
+    return done;
+  };
+  auto resolve = [&](bool polled) {
+    if (local > 0) {
+      if (!polled) poll();
+      while (!consume()) poll();
+      dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], kLbInc | (excl + cnt));
+    }
+    s_goff[tid] = gstart + excl - tstart;
+  };
+  if (tid < 256) {
+    uint32_t cpre = 0, gpre = 0;
+    for (uint32_t wv = 0; wv < wave; ++wv) {
+      cpre += s_wsum[0][wv];
+      gpre += s_wsum[1][wv];
+    }
+    tstart = cpre + cinc - cnt;  // first staging slot of this digit
+    gstart = SEG ? gpre + ginc - gval : g[tid];
+#pragma unroll
+    for (int u = 0; u < UNITS; ++u) hiword[2 * (u * 256 + tid)] += tstart;
+    if (!LATE) resolve(false);
+  }
+  __syncthreads();
 
-    if (!more) break;
-    cur = nxt;
-    if constexpr (PIPE) {
+  // Staging slot of every key: its unit's start for the digit + its rank.  The
+  // empty asm makes the digits be recomputed here rather than kept live in
+  // registers since the ranking.
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) key[i] = knext[i];
+  for (int i = 0; i < ITEMS; ++i) {
+    asm volatile("" : "+v"(key[i]));
+    rank[i] += hiword[2 * (unit * 256 + dev::digit8<K>(key[i], shift))];
+  }
+  __syncthreads();
+  // LATE: the first look-back round trip is issued halfway through the
+  // scatter, when half of the keys and ranks no longer need registers.
+#pragma unroll
+  for (int i = 0; i < ITEMS / 2; ++i) s_keys[rank[i]] = key[i];
+  if (LATE && tid < 256 && local > 0) poll();
+#pragma unroll
+  for (int i = ITEMS / 2; i < ITEMS; ++i) s_keys[rank[i]] = key[i];
+  if (LATE && tid < 256) resolve(true);
+  __syncthreads();
+
+  // Coalesced write-out: consecutive slots of one digit go to consecutive
+  // addresses of that digit's bin.  Groups of G items keep G LDS reads in
+  // flight without holding every item's address in registers.
+  constexpr int G = ITEMS % 8 == 0 ? 8 : (ITEMS % 4 == 0 ? 4 : 1);
+  uint32_t gpos[KV ? ITEMS : 1];
+  auto write_out = [&](auto fo) {
+#pragma unroll
+    for (int g0 = 0; g0 < ITEMS; g0 += G) {
+      K wk[G];
+      uint32_t gp[G];
+#pragma unroll
+      for (int i = 0; i < G; ++i) wk[i] = s_keys[tid + (g0 + i) * BLOCK];
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        gp[i] = s_goff[dev::digit8<K>(wk[i], shift)] + tid + (g0 + i) * BLOCK;
+        if constexpr (KV) gpos[g0 + i] = gp[i];
+        if constexpr (decltype(fo)::value) wk[i] = (K)dev::f32_inv((uint32_t)wk[i]);
+      }
+      if (valid == (uint32_t)TILE) {
+#pragma unroll
+        for (int i = 0; i < G; ++i) out[gp[i]] = wk[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+          if (tid + (g0 + i) * BLOCK < valid) out[gp[i]] = wk[i];
+      }
     }
-    __syncthreads();  // the staging area is read before the slots are cleared
+  };
+  if (float_out)
+    write_out(std::true_type{});
+  else
+    write_out(std::false_type{});
+  if constexpr (KV) {
+    __syncthreads();
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) s_vals[rank[i]] = val[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if (tid + i * BLOCK < valid) vdst[gpos[i]] = s_vals[tid + i * BLOCK];
   }
 }
 
