@@ -276,9 +276,9 @@ struct PassVariant {
   int (*launch)(hipStream_t s, const PassArgs& a);
 };
 
-template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool LATE, int MINW, int MODE, bool UNI>
+template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool LATE, int MINW, int MODE, bool UNI, int SPLIT>
 int launch_pass(hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI>), dim3(a.ntiles), dim3(BLOCK), 0, s,
+  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI, SPLIT>), dim3(a.ntiles), dim3(BLOCK), 0, s,
                      static_cast<const K*>(a.kin), static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr,
                      a.ntiles, a.shift, a.fmode, nullptr, nullptr, nullptr, nullptr, 0u, a.chain_info, a.chain_tiles);
   PA_HIP(hipGetLastError());
@@ -286,36 +286,35 @@ int launch_pass(hipStream_t s, const PassArgs& a) {
 }
 
 // PA_VARIANT(key, payload, block, items, look-back width, late look-back, min waves/SIMD, chained,
-//            uniform-digit fast path)
-#define PA_VARIANT(K, KV, B, I, W, P, M, C, U) \
-  { B, I, W, P, M, C, &launch_pass<K, KV, B, I, W, P, M, (C) ? kModeChain : kModePlain, U> }
+//            uniform-digit fast path, staging rounds)
+#define PA_VARIANT(K, KV, B, I, W, P, M, C, U, S) \
+  { B, I, W, P, M, C, &launch_pass<K, KV, B, I, W, P, M, (C) ? kModeChain : kModePlain, U, S> }
 
 template <typename K, bool KV>
 struct Variants;
 template <>
 struct Variants<uint32_t, false> {
   static constexpr PassVariant table[] = {
-      PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 1, true),
-      PA_VARIANT(uint32_t, false, 512, 24, 4, false, 6, 0, true), PA_VARIANT(uint32_t, false, 512, 32, 4, true, 4, 0, true),
-      PA_VARIANT(uint32_t, false, 512, 24, 8, true, 6, 0, true),  PA_VARIANT(uint32_t, false, 512, 24, 2, true, 6, 0, true),
-      PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, false),
+      PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1),
+      PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2), PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 1, true, 2),
+      PA_VARIANT(uint32_t, false, 512, 48, 8, false, 4, 0, true, 2), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2),
   };
 };
 template <>
 struct Variants<uint32_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 5, 0, true),
-                                          PA_VARIANT(uint32_t, true, 512, 16, 4, true, 5, 1, true)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 0, true, 1),
+                                          PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 1, true, 1)};
 };
 template <>
 struct Variants<uint64_t, false> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 0, true),
-                                          PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true),
-                                          PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 0, true, 1),
+                                          PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true, 1),
+                                          PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true, 1)};
 };
 template <>
 struct Variants<uint64_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 0, true),
-                                          PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 1, true)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 0, true, 1),
+                                          PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 1, true, 1)};
 };
 
 template <typename K, bool KV>

@@ -162,8 +162,12 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 //
 // MINW: minimum waves per SIMD the register allocation must allow.
 // UNI: items whose 64 lanes all hold one digit take a path without LDS atomics.
+// SPLIT: the tile is staged and written out in SPLIT rounds of TILE / SPLIT
+// slots (round r takes the keys whose staging slot is in its range), so a
+// tile larger than the LDS staging area can be held in registers: the block
+// then has more keys in flight per byte of LDS and per look-back.
 template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, int MODE = kModePlain, bool LATE = false, int MINW = 1,
-          bool UNI = true>
+          bool UNI = true, int SPLIT = 1>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void k_onesweep(
     const K* __restrict__ kin, K* __restrict__ kout, const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
     uint32_t n, const uint32_t* __restrict__ gtab, uint32_t* __restrict__ lb, uint32_t* __restrict__ tile_counter,
@@ -178,8 +182,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   constexpr int UNITS = BLOCK / 32;
   constexpr int USPAN = 32 * ITEMS;
   constexpr int TILE = BLOCK * ITEMS;
-  static_assert(TILE * sizeof(K) >= UNITS * 256 * sizeof(uint64_t), "rank slots must fit the key staging area");
-  __shared__ __align__(16) K s_keys[TILE];
+  static_assert(ITEMS % SPLIT == 0 && (SPLIT == 1 || !KV), "split staging is for keys-only tiles");
+  constexpr int STAGE = TILE / SPLIT;      // staging slots per round
+  constexpr int RITEMS = ITEMS / SPLIT;    // write-out slots per thread per round
+  static_assert(STAGE * sizeof(K) >= UNITS * 256 * sizeof(uint64_t), "rank slots must fit the key staging area");
+  __shared__ __align__(16) K s_keys[STAGE];
   __shared__ uint32_t s_goff[256];
   __shared__ uint32_t s_wsum[2][4];
   __shared__ uint32_t s_claim;
@@ -290,7 +297,31 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   }
 
   // Stable rank of every key within its unit.
-  uint32_t rank[ITEMS];
+  // Ranks (later staging slots) are below TILE <= 2^16; split tiles keep two
+  // per register to leave room for the larger number of keys.
+  constexpr bool PACK = SPLIT > 1;
+  static_assert(!PACK || (ITEMS % 2 == 0 && TILE <= 65536), "packed ranks need an even item count");
+  uint32_t rank[PACK ? ITEMS / 2 : ITEMS];
+  auto set_rank = [&](int i, uint32_t v) {
+    if constexpr (PACK) {
+      if (i & 1) {
+        rank[i >> 1] |= v << 16;
+        asm volatile("" : "+v"(rank[i >> 1]));
+      } else {
+        rank[i >> 1] = v;
+      }
+    } else {
+      rank[i] = v;
+      // Keep the rank in one register instead of its two operands.
+      asm volatile("" : "+v"(rank[i]));
+    }
+  };
+  auto get_rank = [&](int i) -> uint32_t {
+    if constexpr (PACK)
+      return (i & 1) ? rank[i >> 1] >> 16 : rank[i >> 1] & 0xFFFFu;
+    else
+      return rank[i];
+  };
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t d = dev::digit8<K>(key[i], shift);
@@ -299,8 +330,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       // peers of each unit are its 32 lanes, so no lane mask is needed and 32
       // lanes do not serialise on one LDS atomic.
       const uint32_t seen = uword[2 * d + 1];
-      rank[i] = seen + ul;
-      asm volatile("" : "+v"(rank[i]));
+      set_rank(i, seen + ul);
       if (ul == 0) uword[2 * d + 1] = seen + 32u;
     } else {
       atomicOr(&uword[2 * d], lanebit);
@@ -308,9 +338,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       const uint32_t peers = (uint32_t)sv;
       const uint32_t seen = (uint32_t)(sv >> 32);
       const uint32_t below = __popc(peers & (lanebit - 1u));
-      rank[i] = seen + below;
-      // Keep the rank in one register instead of its two operands.
-      asm volatile("" : "+v"(rank[i]));
+      set_rank(i, seen + below);
       if (below == 0) uslot[d] = (uint64_t)(seen + __popc(peers)) << 32;
     }
   }
@@ -393,39 +421,48 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   __syncthreads();
 
   // Staging slot of every key: its unit's start for the digit + its rank.  The
-  // empty asm makes the digits be recomputed here rather than kept live in
-  // registers since the ranking.
+  // empty asm statements (which keep their order) make the digits be
+  // recomputed here rather than kept live since the ranking, and bound the LDS
+  // reads in flight to one group of 8 keys.
+  constexpr int LG = ITEMS % 8 == 0 ? 8 : (ITEMS % 4 == 0 ? 4 : 2);
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    asm volatile("" : "+v"(key[i]));
-    rank[i] += hiword[2 * (unit * 256 + dev::digit8<K>(key[i], shift))];
+  for (int g0 = 0; g0 < ITEMS; g0 += LG) {
+#pragma unroll
+    for (int i = g0; i < g0 + LG && i < ITEMS; ++i) asm volatile("" : "+v"(key[i]));
+#pragma unroll
+    for (int i = g0; i < g0 + LG && i < ITEMS; ++i) {
+      const uint32_t start = hiword[2 * (unit * 256 + dev::digit8<K>(key[i], shift))];
+      if constexpr (PACK)
+        rank[i >> 1] += (i & 1) ? start << 16 : start;  // halves stay below 2^16: no carry
+      else
+        rank[i] += start;
+    }
+#pragma unroll
+    for (int i = g0; i < g0 + LG && i < ITEMS; ++i) {
+      if constexpr (PACK) {
+        if (i & 1) asm volatile("" : "+v"(rank[i >> 1]));
+      } else {
+        asm volatile("" : "+v"(rank[i]));
+      }
+    }
   }
   __syncthreads();
-  // LATE: the first look-back round trip is issued halfway through the
-  // scatter, when half of the keys and ranks no longer need registers.
-#pragma unroll
-  for (int i = 0; i < ITEMS / 2; ++i) s_keys[rank[i]] = key[i];
-  if (LATE && tid < 256 && local > 0) poll();
-#pragma unroll
-  for (int i = ITEMS / 2; i < ITEMS; ++i) s_keys[rank[i]] = key[i];
-  if (LATE && tid < 256) resolve(true);
-  __syncthreads();
 
-  // Coalesced write-out: consecutive slots of one digit go to consecutive
-  // addresses of that digit's bin.  Groups of G items keep G LDS reads in
-  // flight without holding every item's address in registers.
-  constexpr int G = ITEMS % 8 == 0 ? 8 : (ITEMS % 4 == 0 ? 4 : 1);
+  // Coalesced write-out of one staging round: consecutive slots of one digit
+  // go to consecutive addresses of that digit's bin.  Groups of G items keep G
+  // LDS reads in flight without holding every item's address in registers.
+  constexpr int G = RITEMS % 8 == 0 ? 8 : (RITEMS % 4 == 0 ? 4 : 1);
   uint32_t gpos[KV ? ITEMS : 1];
-  auto write_out = [&](auto fo) {
+  auto write_out = [&](auto fo, uint32_t r) {
 #pragma unroll
-    for (int g0 = 0; g0 < ITEMS; g0 += G) {
+    for (int g0 = 0; g0 < RITEMS; g0 += G) {
       K wk[G];
       uint32_t gp[G];
 #pragma unroll
       for (int i = 0; i < G; ++i) wk[i] = s_keys[tid + (g0 + i) * BLOCK];
 #pragma unroll
       for (int i = 0; i < G; ++i) {
-        gp[i] = s_goff[dev::digit8<K>(wk[i], shift)] + tid + (g0 + i) * BLOCK;
+        gp[i] = s_goff[dev::digit8<K>(wk[i], shift)] + r * STAGE + tid + (g0 + i) * BLOCK;
         if constexpr (KV) gpos[g0 + i] = gp[i];
         if constexpr (decltype(fo)::value) wk[i] = (K)dev::f32_inv((uint32_t)wk[i]);
       }
@@ -435,19 +472,47 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       } else {
 #pragma unroll
         for (int i = 0; i < G; ++i)
-          if (tid + (g0 + i) * BLOCK < valid) out[gp[i]] = wk[i];
+          if (r * STAGE + tid + (g0 + i) * BLOCK < valid) out[gp[i]] = wk[i];
       }
     }
   };
-  if (float_out)
-    write_out(std::true_type{});
-  else
-    write_out(std::false_type{});
+  // Scatter of items [i0, i1) into staging round r.
+  auto scatter = [&](int i0, int i1, uint32_t r) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if (i < i0 || i >= i1) continue;
+      if constexpr (SPLIT == 1) {
+        s_keys[get_rank(i)] = key[i];
+      } else {
+        const uint32_t q = get_rank(i) - r * STAGE;
+        if (q < (uint32_t)STAGE) s_keys[q] = key[i];
+      }
+    }
+  };
+#pragma unroll
+  for (int r = 0; r < SPLIT; ++r) {
+    if (r == 0) {
+      // LATE: the first look-back round trip is issued halfway through the
+      // scatter, when half of the keys and ranks no longer need registers.
+      scatter(0, ITEMS / 2, 0);
+      if (LATE && tid < 256 && local > 0) poll();
+      scatter(ITEMS / 2, ITEMS, 0);
+      if (LATE && tid < 256) resolve(true);
+    } else {
+      scatter(0, ITEMS, r);
+    }
+    __syncthreads();
+    if (float_out)
+      write_out(std::true_type{}, r);
+    else
+      write_out(std::false_type{}, r);
+    if (r + 1 < SPLIT) __syncthreads();
+  }
   if constexpr (KV) {
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) s_vals[rank[i]] = val[i];
+    for (int i = 0; i < ITEMS; ++i) s_vals[get_rank(i)] = val[i];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i)
