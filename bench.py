@@ -115,13 +115,31 @@ def bench_segmented(pa, torch, count=10_000, seed=1239):
     torch.cuda.synchronize()
     ms_single = e0.elapsed_time(e1)
     same = bool(torch.equal(work, work2))
-    del work, work2, tmp, pristine
+    # graph-captured plan over the same 10,000 arrays (tables and scratch set
+    # up once), replayed on fresh copies of the input
+    work2.copy_(pristine)
+    slices = [work2[int(offs[i]):int(offs[i + 1])] for i in range(count)]
+    plan = pa.SegmentedPlan(slices)
+    plan.run()
+    torch.cuda.synchronize()
+    graph_same = bool(torch.equal(work, work2))
+    work2.copy_(pristine)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    plan.run(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_graph = e0.elapsed_time(e1)
+    graph_same = graph_same and bool(torch.equal(work, work2))
+    plan.close()
+    del slices, work, work2, tmp, pristine
     torch.cuda.empty_cache()
     return {"workload": f"{count} u32 arrays, lengths log-uniform [64, 4e6], 5% presorted", "keys": total,
             "ms_batched": round(ms_batch, 3), "ms_per_array_calls": round(ms_single, 3),
             "gkeys_per_s_batched": round(total / ms_batch / 1e6, 2),
             "gkeys_per_s_per_array": round(total / ms_single / 1e6, 2),
-            "speedup": round(ms_single / ms_batch, 2), "bit_exact_vs_per_array": same}
+            "speedup": round(ms_single / ms_batch, 2), "bit_exact_vs_per_array": same,
+            "ms_graph_plan_replay": round(ms_graph, 3), "graph_bit_exact": graph_same}
 
 
 def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):

@@ -68,6 +68,19 @@ int pa_sort_u32(uint32_t* keys, uint32_t* tmp, size_t n, pa_stream_t stream);
  * reference. */
 int pa_sort_two_phase_u32(uint32_t* a, uint32_t* b, size_t n, pa_stream_t stream);
 
+/* Pass shape per size class (n <= 2^16, <= 2^20, <= 2^24, larger).  The
+ * reference fixes its work split by constants (SortRadixPar's quantum,
+ * RadixSortLsdParallel.h:374-379); here each class has a built-in tile shape,
+ * and pa_autotune_sort_u32 times every candidate shape on this GPU at one size
+ * per class (up to max_n keys, device memory it allocates and frees) and makes
+ * later u32 sorts in this process use the fastest.  pa_sort_variant_u32
+ * returns the candidate index a sort of n u32 keys will use;
+ * pa_autotune_reset restores the built-in choice.  The environment variable
+ * PA_SORT_VARIANT=i forces candidate i for every size. */
+int pa_autotune_sort_u32(size_t max_n, pa_stream_t stream);
+int pa_sort_variant_u32(size_t n);
+int pa_autotune_reset(void);
+
 /* 64-bit keys, 8 passes.  No reference counterpart on the parallel path (the
  * serial u64 LSD at RadixSortLSD.h:356-400 mis-sorts keys >= 2^32). */
 int pa_sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, pa_stream_t stream);
@@ -174,6 +187,18 @@ int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets
  * buffers: segment i is keys[i][0 .. lens[i]) (keys and lens are host arrays;
  * each keys[i] is device memory).  Used to batch independent requests. */
 int pa_segmented_sort_ptrs_u32(uint32_t* const* keys, const uint64_t* lens, size_t num_segments, pa_stream_t stream);
+
+/* Reusable plan for sorting the same device arrays again and again (a
+ * serving loop that re-sorts buffers of fixed shapes): the segment tables and
+ * the library's scratch are set up once, and the launch sequence is captured
+ * into a HIP graph, so each run is one graph launch that sorts the arrays'
+ * current contents in place.  No reference counterpart (the reference sorts
+ * one array per call).  The arrays must stay allocated while the plan lives;
+ * runs of one plan must not overlap. */
+typedef struct pa_segplan pa_segplan;
+pa_segplan* pa_segmented_plan_create_u32(uint32_t* const* keys, const uint64_t* lens, size_t num_segments);
+int pa_segmented_plan_run(pa_segplan* plan, pa_stream_t stream);
+void pa_segmented_plan_destroy(pa_segplan* plan);
 
 /* ---- Serving: continuous batching of sort / select / histogram jobs --------- */
 /* No reference counterpart (the reference is a blocking call-per-array

@@ -33,6 +33,9 @@ _SIGS = {
     "pa_synchronize": (_i, [_vp]),
     "pa_sort_u32": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_two_phase_u32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_autotune_sort_u32": (_i, [_sz, _vp]),
+    "pa_sort_variant_u32": (_i, [_sz]),
+    "pa_autotune_reset": (_i, []),
     "pa_sort_u64": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_f32": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_kv_u32_u32": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
@@ -52,6 +55,9 @@ _SIGS = {
     "pa_merge_inplace_u64": (_i, [_vp, _sz, _sz, _vp]),
     "pa_segmented_sort_u32": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "pa_segmented_sort_ptrs_u32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_segmented_plan_create_u32": (_vp, [_vp, _vp, _sz]),
+    "pa_segmented_plan_run": (_i, [_vp, _vp]),
+    "pa_segmented_plan_destroy": (None, [_vp]),
     "pa_copy_u32": (_i, [_vp, _vp, _sz, _vp]),
     "pa_server_create": (_vp, [_sz, _i, _sz]),
     "pa_server_submit": (_i, [_vp, _vp, _vp]),
@@ -154,6 +160,18 @@ def sort_two_phase_u32(a, b, stream=None):
     call("pa_sort_two_phase_u32", _ptr(a), _ptr(b), numel(a), _stream(stream))
 
 
+def autotune_sort_u32(max_n, stream=None):
+    call("pa_autotune_sort_u32", max_n, _stream(stream))
+
+
+def sort_variant_u32(n):
+    return load().pa_sort_variant_u32(n)
+
+
+def autotune_reset():
+    call("pa_autotune_reset")
+
+
 def sort_u64(keys, tmp=None, stream=None):
     call("pa_sort_u64", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
@@ -233,6 +251,35 @@ def segmented_sort_ptrs_u32(tensors, stream=None):
     ptrs = np.array([_ptr(t) or 0 for t in tensors], dtype=np.uint64)
     lens = np.array([numel(t) for t in tensors], dtype=np.uint64)
     call("pa_segmented_sort_ptrs_u32", ptrs.ctypes.data, lens.ctypes.data, len(tensors), _stream(stream))
+
+
+class SegmentedPlan:
+    """A graph-captured segmented sort of fixed device arrays
+    (pa_segmented_plan_*): each run() sorts their current contents."""
+
+    def __init__(self, tensors):
+        import numpy as np
+        self.lib = load()
+        self.keep = list(tensors)  # the arrays must outlive the plan
+        ptrs = np.array([_ptr(t) or 0 for t in tensors], dtype=np.uint64)
+        lens = np.array([numel(t) for t in tensors], dtype=np.uint64)
+        self.h = self.lib.pa_segmented_plan_create_u32(ptrs.ctypes.data, lens.ctypes.data, len(tensors))
+        if not self.h:
+            raise PAError("pa_segmented_plan_create_u32", 1, self.lib.pa_last_error_string().decode())
+
+    def run(self, stream=None):
+        call("pa_segmented_plan_run", self.h, _stream(stream))
+
+    def close(self):
+        if self.h:
+            self.lib.pa_segmented_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 OP_SORT, OP_SELECT, OP_HIST = 0, 1, 2

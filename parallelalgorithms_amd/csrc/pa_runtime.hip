@@ -139,6 +139,11 @@ static std::vector<ProfRec> g_prof;
 
 ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
   if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();  // launches being captured into a graph are not timed
+    return;
+  }
   if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
     (void)hipGetLastError();
     start = stop = nullptr;

@@ -28,11 +28,18 @@ constexpr int kSmallBlock = 256;
 constexpr int kSmallItems = 16;
 constexpr uint32_t kSmallTile = kSmallBlock * kSmallItems;  // 4096 keys
 
-// Segmented onesweep tile shape (keys only).
+// Segmented onesweep tile shape (keys only): the plain sort's default pass,
+// 512 x 48 keys staged in two rounds.
 constexpr int kSegBlock = 512;
-constexpr int kSegItems = 16;
+constexpr int kSegItems = 48;
 constexpr int kSegLbw = 8;
+constexpr int kSegSplit = 2;
 constexpr uint32_t kSegTile = kSegBlock * kSegItems;
+// Claim-order resolution: tile k of a segment with T tiles goes to bucket
+// floor((2k + 1) * kSegBuckets / (2T)), buckets in order, so every segment's
+// tiles are spread over the whole launch and a tile's predecessor in its
+// segment was claimed well before it.
+constexpr uint32_t kSegBuckets = 1024;
 
 // Whole-segment LDS sort.  Padding slots hold 0xFFFFFFFF: a stable sort
 // keeps them after every real key, so the first len slots are the answer.
@@ -146,7 +153,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restri
 // ping-pong space from `work` (their tmp addresses).  ctx.mu must be held.
 struct SegPlan {
   std::vector<uint64_t> small_ptr, large_in, large_tmp, tile_info;
-  std::vector<uint32_t> small_len, large_len;
+  std::vector<uint32_t> small_len, large_len, large_tiles, large_row;
   void add(uint64_t ptr, uint64_t len, uint64_t tmp_ptr) {
     if (len <= 1) return;
     if (len <= kSmallTile) {
@@ -154,46 +161,98 @@ struct SegPlan {
       small_len.push_back((uint32_t)len);
       return;
     }
-    const uint64_t seg = large_in.size();
     large_in.push_back(ptr);
     large_tmp.push_back(tmp_ptr);
     large_len.push_back((uint32_t)len);
-    const uint32_t tiles = (uint32_t)((len + kSegTile - 1) / kSegTile);
-    for (uint32_t t = 0; t < tiles; ++t) tile_info.push_back((seg << 32) | t);
+    large_tiles.push_back((uint32_t)((len + kSegTile - 1) / kSegTile));
+  }
+  // Look-back rows (contiguous per segment) and the interleaved claim order:
+  // a counting sort of the tiles by bucket, stable in (segment, tile).
+  void finalize() {
+    const size_t nl = large_tiles.size();
+    large_row.resize(nl);
+    uint64_t rows = 0;
+    std::vector<uint32_t> count(kSegBuckets + 1, 0);
+    for (size_t g = 0; g < nl; ++g) {
+      large_row[g] = (uint32_t)rows;
+      const uint64_t T = large_tiles[g];
+      rows += T;
+      for (uint64_t k = 0; k < T; ++k) ++count[(2 * k + 1) * kSegBuckets / (2 * T) + 1];
+    }
+    for (uint32_t b = 0; b < kSegBuckets; ++b) count[b + 1] += count[b];
+    tile_info.assign(rows, 0);
+    for (size_t g = 0; g < nl; ++g) {
+      const uint64_t T = large_tiles[g];
+      for (uint64_t k = 0; k < T; ++k) tile_info[count[(2 * k + 1) * kSegBuckets / (2 * T)]++] = ((uint64_t)g << 32) | k;
+    }
   }
 };
 
-static int run_plan(StreamCtx& ctx, const SegPlan& P, char* dev_base, size_t o_tab, size_t o_cnt, size_t o_ctr,
-                    size_t o_lb, hipStream_t s) {
-  const size_t ns = P.small_ptr.size(), nl = P.large_in.size(), nt = P.tile_info.size();
-  Layout T;
-  const size_t t_sp = T.take(ns * 8), t_sl = T.take(ns * 4), t_li = T.take(nl * 8), t_lt = T.take(nl * 8),
-               t_ll = T.take(nl * 4), t_ti = T.take(nt * 8);
-  const size_t table_bytes = T.off;
-  void* pinned;
-  int slot;
-  PA_TRY(acquire_pinned(ctx, table_bytes, &pinned, &slot));
-  char* pin = (char*)pinned;
-  std::copy(P.small_ptr.begin(), P.small_ptr.end(), (uint64_t*)(pin + t_sp));
-  std::copy(P.small_len.begin(), P.small_len.end(), (uint32_t*)(pin + t_sl));
-  std::copy(P.large_in.begin(), P.large_in.end(), (uint64_t*)(pin + t_li));
-  std::copy(P.large_tmp.begin(), P.large_tmp.end(), (uint64_t*)(pin + t_lt));
-  std::copy(P.large_len.begin(), P.large_len.end(), (uint32_t*)(pin + t_ll));
-  std::copy(P.tile_info.begin(), P.tile_info.end(), (uint64_t*)(pin + t_ti));
-  char* tab = dev_base + o_tab;
-  PA_HIP(hipMemcpyAsync(tab, pin, table_bytes, hipMemcpyHostToDevice, s));
-  PA_TRY(release_pinned(ctx, slot));
-  const uint64_t* d_sp = (const uint64_t*)(tab + t_sp);
-  const uint32_t* d_sl = (const uint32_t*)(tab + t_sl);
-  const uint64_t* d_li = (const uint64_t*)(tab + t_li);
-  const uint64_t* d_lt = (const uint64_t*)(tab + t_lt);
-  const uint32_t* d_ll = (const uint32_t*)(tab + t_ll);
-  const uint64_t* d_ti = (const uint64_t*)(tab + t_ti);
-  uint32_t* counts = (uint32_t*)(dev_base + o_cnt);
-  uint32_t* ctr = (uint32_t*)(dev_base + o_ctr);
-  uint32_t* lbp = (uint32_t*)(dev_base + o_lb);
-  const size_t lb_bytes = (size_t)4 * nt * 256 * 4;
+// Device image of a plan: the table block (pointer and length lists, rows,
+// tile order) followed by per-segment counts, tile counters, the look-back
+// table and `extra` bytes of ping-pong space for large segments.
+struct PlanLayout {
+  size_t t_sp, t_sl, t_li, t_lt, t_ll, t_lr, t_ti, table_bytes;
+  size_t o_tab, o_cnt, o_ctr, o_lb, o_work, total;
+};
 
+static PlanLayout plan_layout(const SegPlan& P, size_t extra) {
+  const size_t ns = P.small_ptr.size(), nl = P.large_in.size(), nt = P.tile_info.size();
+  PlanLayout Y;
+  Layout T;
+  Y.t_sp = T.take(ns * 8);
+  Y.t_sl = T.take(ns * 4);
+  Y.t_li = T.take(nl * 8);
+  Y.t_lt = T.take(nl * 8);
+  Y.t_ll = T.take(nl * 4);
+  Y.t_lr = T.take(nl * 4);
+  Y.t_ti = T.take(nt * 8);
+  Y.table_bytes = T.off;
+  Layout L;
+  Y.o_tab = L.take(Y.table_bytes);
+  Y.o_cnt = L.take(nl * 1024 * 4);
+  Y.o_ctr = L.take(16);
+  Y.o_lb = L.take((size_t)4 * nt * 256 * 4);
+  Y.o_work = L.take(extra);
+  Y.total = L.off;
+  return Y;
+}
+
+// Large segments without a caller tmp get consecutive ping-pong space.
+static void assign_work(SegPlan& P, char* work) {
+  uint64_t off = 0;
+  for (size_t i = 0; i < P.large_tmp.size(); ++i) {
+    P.large_tmp[i] = (uint64_t)(uintptr_t)work + off * 4;
+    off += P.large_len[i];
+  }
+}
+
+static void write_tables(const SegPlan& P, const PlanLayout& Y, char* pin) {
+  std::copy(P.small_ptr.begin(), P.small_ptr.end(), (uint64_t*)(pin + Y.t_sp));
+  std::copy(P.small_len.begin(), P.small_len.end(), (uint32_t*)(pin + Y.t_sl));
+  std::copy(P.large_in.begin(), P.large_in.end(), (uint64_t*)(pin + Y.t_li));
+  std::copy(P.large_tmp.begin(), P.large_tmp.end(), (uint64_t*)(pin + Y.t_lt));
+  std::copy(P.large_len.begin(), P.large_len.end(), (uint32_t*)(pin + Y.t_ll));
+  std::copy(P.large_row.begin(), P.large_row.end(), (uint32_t*)(pin + Y.t_lr));
+  std::copy(P.tile_info.begin(), P.tile_info.end(), (uint64_t*)(pin + Y.t_ti));
+}
+
+// The kernels of one plan, on device memory laid out by plan_layout.  Only
+// kernels (no memset, no copy), so the sequence can be captured in a graph.
+static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStream_t s) {
+  const size_t ns = P.small_ptr.size(), nl = P.large_in.size(), nt = P.tile_info.size();
+  const char* tab = dev + Y.o_tab;
+  const uint64_t* d_sp = (const uint64_t*)(tab + Y.t_sp);
+  const uint32_t* d_sl = (const uint32_t*)(tab + Y.t_sl);
+  const uint64_t* d_li = (const uint64_t*)(tab + Y.t_li);
+  const uint64_t* d_lt = (const uint64_t*)(tab + Y.t_lt);
+  const uint32_t* d_ll = (const uint32_t*)(tab + Y.t_ll);
+  const uint32_t* d_lr = (const uint32_t*)(tab + Y.t_lr);
+  const uint64_t* d_ti = (const uint64_t*)(tab + Y.t_ti);
+  uint32_t* counts = (uint32_t*)(dev + Y.o_cnt);
+  uint32_t* ctr = (uint32_t*)(dev + Y.o_ctr);
+  uint32_t* lbp = (uint32_t*)(dev + Y.o_lb);
+  const size_t lb_bytes = (size_t)4 * nt * 256 * 4;
   if (ns) {
     ProfScope ps("k_seg_small", s);
     hipLaunchKernelGGL(k_seg_small, dim3((unsigned)ns), dim3(kSmallBlock), 0, s, d_sp, d_sl);
@@ -210,40 +269,55 @@ static int run_plan(StreamCtx& ctx, const SegPlan& P, char* dev_base, size_t o_t
       const uint64_t* tin = (p & 1) ? d_lt : d_li;
       const uint64_t* tout = (p & 1) ? d_li : d_lt;
       ProfScope ps("k_onesweep_seg", s);
-      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, kModeSeg, false, 8>), dim3((unsigned)nt),
-                         dim3(kSegBlock), 0, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)nullptr,
-                         (uint32_t*)nullptr, 0u, counts + p * 256, lbp + (size_t)p * nt * 256, ctr + p,
-                         (uint32_t)nt, (uint32_t)(8 * p), 0u, d_ti, tin, tout, d_ll, 4u);
+      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, kModeSeg, true, 4, true, kSegSplit>),
+                         dim3((unsigned)nt), dim3(kSegBlock), 0, s, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                         (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u, counts + p * 256,
+                         lbp + (size_t)p * nt * 256, ctr + p, (uint32_t)nt, (uint32_t)(8 * p), 0u, d_ti, tin, tout, d_ll,
+                         4u, d_lr, (const uint32_t*)nullptr);
     }
   }
   PA_HIP(hipGetLastError());
   return PA_OK;
 }
 
-// Scratch layout for a plan: tables, per-segment counts, tile counters,
-// look-back table, and `extra` bytes of ping-pong space.
-static int run_plan_alloc(const SegPlan& P, size_t extra, hipStream_t s, StreamCtx& ctx, char** work_out,
-                          bool fill_tmp, SegPlan* fix) {
-  const size_t ns = P.small_ptr.size(), nl = P.large_in.size(), nt = P.tile_info.size();
-  if (nt >= (1ull << 32)) return arg_error("segmented sort: too many tiles");
-  Layout L;
-  const size_t o_tab = L.take(ns * 12 + nl * 20 + nt * 8 + 6 * 256);
-  const size_t o_cnt = L.take(nl * 1024 * 4);
-  const size_t o_ctr = L.take(16);
-  const size_t o_lb = L.take((size_t)4 * nt * 256 * 4);
-  const size_t o_work = L.take(extra);
+// One-shot run on the stream's scratch: tables go up through pinned memory.
+// `extra` bytes of ping-pong space are reserved when no caller tmp exists.
+// ctx.mu must be held.
+static int run_plan_alloc(SegPlan& P, size_t extra, bool fill_tmp, hipStream_t s, StreamCtx& ctx) {
+  if (P.tile_info.size() >= (1ull << 32)) return arg_error("segmented sort: too many tiles");
+  const PlanLayout Y = plan_layout(P, extra);
   void* base;
-  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  PA_TRY(ensure_scratch(ctx, Y.total, &base));
   char* b = (char*)base;
-  *work_out = b + o_work;
-  if (fill_tmp) {  // large segments without a caller tmp get consecutive work space
-    uint64_t off = 0;
-    for (size_t i = 0; i < fix->large_tmp.size(); ++i) {
-      fix->large_tmp[i] = (uint64_t)(uintptr_t)(b + o_work) + off * 4;
-      off += fix->large_len[i];
+  if (fill_tmp) assign_work(P, b + Y.o_work);
+  void* pinned;
+  int slot;
+  PA_TRY(acquire_pinned(ctx, Y.table_bytes, &pinned, &slot));
+  write_tables(P, Y, (char*)pinned);
+  PA_HIP(hipMemcpyAsync(b + Y.o_tab, pinned, Y.table_bytes, hipMemcpyHostToDevice, s));
+  PA_TRY(release_pinned(ctx, slot));
+  return launch_plan(P, Y, b, s);
+}
+
+// Builds the plan of a pointer-table call; returns PA_OK or an error status.
+static int build_ptr_plan(uint32_t* const* keys, const uint64_t* lens, size_t num_segments, const char* who,
+                          SegPlan* P, uint64_t* large_total) {
+  *large_total = 0;
+  for (size_t i = 0; i < num_segments; ++i) {
+    if (lens[i] == 0) continue;
+    if (!keys[i]) return arg_error((std::string(who) + ": null segment pointer").c_str());
+    if (lens[i] >= (1ull << 30)) {
+      set_last_error(std::string(who) + ": a segment holds 2^30 or more keys");
+      return PA_ERR_TOO_LARGE;
     }
+    P->add((uint64_t)(uintptr_t)keys[i], lens[i], 0);
+    if (lens[i] > kSmallTile) *large_total += lens[i];
   }
-  return run_plan(ctx, *fix, b, o_tab, o_cnt, o_ctr, o_lb, s);
+  for (size_t i = 0; i < num_segments; ++i)
+    if (lens[i] > 1 && !is_device_ptr(keys[i]))
+      return arg_error((std::string(who) + ": segments must be device memory").c_str());
+  P->finalize();
+  return PA_OK;
 }
 
 }  // namespace pa
@@ -274,10 +348,10 @@ int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets
     P.add((uint64_t)(uintptr_t)(keys + offsets[i]), len, tmp ? (uint64_t)(uintptr_t)(tmp + offsets[i]) : 0);
     if (len > kSmallTile) large_total += len;
   }
+  P.finalize();
   StreamCtx& ctx = stream_ctx(s);
   std::lock_guard<std::mutex> lk(ctx.mu);
-  char* work;
-  return run_plan_alloc(P, tmp ? 0 : large_total * 4, s, ctx, &work, tmp == nullptr, &P);
+  return run_plan_alloc(P, tmp ? 0 : large_total * 4, tmp == nullptr, s, ctx);
 }
 
 int pa_segmented_sort_ptrs_u32(uint32_t* const* keys, const uint64_t* lens, size_t num_segments, pa_stream_t stream) {
@@ -286,23 +360,75 @@ int pa_segmented_sort_ptrs_u32(uint32_t* const* keys, const uint64_t* lens, size
   hipStream_t s = (hipStream_t)stream;
   SegPlan P;
   uint64_t large_total = 0;
-  for (size_t i = 0; i < num_segments; ++i) {
-    if (lens[i] == 0) continue;
-    if (!keys[i]) return arg_error("pa_segmented_sort_ptrs_u32: null segment pointer");
-    if (lens[i] >= (1ull << 30)) {
-      set_last_error("pa_segmented_sort_ptrs_u32: a segment holds 2^30 or more keys");
-      return PA_ERR_TOO_LARGE;
-    }
-    P.add((uint64_t)(uintptr_t)keys[i], lens[i], 0);
-    if (lens[i] > kSmallTile) large_total += lens[i];
-  }
-  for (size_t i = 0; i < num_segments; ++i)
-    if (lens[i] > 1 && !is_device_ptr(keys[i]))
-      return arg_error("pa_segmented_sort_ptrs_u32: segments must be device memory");
+  PA_TRY(build_ptr_plan(keys, lens, num_segments, "pa_segmented_sort_ptrs_u32", &P, &large_total));
   StreamCtx& ctx = stream_ctx(s);
   std::lock_guard<std::mutex> lk(ctx.mu);
-  char* work;
-  return run_plan_alloc(P, large_total * 4, s, ctx, &work, true, &P);
+  return run_plan_alloc(P, large_total * 4, true, s, ctx);
+}
+
+struct pa_segplan {
+  pa::SegPlan plan;
+  pa::PlanLayout layout;
+  char* mem = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+void pa_segmented_plan_destroy(pa_segplan* plan) {
+  if (!plan) return;
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  if (plan->mem) (void)hipFree(plan->mem);
+  (void)hipGetLastError();
+  delete plan;
+}
+
+pa_segplan* pa_segmented_plan_create_u32(uint32_t* const* keys, const uint64_t* lens, size_t num_segments) {
+  if (!keys || !lens || num_segments == 0) {
+    arg_error("pa_segmented_plan_create_u32: empty or null table");
+    return nullptr;
+  }
+  pa_segplan* G = new pa_segplan();
+  uint64_t large_total = 0;
+  hipStream_t cap = nullptr;
+  auto fail = [&](hipError_t e, const char* what) -> pa_segplan* {
+    if (e != hipSuccess) hip_fail(e, what, __FILE__, __LINE__);
+    if (cap) (void)hipStreamDestroy(cap);
+    pa_segmented_plan_destroy(G);
+    return nullptr;
+  };
+  if (build_ptr_plan(keys, lens, num_segments, "pa_segmented_plan_create_u32", &G->plan, &large_total) != PA_OK)
+    return fail(hipSuccess, "");
+  if (G->plan.tile_info.size() >= (1ull << 32)) {
+    arg_error("pa_segmented_plan_create_u32: too many tiles");
+    return fail(hipSuccess, "");
+  }
+  G->layout = plan_layout(G->plan, large_total * 4);
+  hipError_t e = hipMalloc(&G->mem, std::max<size_t>(G->layout.total, 256));
+  if (e != hipSuccess) return fail(e, "hipMalloc");
+  assign_work(G->plan, G->mem + G->layout.o_work);
+  std::vector<char> image(G->layout.table_bytes);
+  write_tables(G->plan, G->layout, image.data());
+  if (!image.empty() &&
+      (e = hipMemcpy(G->mem + G->layout.o_tab, image.data(), image.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy");
+  if ((e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
+  if ((e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+    return fail(e, "hipStreamBeginCapture");
+  const int st = launch_plan(G->plan, G->layout, G->mem, cap);
+  e = hipStreamEndCapture(cap, &G->graph);
+  if (st != PA_OK) return fail(hipSuccess, "");
+  if (e != hipSuccess) return fail(e, "hipStreamEndCapture");
+  if ((e = hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0)) != hipSuccess)
+    return fail(e, "hipGraphInstantiate");
+  (void)hipStreamDestroy(cap);
+  return G;
+}
+
+int pa_segmented_plan_run(pa_segplan* plan, pa_stream_t stream) {
+  if (!plan || !plan->exec) return arg_error("pa_segmented_plan_run: null plan");
+  PA_HIP(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+  return PA_OK;
 }
 
 }  // extern "C"

@@ -25,6 +25,8 @@
 // D = sizeof(K) is even, so the result lands back in `keys`.  Scratch per
 // call: counts D x 256 u32, tile counters D u32, look-back D x tiles x 256 u32.
 #include <algorithm>
+#include <atomic>
+#include <type_traits>
 #include <cstdlib>
 
 #include "pa_sort_kernels.h"
@@ -290,49 +292,79 @@ int launch_pass(hipStream_t s, const PassArgs& a) {
 #define PA_VARIANT(K, KV, B, I, W, P, M, C, U, S) \
   { B, I, W, P, M, C, &launch_pass<K, KV, B, I, W, P, M, (C) ? kModeChain : kModePlain, U, S> }
 
+// Size classes of a sort call: n <= 2^16, <= 2^20, <= 2^24, larger.
+constexpr int kClasses = 4;
+inline int size_class(size_t n) { return n <= (1u << 16) ? 0 : n <= (1u << 20) ? 1 : n <= (1u << 24) ? 2 : 3; }
+
+// Candidate passes per key type; by_class is the built-in choice per size
+// class (small arrays want small tiles for parallelism, large arrays the
+// split 24K-key tile).
 template <typename K, bool KV>
 struct Variants;
 template <>
 struct Variants<uint32_t, false> {
   static constexpr PassVariant table[] = {
       PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1),
-      PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2), PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 1, true, 2),
-      PA_VARIANT(uint32_t, false, 512, 48, 8, false, 4, 0, true, 2), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2),
+      PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1),  PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 1, true, 2),
+      PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2),
   };
+  static constexpr int by_class[kClasses] = {2, 2, 0, 0};
 };
 template <>
 struct Variants<uint32_t, true> {
   static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 0, true, 1),
                                           PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 1, true, 1)};
+  static constexpr int by_class[kClasses] = {0, 0, 0, 0};
 };
 template <>
 struct Variants<uint64_t, false> {
   static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 0, true, 1),
+                                          PA_VARIANT(uint64_t, false, 512, 16, 8, true, 6, 0, true, 2),
                                           PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true, 1),
                                           PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true, 1)};
+  static constexpr int by_class[kClasses] = {0, 0, 0, 0};
 };
 template <>
 struct Variants<uint64_t, true> {
   static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 0, true, 1),
                                           PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 1, true, 1)};
+  static constexpr int by_class[kClasses] = {0, 0, 0, 0};
 };
 
+// Choices recorded by the autotuner (-1: none) for u32 keys-only sorts.
+static std::atomic<int> g_tuned_u32[kClasses] = {{-1}, {-1}, {-1}, {-1}};
+
 template <typename K, bool KV>
-const PassVariant& pick_variant() {
-  constexpr size_t count = sizeof(Variants<K, KV>::table) / sizeof(PassVariant);
-  static int idx = [] {
+constexpr size_t variant_count() {
+  return sizeof(Variants<K, KV>::table) / sizeof(PassVariant);
+}
+
+// PA_SORT_VARIANT (an index into the key type's table) overrides every
+// class; otherwise the tuned choice, otherwise the built-in one.
+template <typename K, bool KV>
+int variant_index(size_t n) {
+  static const int forced = [] {
     const char* e = std::getenv("PA_SORT_VARIANT");
-    int v = e ? std::atoi(e) : 0;
-    return (v >= 0 && (size_t)v < count) ? v : 0;
+    const int v = e ? std::atoi(e) : -1;
+    return (v >= 0 && (size_t)v < variant_count<K, KV>()) ? v : -1;
   }();
-  return Variants<K, KV>::table[idx];
+  if (forced >= 0) return forced;
+  const int c = size_class(n);
+  if (std::is_same<K, uint32_t>::value && !KV) {
+    const int t = g_tuned_u32[c].load(std::memory_order_relaxed);
+    if (t >= 0) return t;
+  }
+  return Variants<K, KV>::by_class[c];
+}
+
+template <typename K, bool KV>
+const PassVariant& pick_variant(size_t n) {
+  return Variants<K, KV>::table[variant_index<K, KV>(n)];
 }
 
 // Look-back rows (and launch bound) of one pass: every chain rounds up to
 // whole tiles.
-template <typename K, bool KV>
-size_t sort_rows(size_t n) {
-  const PassVariant& v = pick_variant<K, KV>();
+inline size_t sort_rows(const PassVariant& v, size_t n) {
   const size_t tile = (size_t)v.block * v.items;
   return (n + tile - 1) / tile + (v.chained ? kChains : 0);
 }
@@ -347,17 +379,18 @@ inline uint32_t chain_log2s(size_t n) {
 
 constexpr unsigned kHistChainBlocks = 512;
 
-// Scratch of one sort call (see enqueue_sort for the layout).
+// Scratch of one sort call with pass variant var (see enqueue_sort for the
+// layout).
 template <typename K, bool KV>
-size_t sort_scratch_bytes(size_t n) {
+size_t sort_scratch_bytes(const PassVariant& var, size_t n) {
   constexpr int D = sizeof(K);
-  const size_t rows = sort_rows<K, KV>(n);
+  const size_t rows = sort_rows(var, n);
   Layout L;
   L.take((size_t)D * kChains * 256 * 4);  // counts
   L.take((size_t)D * kChains * 256 * 4);  // bases
   L.take(D * 4);                          // tile counters
   L.take((size_t)D * rows * 256 * 4);     // look-back table
-  if (pick_variant<K, KV>().chained) {
+  if (var.chained) {
     L.take((size_t)D * 36 * 4);                                  // chain tables
     L.take((size_t)D * rows * 4);                                // claim orders
     L.take((size_t)kHistChainBlocks * D * kChains * 256 * 4);    // histogram partials
@@ -366,11 +399,10 @@ size_t sort_scratch_bytes(size_t n) {
 }
 
 template <typename K, bool KV>
-int enqueue_sort(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, char* scratch, hipStream_t s,
-                 bool float_keys) {
+int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, char* scratch,
+                 hipStream_t s, bool float_keys) {
   constexpr int D = sizeof(K);
-  const PassVariant& var = pick_variant<K, KV>();
-  const size_t rows = sort_rows<K, KV>(n);
+  const size_t rows = sort_rows(var, n);
   const size_t ncount = (size_t)D * kChains * 256;
   Layout L;
   uint32_t* counts = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
@@ -461,8 +493,9 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   const bool dv = !KV || is_device_ptr(vals);
   const bool dkt = ktmp && is_device_ptr(ktmp);
   const bool dvt = !KV || (vtmp && is_device_ptr(vtmp));
+  const PassVariant& var = pick_variant<K, KV>(n);
   Layout L;
-  const size_t o_sort = L.take(sort_scratch_bytes<K, KV>(n));
+  const size_t o_sort = L.take(sort_scratch_bytes<K, KV>(var, n));
   const size_t o_k = dk ? 0 : L.take(n * sizeof(K));
   const size_t o_kt = dkt ? 0 : L.take(n * sizeof(K));
   const size_t o_v = dv ? 0 : L.take(n * 4);
@@ -476,7 +509,7 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   uint32_t* dvtmp = KV ? (dvt ? vtmp : reinterpret_cast<uint32_t*>(b + o_vt)) : nullptr;
   if (!dk) PA_HIP(hipMemcpyAsync(dkeys, keys, n * sizeof(K), hipMemcpyHostToDevice, s));
   if (KV && !dv) PA_HIP(hipMemcpyAsync(dvals, vals, n * 4, hipMemcpyHostToDevice, s));
-  PA_TRY((enqueue_sort<K, KV>(dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, float_keys)));
+  PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, float_keys)));
   if (!dk) PA_HIP(hipMemcpyAsync(keys, dkeys, n * sizeof(K), hipMemcpyDeviceToHost, s));
   if (KV && !dv) PA_HIP(hipMemcpyAsync(vals, dvals, n * 4, hipMemcpyDeviceToHost, s));
   if (!dk || (KV && !dv)) PA_HIP(hipStreamSynchronize(s));
@@ -508,6 +541,92 @@ int pa_sort_kv_u32_u32(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t*
 
 int pa_sort_kv_u64_u32(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, size_t n, pa_stream_t stream) {
   return sort_entry<uint64_t, true>(keys, vals, ktmp, vtmp, n, (hipStream_t)stream, false, "pa_sort_kv_u64_u32");
+}
+
+int pa_autotune_sort_u32(size_t max_n, pa_stream_t stream) {
+  if (max_n < 2) return arg_error("pa_autotune_sort_u32: need max_n >= 2");
+  hipStream_t s = (hipStream_t)stream;
+  static const size_t rep[kClasses] = {size_t(1) << 16, size_t(1) << 20, size_t(1) << 24, size_t(100000000)};
+  const size_t nmax = std::min(max_n, std::min(rep[kClasses - 1], kMaxSortN));
+  constexpr size_t NV = variant_count<uint32_t, false>();
+  size_t scratch_bytes = 0;
+  for (size_t v = 0; v < NV; ++v)
+    scratch_bytes = std::max(scratch_bytes, sort_scratch_bytes<uint32_t, false>(Variants<uint32_t, false>::table[v], nmax));
+  uint32_t *pristine = nullptr, *keys = nullptr, *tmp = nullptr;
+  char* scratch = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = PA_OK;
+  auto fail = [&](hipError_t e, const char* what) {
+    st = hip_fail(e, what, __FILE__, __LINE__);
+    return st;
+  };
+  do {
+    hipError_t e;
+    if ((e = hipMalloc(&pristine, nmax * 4)) != hipSuccess || (e = hipMalloc(&keys, nmax * 4)) != hipSuccess ||
+        (e = hipMalloc(&tmp, nmax * 4)) != hipSuccess || (e = hipMalloc(&scratch, scratch_bytes)) != hipSuccess) {
+      fail(e, "hipMalloc");
+      break;
+    }
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+      fail(e, "hipEventCreate");
+      break;
+    }
+    if ((st = pa_fill_random_u32(pristine, nmax, 0x5eed, stream)) != PA_OK) break;
+    size_t prev_n = 0;
+    int prev_best = -1;
+    for (int c = 0; c < kClasses && st == PA_OK; ++c) {
+      const size_t n = std::min(rep[c], nmax);
+      if (n == prev_n) {
+        g_tuned_u32[c].store(prev_best);
+        continue;
+      }
+      float best_ms = 0.f;
+      int best = -1;
+      for (size_t v = 0; v < NV && st == PA_OK; ++v) {
+        const PassVariant& var = Variants<uint32_t, false>::table[v];
+        float total = 0.f;
+        for (int rep_i = 0; rep_i < 4 && st == PA_OK; ++rep_i) {  // first run warms up, three are timed
+          if ((e = hipMemcpyAsync(keys, pristine, n * 4, hipMemcpyDeviceToDevice, s)) != hipSuccess) {
+            fail(e, "hipMemcpyAsync");
+            break;
+          }
+          (void)hipEventRecord(e0, s);
+          if ((st = enqueue_sort<uint32_t, false>(var, keys, nullptr, tmp, nullptr, n, scratch, s, false)) != PA_OK) break;
+          (void)hipEventRecord(e1, s);
+          if ((e = hipEventSynchronize(e1)) != hipSuccess) {
+            fail(e, "hipEventSynchronize");
+            break;
+          }
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (rep_i > 0) total += ms;
+        }
+        if (st == PA_OK && (best < 0 || total < best_ms)) {
+          best_ms = total;
+          best = (int)v;
+        }
+      }
+      if (st == PA_OK) {
+        g_tuned_u32[c].store(best);
+        prev_n = n;
+        prev_best = best;
+      }
+    }
+  } while (false);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(pristine);
+  (void)hipFree(keys);
+  (void)hipFree(tmp);
+  (void)hipFree(scratch);
+  return st;
+}
+
+int pa_sort_variant_u32(size_t n) { return variant_index<uint32_t, false>(n); }
+
+int pa_autotune_reset(void) {
+  for (int c = 0; c < kClasses; ++c) g_tuned_u32[c].store(-1);
+  return PA_OK;
 }
 
 int pa_sort_two_phase_u32(uint32_t* a, uint32_t* b, size_t n, pa_stream_t stream) {

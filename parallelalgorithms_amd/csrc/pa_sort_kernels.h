@@ -145,7 +145,9 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 //
 // Segmented mode: tile_info[t] = (segment << 32) | tile index within the
 // segment; the segment's input/output addresses and length come from seg_in /
-// seg_out / seg_len and its digit counts from gtab + segment * gstride * 256.
+// seg_out / seg_len, its digit counts from gtab + segment * gstride * 256 and
+// its first look-back row from chain_info[segment] (a segment's rows are
+// contiguous; its tiles may be claimed interleaved with other segments').
 // Plain mode: gtab holds the pass's exclusive digit bases.
 //
 // Chain mode: the array is split into kChains contiguous chains, each with its
@@ -246,6 +248,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     const uint64_t info = tile_info[claimed];
     const uint32_t seg = (uint32_t)(info >> 32);
     local = (uint32_t)info;
+    row = chain_info[seg] + local;
     in = reinterpret_cast<const K*>(seg_in[seg]);
     out = reinterpret_cast<K*>(seg_out[seg]);
     len = seg_len[seg];

@@ -88,3 +88,32 @@ def test_segmented_pointer_table(pa):
     pa.segmented_sort_ptrs_u32(devs)
     for h, d in zip(hosts, devs):
         np.testing.assert_array_equal(to_host(d, np.uint32) if h.size else h, np.sort(h))
+
+
+def test_segmented_plan_graph_replays(pa):
+    """The graph-captured plan sorts the arrays' current contents on every run:
+    three runs with fresh keys written eagerly (and synchronised) between them,
+    each checked against np.sort and against the one-shot batched call."""
+    import torch
+    import parallelalgorithms_amd as P
+    lens = [1, 7, 4096, 4097, 30_000, 24_577, 100_000, 2, 333_333]
+    devs = [torch.empty(m, dtype=torch.int32, device="cuda") for m in lens]
+    plan = P.SegmentedPlan(devs)
+    try:
+        for it in range(3):
+            for i, d in enumerate(devs):
+                P.fill_random_u32(d, 1000 * it + i)
+            if it == 1:  # a presorted array in the mix
+                devs[6].copy_(torch.arange(lens[6], dtype=torch.int32, device="cuda"))
+            torch.cuda.synchronize()
+            before = [to_host(d, np.uint32).copy() for d in devs]
+            copies = [d.clone() for d in devs]
+            plan.run()
+            torch.cuda.synchronize()
+            pa.segmented_sort_ptrs_u32(copies)
+            for h, d, c in zip(before, devs, copies):
+                got = to_host(d, np.uint32)
+                np.testing.assert_array_equal(got, np.sort(h))
+                np.testing.assert_array_equal(got, to_host(c, np.uint32))
+    finally:
+        plan.close()

@@ -119,3 +119,20 @@ def test_sort_f32_order_transform(pa, orc, n):
     d = to_dev(bits.view(np.float32))
     pa.sort_f32(d, dev_empty(n, np.float32))
     np.testing.assert_array_equal(to_host(d, np.float32).view(np.uint32), expect)
+
+
+def test_autotune_then_sort(pa):
+    """After the tuner records a pass shape per size class, sorts in every
+    class still match np.sort; the choice is a valid candidate index."""
+    import parallelalgorithms_amd as P
+    P.autotune_sort_u32(1 << 20)
+    try:
+        for n in (1000, 70_001, 1 << 20, (1 << 20) + 3):
+            v = P.sort_variant_u32(n)
+            assert 0 <= v < 6
+            a = P.gen_u32(7 + n, n)
+            d = to_dev(a)
+            pa.sort_u32(d)
+            np.testing.assert_array_equal(to_host(d, np.uint32), np.sort(a))
+    finally:
+        P.autotune_reset()
