@@ -278,9 +278,11 @@ struct PassVariant {
   int (*launch)(hipStream_t s, const PassArgs& a);
 };
 
-template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool LATE, int MINW, int MODE, bool UNI, int SPLIT>
+template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool LATE, int MINW, int MODE, bool UNI, int SPLIT,
+          int UL>
 int launch_pass(hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI, SPLIT>), dim3(a.ntiles), dim3(BLOCK), 0, s,
+  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI, SPLIT, UL>), dim3(a.ntiles), dim3(BLOCK),
+                     0, s,
                      static_cast<const K*>(a.kin), static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr,
                      a.ntiles, a.shift, a.fmode, nullptr, nullptr, nullptr, nullptr, 0u, a.chain_info, a.chain_tiles);
   PA_HIP(hipGetLastError());
@@ -288,9 +290,9 @@ int launch_pass(hipStream_t s, const PassArgs& a) {
 }
 
 // PA_VARIANT(key, payload, block, items, look-back width, late look-back, min waves/SIMD, chained,
-//            uniform-digit fast path, staging rounds)
-#define PA_VARIANT(K, KV, B, I, W, P, M, C, U, S) \
-  { B, I, W, P, M, C, &launch_pass<K, KV, B, I, W, P, M, (C) ? kModeChain : kModePlain, U, S> }
+//            uniform-digit fast path, staging rounds, lanes per ranking unit)
+#define PA_VARIANT(K, KV, B, I, W, P, M, C, U, S, L) \
+  { B, I, W, P, M, C, &launch_pass<K, KV, B, I, W, P, M, (C) ? kModeChain : kModePlain, U, S, L> }
 
 // Size classes of a sort call: n <= 2^16, <= 2^20, <= 2^24, larger.
 constexpr int kClasses = 4;
@@ -304,30 +306,32 @@ struct Variants;
 template <>
 struct Variants<uint32_t, false> {
   static constexpr PassVariant table[] = {
-      PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1),
-      PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1),  PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 1, true, 2),
-      PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2),
+      PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2, 32),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 32),
+      PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 32),  PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 1, true, 2, 32),
+      PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2, 32), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2, 32),
+      PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2, 16), PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 16),
+      PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 16),
   };
-  static constexpr int by_class[kClasses] = {2, 2, 0, 0};
+  static constexpr int by_class[kClasses] = {2, 2, 6, 6};
 };
 template <>
 struct Variants<uint32_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 0, true, 1),
-                                          PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 1, true, 1)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 0, true, 1, 32),
+                                          PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 1, true, 1, 32)};
   static constexpr int by_class[kClasses] = {0, 0, 0, 0};
 };
 template <>
 struct Variants<uint64_t, false> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 0, true, 1),
-                                          PA_VARIANT(uint64_t, false, 512, 16, 8, true, 6, 0, true, 2),
-                                          PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true, 1),
-                                          PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true, 1)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 0, true, 1, 32),
+                                          PA_VARIANT(uint64_t, false, 512, 16, 8, true, 6, 0, true, 2, 32),
+                                          PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true, 1, 32),
+                                          PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true, 1, 32)};
   static constexpr int by_class[kClasses] = {0, 0, 0, 0};
 };
 template <>
 struct Variants<uint64_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 0, true, 1),
-                                          PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 1, true, 1)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 0, true, 1, 32),
+                                          PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 1, true, 1, 32)};
   static constexpr int by_class[kClasses] = {0, 0, 0, 0};
 };
 

@@ -125,17 +125,17 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 // Tiles are claimed in order from a global counter, so every tile a block
 // waits on in the look-back is held by a block that is already running.
 //
-// Ranking units are half-waves (32 lanes).  Unit u owns the USPAN = 32 x
-// ITEMS consecutive keys [u*USPAN, (u+1)*USPAN) of the tile; item i of lane l
-// is key u*USPAN + i*32 + l, so (unit, item, lane) order is input order and a
-// load instruction reads two 128-byte runs.  Each unit has 256 LDS slots of
-// 64 bits: the low word collects the lanes holding a digit in the current
-// item (each lane ORs its bit), the high word counts that digit's keys seen so
-// far in the unit.  One read returns both, so a key's stable rank in its unit
-// is count + (peer lanes below it); the lowest peer then writes back
-// (count + peers, 0), which also clears the lane mask.  LDS operations of
-// one wave execute in program order and OR commutes, so the result does not
-// depend on the order in which the LDS applies the lanes.
+// Ranking units are groups of UL lanes (32 or 16).  Unit u owns the USPAN =
+// UL x ITEMS consecutive keys [u*USPAN, (u+1)*USPAN) of the tile; item i of
+// lane l is key u*USPAN + i*UL + l, so (unit, item, lane) order is input order
+// and a load instruction reads 64/UL contiguous runs.  Each unit has 256 LDS
+// slots: one part collects the lanes holding a digit in the current item
+// (each lane ORs its bit), the other counts that digit's keys seen so far in
+// the unit.  One read returns both, so a key's stable rank in its unit is
+// count + (peer lanes below it); the lowest peer then writes back
+// (count + peers, no lanes), which also clears the lane mask.  LDS operations
+// of one wave execute in program order and OR commutes, so the result does
+// not depend on the order in which the LDS applies the lanes.
 //
 // Partial tiles are padded with all-ones keys: their digit is 255 in every
 // pass, so they rank after every real key of the tile and occupy the staging
@@ -164,12 +164,16 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 //
 // MINW: minimum waves per SIMD the register allocation must allow.
 // UNI: items whose 64 lanes all hold one digit take a path without LDS atomics.
+// UL: lanes per ranking unit.  32: 64-bit slots (32-bit lane mask | 32-bit
+// count).  16: 32-bit slots (16-bit lane mask | 16-bit count); a slot is one
+// LDS bank word, so the 64 lanes of a rank step spread over every bank instead
+// of every second one (the 64-bit slots start on even banks).
 // SPLIT: the tile is staged and written out in SPLIT rounds of TILE / SPLIT
 // slots (round r takes the keys whose staging slot is in its range), so a
 // tile larger than the LDS staging area can be held in registers: the block
 // then has more keys in flight per byte of LDS and per look-back.
 template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, int MODE = kModePlain, bool LATE = false, int MINW = 1,
-          bool UNI = true, int SPLIT = 1>
+          bool UNI = true, int SPLIT = 1, int UL = 32>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void k_onesweep(
     const K* __restrict__ kin, K* __restrict__ kout, const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
     uint32_t n, const uint32_t* __restrict__ gtab, uint32_t* __restrict__ lb, uint32_t* __restrict__ tile_counter,
@@ -181,13 +185,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   constexpr bool CHAIN = MODE == kModeChain;
   static_assert(BLOCK >= 256 && BLOCK <= 1024 && BLOCK % 64 == 0, "one thread per digit is needed");
   static_assert(!(SEG && KV), "segmented mode sorts keys only");
-  constexpr int UNITS = BLOCK / 32;
-  constexpr int USPAN = 32 * ITEMS;
+  static_assert(UL == 32 || UL == 16, "ranking units are 16 or 32 lanes");
+  constexpr int UNITS = BLOCK / UL;
+  constexpr int USPAN = UL * ITEMS;
+  constexpr int SLOTB = UL == 32 ? 8 : 4;  // bytes per rank slot
+  static_assert(UL == 32 || BLOCK * ITEMS <= 65535, "16-bit slot counts need tiles below 2^16 keys");
   constexpr int TILE = BLOCK * ITEMS;
   static_assert(ITEMS % SPLIT == 0 && (SPLIT == 1 || !KV), "split staging is for keys-only tiles");
   constexpr int STAGE = TILE / SPLIT;      // staging slots per round
   constexpr int RITEMS = ITEMS / SPLIT;    // write-out slots per thread per round
-  static_assert(STAGE * sizeof(K) >= UNITS * 256 * sizeof(uint64_t), "rank slots must fit the key staging area");
+  static_assert(STAGE * sizeof(K) >= UNITS * 256 * SLOTB, "rank slots must fit the key staging area");
   __shared__ __align__(16) K s_keys[STAGE];
   __shared__ uint32_t s_goff[256];
   __shared__ uint32_t s_wsum[2][4];
@@ -196,8 +203,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
   const uint32_t wave = tid >> 6;
-  const uint32_t unit = tid >> 5;
-  const uint32_t ul = tid & 31;
+  const uint32_t unit = tid / UL;
+  const uint32_t ul = tid % UL;
   const uint32_t lanebit = 1u << ul;
   // Rank slots of this thread's unit (they alias the key staging area, which
   // is idle until the ranking is over).
@@ -205,6 +212,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   uint32_t* const uword = reinterpret_cast<uint32_t*>(uslot);
   // High words of every unit's slots, for the per-digit pass over units.
   uint32_t* const hiword = reinterpret_cast<uint32_t*>(s_keys) + 1;
+  // 16-lane units: slot (u, d) is word u * 256 + d, count in the high half.
+  uint32_t* const slot16 = reinterpret_cast<uint32_t*>(s_keys);
+  uint32_t* const uslot16 = slot16 + unit * 256;
   const bool float_in = sizeof(K) == 4 && (fmode & kFloatIn);
   const bool float_out = sizeof(K) == 4 && (fmode & kFloatOut);
 
@@ -212,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   {
     uint4* z = reinterpret_cast<uint4*>(s_keys);
 #pragma unroll
-    for (int r = 0; r < UNITS * 256 * 8 / 16 / BLOCK; ++r) z[tid + r * BLOCK] = make_uint4(0, 0, 0, 0);
+    for (int r = 0; r < UNITS * 256 * SLOTB / 16 / BLOCK; ++r) z[tid + r * BLOCK] = make_uint4(0, 0, 0, 0);
   }
   if (CHAIN) ntiles = chain_info[32];
   if (tid == 0) s_claim = atomicAdd(tile_counter, 1u);
@@ -272,7 +282,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     const K* src = in + first;
     if (valid == (uint32_t)TILE) {
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) key[i] = src[i * 32];
+      for (int i = 0; i < ITEMS; ++i) key[i] = src[i * UL];
       if (float_in) {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) key[i] = (K)dev::f32_fwd((uint32_t)key[i]);
@@ -281,8 +291,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) {
         K v = ~K(0);
-        if (first + i * 32 < len) {
-          v = src[i * 32];
+        if (first + i * UL < len) {
+          v = src[i * UL];
           if (float_in) v = (K)dev::f32_fwd((uint32_t)v);
         }
         key[i] = v;
@@ -294,7 +304,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     const uint32_t first = tile_first + unit * USPAN + ul;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t idx = first + i * 32;
+      const uint32_t idx = first + i * UL;
       val[i] = idx < len ? vsrc[idx] : 0u;
     }
   }
@@ -330,12 +340,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     const uint32_t d = dev::digit8<K>(key[i], shift);
     if (UNI && __all(d == __builtin_amdgcn_readfirstlane(d))) {
       // Every lane holds the same digit (constant or presorted runs): the
-      // peers of each unit are its 32 lanes, so no lane mask is needed and 32
-      // lanes do not serialise on one LDS atomic.
-      const uint32_t seen = uword[2 * d + 1];
-      set_rank(i, seen + ul);
-      if (ul == 0) uword[2 * d + 1] = seen + 32u;
-    } else {
+      // peers of each unit are all its lanes, so no lane mask is needed and
+      // the lanes do not serialise on one LDS atomic.
+      if constexpr (UL == 32) {
+        const uint32_t seen = uword[2 * d + 1];
+        set_rank(i, seen + ul);
+        if (ul == 0) uword[2 * d + 1] = seen + 32u;
+      } else {
+        const uint32_t seen = uslot16[d] >> 16;
+        set_rank(i, seen + ul);
+        if (ul == 0) uslot16[d] = (seen + 16u) << 16;
+      }
+    } else if constexpr (UL == 32) {
       atomicOr(&uword[2 * d], lanebit);
       const uint64_t sv = uslot[d];
       const uint32_t peers = (uint32_t)sv;
@@ -343,6 +359,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       const uint32_t below = __popc(peers & (lanebit - 1u));
       set_rank(i, seen + below);
       if (below == 0) uslot[d] = (uint64_t)(seen + __popc(peers)) << 32;
+    } else {
+      atomicOr(&uslot16[d], lanebit);
+      const uint32_t sv = uslot16[d];
+      const uint32_t peers = sv & 0xFFFFu;
+      const uint32_t seen = sv >> 16;
+      const uint32_t below = __popc(peers & (lanebit - 1u));
+      set_rank(i, seen + below);
+      if (below == 0) uslot16[d] = (seen + __popc(peers)) << 16;
     }
   }
   __syncthreads();
@@ -354,9 +378,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   if (tid < 256) {
 #pragma unroll
     for (int u = 0; u < UNITS; ++u) {
-      const uint32_t c = hiword[2 * (u * 256 + tid)];
-      hiword[2 * (u * 256 + tid)] = cnt;
-      cnt += c;
+      if constexpr (UL == 32) {
+        const uint32_t c = hiword[2 * (u * 256 + tid)];
+        hiword[2 * (u * 256 + tid)] = cnt;
+        cnt += c;
+      } else {
+        const uint32_t c = slot16[u * 256 + tid] >> 16;
+        slot16[u * 256 + tid] = cnt << 16;
+        cnt += c;
+      }
     }
     dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
     if constexpr (SEG) gval = g[tid];
@@ -418,7 +448,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     tstart = cpre + cinc - cnt;  // first staging slot of this digit
     gstart = SEG ? gpre + ginc - gval : g[tid];
 #pragma unroll
-    for (int u = 0; u < UNITS; ++u) hiword[2 * (u * 256 + tid)] += tstart;
+    for (int u = 0; u < UNITS; ++u) {
+      if constexpr (UL == 32)
+        hiword[2 * (u * 256 + tid)] += tstart;
+      else
+        slot16[u * 256 + tid] += tstart << 16;
+    }
     if (!LATE) resolve(false);
   }
   __syncthreads();
@@ -434,7 +469,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     for (int i = g0; i < g0 + LG && i < ITEMS; ++i) asm volatile("" : "+v"(key[i]));
 #pragma unroll
     for (int i = g0; i < g0 + LG && i < ITEMS; ++i) {
-      const uint32_t start = hiword[2 * (unit * 256 + dev::digit8<K>(key[i], shift))];
+      const uint32_t dg = dev::digit8<K>(key[i], shift);
+      const uint32_t start = UL == 32 ? hiword[2 * (unit * 256 + dg)] : uslot16[dg] >> 16;
       if constexpr (PACK)
         rank[i >> 1] += (i & 1) ? start << 16 : start;  // halves stay below 2^16: no carry
       else
