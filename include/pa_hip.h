@@ -51,6 +51,10 @@ int pa_version(void);
 int pa_set_device(int device);
 /* Blocks until all work enqueued on `stream` has finished. */
 int pa_synchronize(pa_stream_t stream);
+/* Copies `bytes` bytes between any mix of host and device memory, stream-
+ * ordered (blocking when host memory is involved).  Used by the drop-in
+ * headers where a reference call puts its result in a second buffer. */
+int pa_copy(void* dst, const void* src, size_t bytes, pa_stream_t stream);
 
 /* ---- LSD radix sort (ascending, stable) --------------------------------- */
 
@@ -91,6 +95,17 @@ int pa_sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, pa_stream_t stream);
  * kept.  No reference counterpart (RadixSortCommon.h:62-68 is the only order
  * transform in the reference, for signed integers, and is unused). */
 int pa_sort_f32(float* keys, float* tmp, size_t n, pa_stream_t stream);
+
+/* float64 keys through the 64-bit form of the same transform, and signed
+ * 32/64-bit integers through a sign-bit flip (the order transform the
+ * reference writes for signed digits, extractDigitNegate,
+ * RadixSortCommon.h:62-68).  These are the element types of the reference's
+ * comparison sorts on this path (sort_par, SortParallel.h:24-94, and the
+ * merge sorts of ParallelMergeSort.h, which its benchmark runs on doubles,
+ * ParallelMergeSortBenchmark.cpp:61). */
+int pa_sort_f64(double* keys, double* tmp, size_t n, pa_stream_t stream);
+int pa_sort_i32(int32_t* keys, int32_t* tmp, size_t n, pa_stream_t stream);
+int pa_sort_i64(int64_t* keys, int64_t* tmp, size_t n, pa_stream_t stream);
 
 /* Stable key-value sorts: keys ascending, vals permuted with their keys, ties
  * keep input order.  No reference counterpart (new component). */
@@ -137,6 +152,20 @@ int pa_start_of_bins_u32(const uint32_t* keys, size_t n, size_t quantum, size_t 
  * (HistogramParallel.h:16-247). */
 int pa_hist_u8(const uint8_t* bytes, size_t n, uint64_t* counts, pa_stream_t stream);
 
+/* ---- Counting sort and fill ------------------------------------------------ */
+
+/* Sorts a byte array in place by counting: one histogram read, then each bin's
+ * value written over its range.  Replaces counting_sort_parallel(a, a_size)
+ * and counting_sort_parallel_inner<256>(a, l, r, threshold_count,
+ * threshold_fill) (CountingSortParallel.h:29-90).  n < 2^32. */
+int pa_counting_sort_u8(uint8_t* bytes, size_t n, pa_stream_t stream);
+
+/* Writes n copies of the elem_bytes-byte value *value (a host pointer) to a,
+ * which must be aligned to elem_bytes (1, 2, 4, 8 or 16).  Replaces
+ * parallel_fill(src, value, l, r, thr) (FillParallel.h:22-66) with
+ * a = src + l, n = r - l. */
+int pa_fill(void* a, size_t elem_bytes, const void* value, size_t n, pa_stream_t stream);
+
 /* ---- Radix select ---------------------------------------------------------- */
 
 /* *result = k-th smallest (0-based) of keys[0..n).  keys is left unmodified.
@@ -154,6 +183,21 @@ int pa_select_u64(const uint64_t* keys, size_t n, size_t k, uint64_t* result, pa
  * RadixSelectBenchmark.cpp:57-58).  The rest of keys is unchanged (the
  * reference leaves it reordered and partly overwritten, RadixSelect.h:52). */
 int pa_select_radix_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream);
+
+/* ---- Radix partition (nth_element) ---------------------------------------- */
+
+/* Partitions keys[0..n) in place so that keys[k] holds the k-th smallest key,
+ * no key before it is larger and no key after it is smaller; *result = that
+ * key.  Replaces PartitionRadix(unsigned[], size_t aLength, size_t k) and
+ * PartitionRadix(unsigned[], size_t aStart, size_t aLength, size_t k)
+ * (RadixPartition.h:94-145).  The order of the other keys within each side is
+ * their input order (the reference's is its in-place swap order). */
+int pa_partition_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream);
+
+/* The same for several ranks ks[0..num_ks) (ascending, each < n) at once.
+ * Replaces PartitionRadix(unsigned[], size_t aLength, size_t k[], size_t
+ * kLength) and the aStart/kStart form (RadixPartition.h:147-184). */
+int pa_partition_multi_u32(uint32_t* keys, size_t n, const size_t* ks, size_t num_ks, pa_stream_t stream);
 
 /* ---- Merge ----------------------------------------------------------------- */
 

@@ -31,13 +31,18 @@ _SIGS = {
     "orc_ordered_to_f32": (_u32, [_u32]),
     "orc_f32_transform_exhaustive_check": (_u64, []),
     "orc_sort_f32": (None, [_vp, _vp, _sz, _sz]),
+    "orc_sort_f64": (None, [_vp, _vp, _sz, _sz]),
+    "orc_sort_i32": (None, [_vp, _vp, _sz, _sz]),
+    "orc_sort_i64": (None, [_vp, _vp, _sz, _sz]),
     "orc_hist_byte_components_u32": (None, [_vp, _sz, _sz, _sz, _vp]),
     "orc_hist_qc_par_u32": (None, [_vp, _sz, _sz, _sz, _sz, _u, _sz, _vp]),
     "orc_compute_start_of_bins_u32": (None, [_vp, _sz, _sz, _sz, _u, _sz, _vp]),
     "orc_hist_one_byte_u32": (None, [_vp, _sz, _sz, _u, _u, _sz, _vp]),
     "orc_hist_one_byte_2_bugcompat_u32": (None, [_vp, _sz, _sz, _u, _sz, _vp]),
     "orc_hist_bytes": (None, [_vp, _sz, _sz, _sz, _vp]),
+    "orc_counting_sort_u8": (None, [_vp, _sz, _sz, _sz]),
     "orc_select_radix_u32": (_u32, [_vp, _sz, _sz, _sz, _vp]),
+    "orc_partition_radix_u32": (None, [_vp, _sz, _sz, _vp, _sz]),
     "orc_merge_parallel_l5_u32": (None, [_vp, _sz, _sz, _sz, _sz, _vp, _sz, _sz]),
     "orc_merge_parallel_l5_u64": (None, [_vp, _sz, _sz, _sz, _sz, _vp, _sz, _sz]),
     "orc_counting_pass_u32": (None, [_vp, _vp, _sz, _u]),
@@ -139,6 +144,28 @@ def sort_f32_bits(bits, thr=512 * 1024):
     return bits
 
 
+def sort_f64_bits(bits, thr=512 * 1024):
+    """float64 bit patterns sorted in the total order of the transformed bits."""
+    bits = np.array(bits, dtype=np.uint64, copy=True)
+    tmp = np.empty_like(bits)
+    load().orc_sort_f64(_p(bits), _p(tmp), bits.size, thr)
+    return bits
+
+
+def sort_i32(a, thr=512 * 1024):
+    a = np.array(a, dtype=np.int32, copy=True)
+    tmp = np.empty_like(a)
+    load().orc_sort_i32(_p(a), _p(tmp), a.size, thr)
+    return a
+
+
+def sort_i64(a, thr=512 * 1024):
+    a = np.array(a, dtype=np.int64, copy=True)
+    tmp = np.empty_like(a)
+    load().orc_sort_i64(_p(a), _p(tmp), a.size, thr)
+    return a
+
+
 def f32_to_ordered(u):
     return load().orc_f32_to_ordered(u)
 
@@ -190,6 +217,15 @@ def hist_bytes(a, l, r, thr=64 * 1024):
     return out
 
 
+def counting_sort_u8(a, l=0, r=None, thr=64 * 1024):
+    """counting_sort_parallel_inner<256> on a copy, sorting [l, r)."""
+    a = np.array(a, dtype=np.uint8, copy=True)
+    if r is None:
+        r = a.size
+    load().orc_counting_sort_u8(_p(a), l, r, thr)
+    return a
+
+
 def select_radix_u32(a, k, start=0, length=None):
     """SelectRadix on a copy; returns (value, candidate counts per pass)."""
     a = np.array(a, dtype=np.uint32, copy=True)
@@ -198,6 +234,16 @@ def select_radix_u32(a, k, start=0, length=None):
     pc = np.zeros(4, dtype=np.uint64)
     v = load().orc_select_radix_u32(_p(a), start, length, k, _p(pc))
     return v, pc
+
+
+def partition_radix_u32(a, ks, start=0, length=None):
+    """PartitionRadix on a copy with ranks ks (ascending, absolute); returns the array."""
+    a = np.array(a, dtype=np.uint32, copy=True)
+    if length is None:
+        length = a.size - start
+    k = np.ascontiguousarray(ks, dtype=np.uint64)
+    load().orc_partition_radix_u32(_p(a), start, length, _p(k), k.size)
+    return a
 
 
 def merge_parallel_l5_u32(t, p1, r1, p2, r2, out, p3, thr=32768):

@@ -556,8 +556,70 @@ void merge_l5(const T* t, size_t p1, size_t r1, size_t p2, size_t r2, T* a, size
   });
 }
 
+// ---------------------------------------------------------------------------
+// PartitionRadixMsdUIntInner (RadixPartition.h:30-86): MSD in-place radix
+// partition.  One 256-bin count of the current digit over [start, start+size),
+// an in-place cycle permutation into the bins (each element is swapped into
+// the next free slot of its bin until the slot it came from receives an
+// element of the bin being filled), then recursion into only the bins that
+// hold one of the requested ranks k[kStart .. kStart+kLength) (k sorted
+// ascending); bins below Threshold (48) are finished by insertion sort.
+// ---------------------------------------------------------------------------
+template <class T>
+void partition_radix_inner(T* a, size_t start, size_t size, int shift, const size_t* k, size_t k0, size_t kn) {
+  constexpr size_t R = 256;
+  constexpr long Threshold = 48;
+  const size_t last = start + size;
+  size_t count[R] = {0};
+  for (size_t i = start; i < last; ++i) count[(a[i] >> shift) & 0xFF]++;
+  size_t bin_start[R + 1], bin_end[R];
+  bin_start[0] = bin_end[0] = start;
+  bin_start[R] = start;  // sentinel, as in the reference (:44)
+  for (size_t i = 1; i < R; ++i) bin_start[i] = bin_end[i] = bin_start[i - 1] + count[i - 1];
+  size_t next_bin = 1;
+  for (size_t cur = start; cur < last;) {
+    unsigned digit;
+    T e = a[cur];
+    while (bin_end[digit = (unsigned)((e >> shift) & 0xFF)] != cur) std::swap(e, a[bin_end[digit]++]);
+    a[cur] = e;
+    bin_end[digit]++;
+    while (bin_end[next_bin - 1] == bin_start[next_bin]) next_bin++;
+    cur = bin_end[next_bin - 1];
+  }
+  if (shift <= 0) return;
+  shift = shift >= 8 ? shift - 8 : 0;
+  size_t ki = k0;
+  for (size_t bin = 0; bin < R && ki < k0 + kn; ++bin) {
+    if (bin_start[bin] < bin_end[bin] && k[ki] >= bin_start[bin] && k[ki] < bin_end[bin]) {
+      const size_t first_k = ki++;
+      size_t nk = 1;
+      for (; ki < k0 + kn && k[ki] >= bin_start[bin] && k[ki] < bin_end[bin]; ++ki) ++nk;
+      const size_t m = bin_end[bin] - bin_start[bin];
+      if ((long)m >= Threshold)
+        partition_radix_inner(a, bin_start[bin], m, shift, k, first_k, nk);
+      else
+        insertion_sort(&a[bin_start[bin]], m);
+    }
+  }
+}
+
 inline uint32_t f32_to_ordered(uint32_t u) { return u ^ ((u >> 31) ? 0xFFFFFFFFu : 0x80000000u); }
 inline uint32_t ordered_to_f32(uint32_t v) { return v ^ ((v >> 31) ? 0x80000000u : 0xFFFFFFFFu); }
+inline uint64_t f64_to_ordered(uint64_t u) { return u ^ ((u >> 63) ? ~0ull : 1ull << 63); }
+inline uint64_t ordered_to_f64(uint64_t v) { return v ^ ((v >> 63) ? 1ull << 63 : ~0ull); }
+
+// Signed keys: flipping the sign bit maps two's-complement order onto unsigned
+// order (the top-digit flip of extractDigitNegate, RadixSortCommon.h:62-68).
+template <typename U>
+void sort_signed(U* a, U* tmp, size_t n, size_t thr) {
+  const U top = U(1) << (sizeof(U) * 8 - 1);
+  for (size_t i = 0; i < n; ++i) a[i] ^= top;
+  if (n >= 100)
+    sort_inner_par<U>(a, tmp, n, quantum_3arg(n, thr));
+  else
+    insertion_sort(a, n);
+  for (size_t i = 0; i < n; ++i) a[i] ^= top;
+}
 
 }  // namespace
 
@@ -666,6 +728,20 @@ void orc_sort_f32(uint32_t* bits, uint32_t* tmp, size_t n, size_t thr) {
   for (size_t i = 0; i < n; ++i) bits[i] = ordered_to_f32(bits[i]);
 }
 
+// float64 bit patterns: the 64-bit form of the float transform around the
+// u64 LSD restatement.
+void orc_sort_f64(uint64_t* bits, uint64_t* tmp, size_t n, size_t thr) {
+  for (size_t i = 0; i < n; ++i) bits[i] = f64_to_ordered(bits[i]);
+  if (n >= 100)
+    sort_inner_par<uint64_t>(bits, tmp, n, quantum_3arg(n, thr));
+  else
+    insertion_sort(bits, n);
+  for (size_t i = 0; i < n; ++i) bits[i] = ordered_to_f64(bits[i]);
+}
+
+void orc_sort_i32(uint32_t* a, uint32_t* tmp, size_t n, size_t thr) { sort_signed<uint32_t>(a, tmp, n, thr); }
+void orc_sort_i64(uint64_t* a, uint64_t* tmp, size_t n, size_t thr) { sort_signed<uint64_t>(a, tmp, n, thr); }
+
 // HistogramByteComponentsParallel<256,8>(a, l, r, thr): out is 8 x 256.
 void orc_hist_byte_components_u32(const uint32_t* a, size_t l, size_t r, size_t thr, uint64_t* out) {
   hist_byte_components_u32(a, l, r, thr, out);
@@ -750,6 +826,22 @@ void orc_hist_bytes(const uint8_t* a, size_t l, size_t r, size_t thr, uint64_t* 
     for (unsigned j = 0; j < 256; ++j) out[j] += part[li * 256 + j];
 }
 
+// counting_sort_parallel_inner<256> (CountingSortParallel.h:29-76): the byte
+// histogram of [l, r), bin starts by an exclusive scan, then every bin's value
+// written over its run (parallel_fill, FillParallel.h:45-66).  The reference's
+// bin starts begin at index 0 (:56-59), so for l > 0 it writes [0, r - l); this
+// restatement writes [l, r), the sorted range the function's name promises.
+void orc_counting_sort_u8(uint8_t* a, size_t l, size_t r, size_t thr) {
+  if (r <= l) return;
+  uint64_t counts[256];
+  orc_hist_bytes(a, l, r, thr, counts);
+  size_t start = l;
+  for (int b = 0; b < 256; ++b) {
+    std::memset(a + start, b, counts[b]);
+    start += counts[b];
+  }
+}
+
 // SelectRadix(a, start, len, k) / SelectRadix(a, len, k) (RadixSelect.h:139-168).
 // Mutates `a` like the reference.  pass_counts (4 entries, may be NULL)
 // receives the candidate-range length at the start of each digit pass.
@@ -757,6 +849,13 @@ uint32_t orc_select_radix_u32(uint32_t* a, size_t start, size_t len, size_t k, s
   if (pass_counts)
     for (int i = 0; i < 4; ++i) pass_counts[i] = 0;
   return select_radix_inner(a, start, len, 24, 8, k, pass_counts);
+}
+
+// PartitionRadix(a, start, len, k[], kStart, kLength) and the single-k forms
+// (RadixPartition.h:94-184): a[start, start+len) partitioned in place so that
+// each a[k[i]] holds its rank's value; k ascending, absolute indices.
+void orc_partition_radix_u32(uint32_t* a, size_t start, size_t len, const size_t* k, size_t k_len) {
+  partition_radix_inner(a, start, len, 24, k, 0, k_len);
 }
 
 // merge_parallel_L5<T>(t, p1, r1, p2, r2, a, p3, thr) (ParallelMerge.h:285-315).

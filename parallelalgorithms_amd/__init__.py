@@ -38,6 +38,12 @@ _SIGS = {
     "pa_autotune_reset": (_i, []),
     "pa_sort_u64": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_f32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_sort_f64": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_counting_sort_u8": (_i, [_vp, _sz, _vp]),
+    "pa_copy": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_fill": (_i, [_vp, _sz, _vp, _sz, _vp]),
+    "pa_sort_i32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_sort_i64": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_kv_u32_u32": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
     "pa_sort_kv_u64_u32": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
     "pa_hist_digits_u32": (_i, [_vp, _sz, _vp, _vp]),
@@ -48,6 +54,8 @@ _SIGS = {
     "pa_hist_u8": (_i, [_vp, _sz, _vp, _vp]),
     "pa_select_u32": (_i, [_vp, _sz, _sz, _vp, _vp]),
     "pa_select_u64": (_i, [_vp, _sz, _sz, _vp, _vp]),
+    "pa_partition_u32": (_i, [_vp, _sz, _sz, _vp, _vp]),
+    "pa_partition_multi_u32": (_i, [_vp, _sz, _vp, _sz, _vp]),
     "pa_select_radix_u32": (_i, [_vp, _sz, _sz, _vp, _vp]),
     "pa_merge_u32": (_i, [_vp, _sz, _vp, _sz, _vp, _vp]),
     "pa_merge_u64": (_i, [_vp, _sz, _vp, _sz, _vp, _vp]),
@@ -180,6 +188,33 @@ def sort_f32(keys, tmp=None, stream=None):
     call("pa_sort_f32", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
 
+def counting_sort_u8(data, stream=None):
+    call("pa_counting_sort_u8", _ptr(data), numel(data), _stream(stream))
+
+
+def fill(a, value, stream=None):
+    """Fills every element of a (numpy array or torch tensor) with value, a
+    numpy scalar of the element's size (its bytes are copied as they are)."""
+    import numpy as np
+    itemsize = a.element_size() if hasattr(a, "element_size") else a.itemsize
+    v = np.ascontiguousarray(np.asarray(value).reshape(1))
+    if v.itemsize != itemsize:
+        raise ValueError("fill: value must have the element size of a")
+    call("pa_fill", _ptr(a), itemsize, v.ctypes.data, numel(a), _stream(stream))
+
+
+def sort_f64(keys, tmp=None, stream=None):
+    call("pa_sort_f64", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
+
+
+def sort_i32(keys, tmp=None, stream=None):
+    call("pa_sort_i32", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
+
+
+def sort_i64(keys, tmp=None, stream=None):
+    call("pa_sort_i64", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
+
+
 def sort_kv_u32_u32(keys, vals, ktmp=None, vtmp=None, stream=None):
     call("pa_sort_kv_u32_u32", _ptr(keys), _ptr(vals), _ptr(ktmp), _ptr(vtmp), numel(keys), _stream(stream))
 
@@ -218,6 +253,16 @@ def select_u32(keys, k, result, stream=None):
 
 def select_u64(keys, k, result, stream=None):
     call("pa_select_u64", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
+
+
+def partition_u32(keys, k, result, stream=None):
+    call("pa_partition_u32", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
+
+
+def partition_multi_u32(keys, ks, stream=None):
+    import numpy as np
+    ks = np.ascontiguousarray(ks, dtype=np.uint64)
+    call("pa_partition_multi_u32", _ptr(keys), numel(keys), ks.ctypes.data, ks.size, _stream(stream))
 
 
 def select_radix_u32(keys, k, result, stream=None):

@@ -96,6 +96,60 @@ __global__ __launch_bounds__(kHistBlock) void k_hist_u8(const uint8_t* __restric
   if (tid < 256 && s_bins[tid]) atomicAdd(&counts[tid], s_bins[tid]);
 }
 
+// Counting sort of bytes, the fill half (CountingSortParallel.h:30-76: after
+// the histogram, bin b's value fills [start[b], start[b] + count[b])).  Every
+// block scans the 256 counts in LDS, then each thread writes whole 16-byte
+// words: the bin of a word's first byte by binary search over the bin starts,
+// later bytes by stepping to the next non-empty bin.  Pure write traffic.
+__global__ __launch_bounds__(kHistBlock) void k_counting_fill(uint8_t* __restrict__ a, size_t n,
+                                                              const uint32_t* __restrict__ counts) {
+  __shared__ uint32_t s_end[256];  // s_end[b] = first index past bin b
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ uint32_t s_w[4];
+  if (tid < 256) {
+    const uint32_t c = counts[tid];
+    const uint32_t inc = dev::wave_incl_scan(c, lane);
+    if (lane == 63) s_w[wave] = inc;
+    s_end[tid] = inc;  // wave-local for now
+  }
+  __syncthreads();
+  if (tid < 256) {
+    uint32_t pre = 0;
+    for (uint32_t w = 0; w < wave; ++w) pre += s_w[w];
+    s_end[tid] += pre;
+  }
+  __syncthreads();
+  // bin of index i: the first b with s_end[b] > i
+  auto bin_of = [&](size_t i) {
+    uint32_t lo = 0, hi = 255;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_end[mid] > i) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+  };
+  const size_t head = std::min<size_t>(n, (16 - ((uintptr_t)a & 15)) & 15);
+  const size_t nvec = (n - head) / 16;
+  uint4* v = reinterpret_cast<uint4*>(a + head);
+  const size_t stride = (size_t)gridDim.x * kHistBlock;
+  for (size_t w = (size_t)blockIdx.x * kHistBlock + tid; w < nvec; w += stride) {
+    const size_t i0 = head + w * 16;
+    uint32_t b = bin_of(i0);
+    uint32_t word[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      while (s_end[b] <= i0 + j) ++b;  // i0 + j < n = s_end[255], so b stays below 256
+      word[j >> 2] |= b << (8 * (j & 3));
+    }
+    v[w] = make_uint4(word[0], word[1], word[2], word[3]);
+  }
+  if (blockIdx.x == 0) {
+    const size_t tail = head + nvec * 16;
+    for (size_t i = tid; i < head; i += kHistBlock) a[i] = (uint8_t)bin_of(i);
+    for (size_t i = tail + tid; i < n; i += kHistBlock) a[i] = (uint8_t)bin_of(i);
+  }
+}
+
 // Per-work-quantum counts of one byte over keys[first, last): q = index /
 // quantum (absolute index).  Each block counts one chunk that lies inside a
 // single quantum, so its LDS bins go to one output row.
@@ -293,6 +347,41 @@ int pa_hist_u8(const uint8_t* bytes, size_t n, uint64_t* counts, pa_stream_t str
   hipLaunchKernelGGL(k_widen, dim3(1), dim3(256), 0, s, c32, c64, (size_t)256);
   PA_HIP(hipGetLastError());
   return deliver_u64(c64, counts, 256, dout, s);
+}
+
+int pa_counting_sort_u8(uint8_t* bytes, size_t n, pa_stream_t stream) {
+  if (n == 0) return PA_OK;
+  if (!bytes) return arg_error("pa_counting_sort_u8: null pointer");
+  if (n > 0xFFFFFFFFull) return arg_error("pa_counting_sort_u8: n must be < 2^32");
+  hipStream_t s = (hipStream_t)stream;
+  StreamCtx& ctx = stream_ctx(s);
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  const bool dk = is_device_ptr(bytes);
+  Layout L;
+  const size_t o_c32 = L.take(256 * 4), o_in = dk ? 0 : L.take(n + 16);
+  void* base;
+  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  char* b = (char*)base;
+  uint32_t* c32 = (uint32_t*)(b + o_c32);
+  // A host array is staged with the same 16-byte phase, so the device copy is
+  // laid out like the caller's.
+  uint8_t* d = dk ? bytes : (uint8_t*)(b + o_in + ((uintptr_t)bytes & 15));
+  if (!dk) PA_HIP(hipMemcpyAsync(d, bytes, n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, c32, (size_t)256);
+  {
+    ProfScope ps("k_hist_u8", s);
+    hipLaunchKernelGGL(k_hist_u8, dim3(hist_grid(n / 16 + 1)), dim3(kHistBlock), 0, s, (const uint8_t*)d, n, c32);
+  }
+  {
+    ProfScope ps("k_counting_fill", s);
+    hipLaunchKernelGGL(k_counting_fill, dim3(hist_grid(n / 16 + 1)), dim3(kHistBlock), 0, s, d, n, c32);
+  }
+  PA_HIP(hipGetLastError());
+  if (!dk) {
+    PA_HIP(hipMemcpyAsync(bytes, d, n, hipMemcpyDeviceToHost, s));
+    PA_HIP(hipStreamSynchronize(s));
+  }
+  return PA_OK;
 }
 
 // Shared by pa_hist_quanta_u32 and pa_start_of_bins_u32: u32 counts of byte

@@ -97,7 +97,15 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Fused 4-digit histogram of u32 keys into zeroed u32 counts[4][256]
 // (defined with the sort kernels).
+// Radix select of device keys into caller-provided scratch (pa_select.hip).
+size_t select_scratch_bytes_u32(size_t n);
+int select_enqueue_u32(const uint32_t* din, size_t n, size_t k, char* scratch, uint32_t* value_out, hipStream_t s);
 int launch_hist_digits_u32(const uint32_t* keys, size_t n, uint32_t* counts, hipStream_t s);
+// Merge of two sorted device runs into a device output (pa_merge.hip), with
+// merge_split_bytes(na + nb) bytes of caller scratch for the split table.
+size_t merge_split_bytes(size_t n);
+template <typename K>
+int merge_enqueue(const K* a, size_t na, const K* b, size_t nb, K* out, size_t* split, hipStream_t s);
 
 }  // namespace pa
 
@@ -116,9 +124,25 @@ __device__ __forceinline__ uint32_t count_below(uint64_t mask) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
-// Order-preserving float transform and inverse on IEEE-754 bit patterns.
-__device__ __forceinline__ uint32_t f32_fwd(uint32_t u) { return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u); }
-__device__ __forceinline__ uint32_t f32_inv(uint32_t v) { return v ^ (((v >> 31) - 1u) | 0x80000000u); }
+// Order-preserving key transforms to unsigned order and their inverses, on the
+// bit patterns of K (32 or 64 bits):
+//   kXformFloat  IEEE-754: a negative pattern flips every bit, a positive one
+//                its sign bit, so -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN
+//   kXformSigned two's complement: flip the sign bit
+enum : uint32_t { kXformNone = 0u, kXformFloat = 1u, kXformSigned = 2u };
+template <typename K>
+__device__ __forceinline__ K key_fwd(K u, uint32_t xform) {
+  constexpr K top = K(1) << (sizeof(K) * 8 - 1);
+  if (xform == kXformSigned) return u ^ top;
+  const K neg = (K)((std::make_signed_t<K>)u >> (sizeof(K) * 8 - 1));  // all ones if the sign is set
+  return u ^ (neg | top);
+}
+template <typename K>
+__device__ __forceinline__ K key_inv(K v, uint32_t xform) {
+  constexpr K top = K(1) << (sizeof(K) * 8 - 1);
+  if (xform == kXformSigned) return v ^ top;
+  return v ^ (((v >> (sizeof(K) * 8 - 1)) - K(1)) | top);  // top set: flip sign; clear: flip all
+}
 
 template <typename K>
 __device__ __forceinline__ uint32_t digit8(K k, uint32_t shift) {

@@ -135,6 +135,32 @@ int merge_entry(const K* a, size_t na, const K* b, size_t nb, K* out, hipStream_
   return PA_OK;
 }
 
+// Device runs into a device output, with the split table in caller scratch of
+// merge_split_bytes(na + nb) bytes; stream-ordered, no allocation.
+size_t merge_split_bytes(size_t n) { return ((n + kMergeTile - 1) / kMergeTile + 1) * sizeof(size_t); }
+
+template <typename K>
+int merge_enqueue(const K* a, size_t na, const K* b, size_t nb, K* out, size_t* split, hipStream_t s) {
+  const size_t n = na + nb;
+  if (n == 0) return PA_OK;
+  const size_t ntiles = (n + kMergeTile - 1) / kMergeTile;
+  {
+    ProfScope ps("k_merge_partition", s);
+    hipLaunchKernelGGL(k_merge_partition<K>, dim3((unsigned)((ntiles + 1 + 255) / 256)), dim3(256), 0, s, a, na, b,
+                       nb, ntiles, split);
+  }
+  {
+    ProfScope ps("k_merge_tile", s);
+    hipLaunchKernelGGL(k_merge_tile<K>, dim3((unsigned)ntiles), dim3(kMergeBlock), 0, s, a, na, b, nb, split, out);
+  }
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+template int merge_enqueue<uint32_t>(const uint32_t*, size_t, const uint32_t*, size_t, uint32_t*, size_t*,
+                                     hipStream_t);
+template int merge_enqueue<uint64_t>(const uint64_t*, size_t, const uint64_t*, size_t, uint64_t*, size_t*,
+                                     hipStream_t);
+
 // In-place form: merge t[0..n1) and t[n1..n1+n2) into scratch, copy back.
 template <typename K>
 int merge_inplace_entry(K* t, size_t n1, size_t n2, hipStream_t s, const char* who) {

@@ -220,6 +220,15 @@ int pa_synchronize(pa_stream_t stream) {
   return PA_OK;
 }
 
+int pa_copy(void* dst, const void* src, size_t bytes, pa_stream_t stream) {
+  if (bytes == 0 || dst == src) return PA_OK;
+  if (!dst || !src) return arg_error("pa_copy: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  PA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
+  if (!is_device_ptr(dst) || !is_device_ptr(src)) PA_HIP(hipStreamSynchronize(s));
+  return PA_OK;
+}
+
 int pa_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, pa_stream_t stream) {
   if (n == 0) return PA_OK;
   if (!dst || !src) return arg_error("pa_copy_u32: null pointer");

@@ -301,6 +301,49 @@ __global__ void k_select_place(const SelState* st, K* dst) { *dst = (K)st->resul
 template <typename K>
 __global__ void k_select_result(const SelState* st, K* dst) { *dst = (K)st->result; }
 
+// Select on device keys into caller-provided scratch (no allocation, no
+// locking): used by the partition.  The selected key is written to
+// *value_out (device memory).
+template <typename K>
+static size_t select_bytes(size_t n) {
+  constexpr int D = sizeof(K);
+  Layout L;
+  L.take(sizeof(SelState));
+  L.take((D * 256 + 256 + 2) * 4);
+  L.take(std::max<size_t>(n / 2, 1) * sizeof(K));
+  return L.off;
+}
+
+template <typename K>
+static int select_enqueue(const K* din, size_t n, size_t k, char* b, K* value_out, hipStream_t s) {
+  constexpr int D = sizeof(K);
+  const size_t cap = std::max<size_t>(n / 2, 1);
+  const size_t nz = D * 256 + 256 + 2;
+  Layout L;
+  SelState* st = (SelState*)(b + L.take(sizeof(SelState)));
+  uint32_t* z = (uint32_t*)(b + L.take(nz * 4));
+  K* buf = (K*)(b + L.take(cap * sizeof(K)));
+  uint32_t* ghistD = z;
+  uint32_t* ghist = z + D * 256;
+  uint32_t* done_ctr = ghist + 256;
+  uint32_t* buf_ctr = done_ctr + 1;
+  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
+                     (int32_t)(8 * (D - 1)), z, (uint32_t)nz);
+  const size_t nvec = n / (16 / sizeof(K));
+  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + kSelBlock) / kSelBlock, 1024));
+  hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, st, ghistD, done_ctr);
+  for (int pass = 0; pass < D - 1; ++pass)
+    hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, st, ghist, done_ctr, buf_ctr);
+  hipLaunchKernelGGL(k_select_result<K>, dim3(1), dim3(1), 0, s, st, value_out);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+size_t select_scratch_bytes_u32(size_t n) { return select_bytes<uint32_t>(n); }
+int select_enqueue_u32(const uint32_t* din, size_t n, size_t k, char* scratch, uint32_t* value_out, hipStream_t s) {
+  return select_enqueue<uint32_t>(din, n, k, scratch, value_out, s);
+}
+
 template <typename K>
 static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, hipStream_t s, const char* who) {
   constexpr int D = sizeof(K);

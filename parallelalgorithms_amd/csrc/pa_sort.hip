@@ -54,6 +54,61 @@ __global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* __restrict_
   base[blockIdx.x * 256 + d] = pre + inc - c;
 }
 
+// Pass-0 prefixes.  rows[t][d] holds digit d's count in pass-0 tile t; the
+// two kernels below turn every row into the inclusive status word of its tile
+// (count of digit d in tiles 0..t), as if every tile had already resolved its
+// look-back.  Group g of blocks owns tiles [g * per, (g + 1) * per).  Blocks
+// have 1024 threads: thread (q, d) = (tid >> 8, tid & 255) handles digit d for
+// a quarter of the group's tiles, so each thread keeps several independent
+// loads in flight instead of walking the rows one after another.
+constexpr unsigned kPrefixThreads = 1024;
+
+// Step 1: partial[g][d] = count of digit d over group g's tiles.
+__global__ __launch_bounds__(kPrefixThreads) void k_tile_sums(const uint32_t* __restrict__ rows, uint32_t ntiles,
+                                                              uint32_t per, uint32_t* __restrict__ partial) {
+  __shared__ uint32_t s_q[4][256];
+  const uint32_t d = threadIdx.x & 255, q = threadIdx.x >> 8;
+  const uint32_t t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
+  uint32_t acc = 0;
+#pragma unroll 8
+  for (uint32_t t = t0 + q; t < t1; t += 4) acc += rows[(size_t)t * 256 + d];
+  s_q[q][d] = acc;
+  __syncthreads();
+  if (q == 0) partial[blockIdx.x * 256 + d] = s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d];
+}
+
+// Step 2: group g adds the partials of groups < g, then scans its own tiles
+// in chunks of 4 x 8: thread (q, d) loads 8 consecutive rows at once.
+__global__ __launch_bounds__(kPrefixThreads) void k_tile_scan(uint32_t* __restrict__ rows, uint32_t ntiles,
+                                                              uint32_t per, const uint32_t* __restrict__ partial) {
+  __shared__ uint32_t s_q[4][256];
+  const uint32_t d = threadIdx.x & 255, q = threadIdx.x >> 8;
+  uint32_t acc = 0;
+#pragma unroll 8
+  for (uint32_t g = q; g < blockIdx.x; g += 4) acc += partial[g * 256 + d];
+  s_q[q][d] = acc;
+  __syncthreads();
+  uint32_t base = s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d];
+  const uint32_t t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
+  for (uint32_t c0 = t0; c0 < t1; c0 += 32) {
+    const uint32_t tq = c0 + 8 * q;
+    uint32_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = tq + j < t1 ? rows[(size_t)(tq + j) * 256 + d] : 0u;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) v[j] += v[j - 1];
+    __syncthreads();  // every thread has read s_q of the previous step
+    s_q[q][d] = v[7];
+    __syncthreads();
+    uint32_t off = base;
+    for (uint32_t w = 0; w < q; ++w) off += s_q[w][d];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (tq + j < t1) rows[(size_t)(tq + j) * 256 + d] = kLbInc | (off + v[j]);
+    base += s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Chained passes: per-chain digit counts, bases and the interleaved claim order
 // ---------------------------------------------------------------------------
@@ -67,7 +122,7 @@ template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_hist_chains(const K* __restrict__ keys, size_t n, uint32_t log2s,
                                                             uint32_t* __restrict__ partial,
                                                             uint4* __restrict__ zero_ptr, size_t zero_vec,
-                                                            uint32_t float_keys) {
+                                                            uint32_t xform) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
   constexpr int ROW = kChains * 256;
@@ -117,7 +172,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_chains(const K* __restrict_
 #pragma unroll
       for (int j = 0; j < PER; ++j) {
         pos[j] = p0 + j;
-        if (sizeof(K) == 4 && float_keys) k[j] = (K)dev::f32_fwd((uint32_t)k[j]);
+        if (xform) k[j] = dev::key_fwd<K>(k[j], xform);
       }
       count(k, pos);
     }
@@ -128,7 +183,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_chains(const K* __restrict_
     for (size_t e = tid; e < extra; e += kSortBlock) {
       const size_t idx = e < head ? e : tail_start + (e - head);
       K kk = keys[idx];
-      if (sizeof(K) == 4 && float_keys) kk = (K)dev::f32_fwd((uint32_t)kk);
+      if (xform) kk = dev::key_fwd<K>(kk, xform);
       atomicAdd(&s_hist[(uint32_t)(idx >> log2s) * 256 + dev::digit8<K>(kk, 0)], 1u);
 #pragma unroll
       for (int d = 1; d < D; ++d)
@@ -382,6 +437,7 @@ inline uint32_t chain_log2s(size_t n) {
 }
 
 constexpr unsigned kHistChainBlocks = 512;
+constexpr unsigned kPrefixGroups = 128;  // blocks of the pass-0 prefix scan
 
 // Scratch of one sort call with pass variant var (see enqueue_sort for the
 // layout).
@@ -394,6 +450,7 @@ size_t sort_scratch_bytes(const PassVariant& var, size_t n) {
   L.take((size_t)D * kChains * 256 * 4);  // bases
   L.take(D * 4);                          // tile counters
   L.take((size_t)D * rows * 256 * 4);     // look-back table
+  L.take((size_t)kPrefixGroups * 256 * 4);  // pass-0 prefix partial sums
   if (var.chained) {
     L.take((size_t)D * 36 * 4);                                  // chain tables
     L.take((size_t)D * rows * 4);                                // claim orders
@@ -404,7 +461,7 @@ size_t sort_scratch_bytes(const PassVariant& var, size_t n) {
 
 template <typename K, bool KV>
 int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, char* scratch,
-                 hipStream_t s, bool float_keys) {
+                 hipStream_t s, uint32_t xform) {
   constexpr int D = sizeof(K);
   const size_t rows = sort_rows(var, n);
   const size_t ncount = (size_t)D * kChains * 256;
@@ -414,6 +471,7 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
   uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch + L.take(D * 4));
   const size_t lb_bytes = (size_t)D * rows * 256 * 4;
   uint32_t* lb = reinterpret_cast<uint32_t*>(scratch + L.take(lb_bytes));
+  uint32_t* prefix_partial = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)kPrefixGroups * 256 * 4));
   uint32_t* chain_info = nullptr;
   uint32_t* chain_tiles = nullptr;
   uint32_t* partial = nullptr;
@@ -433,22 +491,31 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
     {
       ProfScope ps("k_hist_digits", s);
       hipLaunchKernelGGL(k_hist_chains<K>, dim3(kHistChainBlocks), dim3(kSortBlock), 0, s, keys, n, log2s, partial,
-                         reinterpret_cast<uint4*>(lb), lb_bytes / 16, (uint32_t)float_keys);
+                         reinterpret_cast<uint4*>(lb), lb_bytes / 16, xform);
     }
     hipLaunchKernelGGL(k_reduce_counts, dim3((unsigned)((ncount + 255) / 256), 16), dim3(256), 0, s, partial,
                        kHistChainBlocks, (uint32_t)ncount, counts);
     hipLaunchKernelGGL(k_chain_plan, dim3(D), dim3(1024), 0, s, counts, (uint32_t)n, log2s, (uint32_t)tile, bases,
                        chain_info, chain_tiles, (uint32_t)rows);
   } else {
-    const size_t nvec = n / (16 / sizeof(K)) + 1;
-    const size_t grid = std::max<size_t>(std::min<size_t>((nvec + kSortBlock - 1) / kSortBlock, 2048),
-                                         std::min<size_t>(lb_bytes / 16 / kSortBlock + 1, 2048));
+    // One read: all digit histograms, pass 0's per-tile digit counts (into
+    // pass 0's look-back rows) and the clearing of the other passes' rows.
+    // Two small kernels then turn pass 0's rows into inclusive prefixes, so
+    // pass 0 runs without look-back waits.
+    const size_t pass_rows_bytes = rows * 256 * 4;
+    const size_t zero_bytes = lb_bytes - pass_rows_bytes;
+    const size_t grid = std::max<size_t>(1, std::min<size_t>(rows, 2048));
     {
       ProfScope ps("k_hist_digits", s);
-      hipLaunchKernelGGL(k_hist_digits<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, counts,
-                         reinterpret_cast<uint4*>(lb), lb_bytes / 16, (uint32_t)float_keys);
+      hipLaunchKernelGGL(k_hist_tiles<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, (uint32_t)tile, lb,
+                         counts, reinterpret_cast<uint4*>(lb + rows * 256), zero_bytes / 16, xform);
     }
     hipLaunchKernelGGL(k_digit_bases, dim3(D), dim3(256), 0, s, counts, bases);
+    const uint32_t groups = (uint32_t)std::min<size_t>(rows, kPrefixGroups);
+    const uint32_t per = (uint32_t)((rows + groups - 1) / groups);
+    ProfScope ps("k_tile_prefix", s);
+    hipLaunchKernelGGL(k_tile_sums, dim3(groups), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per, prefix_partial);
+    hipLaunchKernelGGL(k_tile_scan, dim3(groups), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per, prefix_partial);
   }
   K* kin = keys;
   K* kout = ktmp;
@@ -456,8 +523,10 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
   uint32_t* vout = vtmp;
   for (int p = 0; p < D; ++p) {
     uint32_t fmode = 0;
-    if (float_keys && p == 0) fmode |= kFloatIn;
-    if (float_keys && p == D - 1) fmode |= kFloatOut;
+    if (!var.chained && p == 0) fmode |= kPrefixReady;
+    if (xform && p == 0) fmode |= kFloatIn;
+    if (xform && p == D - 1) fmode |= kFloatOut;
+    if (xform == dev::kXformSigned) fmode |= kSignedKeys;
     PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * (var.chained ? kChains * 256 : 256),
                lb + (size_t)p * rows * 256, ctr + p, (uint32_t)rows, (uint32_t)(8 * p), fmode,
                var.chained ? chain_info + (size_t)p * 36 : nullptr,
@@ -483,7 +552,7 @@ int launch_hist_digits_u32(const uint32_t* keys, size_t n, uint32_t* counts, hip
 
 // Host entry: classifies pointers, stages host arrays, runs, copies back.
 template <typename K, bool KV>
-int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipStream_t s, bool float_keys,
+int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipStream_t s, uint32_t xform,
                const char* who) {
   if (n == 0) return PA_OK;
   if (!keys || (KV && !vals)) return arg_error(who);
@@ -513,7 +582,7 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   uint32_t* dvtmp = KV ? (dvt ? vtmp : reinterpret_cast<uint32_t*>(b + o_vt)) : nullptr;
   if (!dk) PA_HIP(hipMemcpyAsync(dkeys, keys, n * sizeof(K), hipMemcpyHostToDevice, s));
   if (KV && !dv) PA_HIP(hipMemcpyAsync(dvals, vals, n * 4, hipMemcpyHostToDevice, s));
-  PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, float_keys)));
+  PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
   if (!dk) PA_HIP(hipMemcpyAsync(keys, dkeys, n * sizeof(K), hipMemcpyDeviceToHost, s));
   if (KV && !dv) PA_HIP(hipMemcpyAsync(vals, dvals, n * 4, hipMemcpyDeviceToHost, s));
   if (!dk || (KV && !dv)) PA_HIP(hipStreamSynchronize(s));
@@ -527,24 +596,39 @@ using namespace pa;
 extern "C" {
 
 int pa_sort_u32(uint32_t* keys, uint32_t* tmp, size_t n, pa_stream_t stream) {
-  return sort_entry<uint32_t, false>(keys, nullptr, tmp, nullptr, n, (hipStream_t)stream, false, "pa_sort_u32");
+  return sort_entry<uint32_t, false>(keys, nullptr, tmp, nullptr, n, (hipStream_t)stream, dev::kXformNone, "pa_sort_u32");
 }
 
 int pa_sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, pa_stream_t stream) {
-  return sort_entry<uint64_t, false>(keys, nullptr, tmp, nullptr, n, (hipStream_t)stream, false, "pa_sort_u64");
+  return sort_entry<uint64_t, false>(keys, nullptr, tmp, nullptr, n, (hipStream_t)stream, dev::kXformNone, "pa_sort_u64");
 }
 
 int pa_sort_f32(float* keys, float* tmp, size_t n, pa_stream_t stream) {
   return sort_entry<uint32_t, false>(reinterpret_cast<uint32_t*>(keys), nullptr, reinterpret_cast<uint32_t*>(tmp),
-                                     nullptr, n, (hipStream_t)stream, true, "pa_sort_f32");
+                                     nullptr, n, (hipStream_t)stream, dev::kXformFloat, "pa_sort_f32");
+}
+
+int pa_sort_f64(double* keys, double* tmp, size_t n, pa_stream_t stream) {
+  return sort_entry<uint64_t, false>(reinterpret_cast<uint64_t*>(keys), nullptr, reinterpret_cast<uint64_t*>(tmp),
+                                     nullptr, n, (hipStream_t)stream, dev::kXformFloat, "pa_sort_f64");
+}
+
+int pa_sort_i32(int32_t* keys, int32_t* tmp, size_t n, pa_stream_t stream) {
+  return sort_entry<uint32_t, false>(reinterpret_cast<uint32_t*>(keys), nullptr, reinterpret_cast<uint32_t*>(tmp),
+                                     nullptr, n, (hipStream_t)stream, dev::kXformSigned, "pa_sort_i32");
+}
+
+int pa_sort_i64(int64_t* keys, int64_t* tmp, size_t n, pa_stream_t stream) {
+  return sort_entry<uint64_t, false>(reinterpret_cast<uint64_t*>(keys), nullptr, reinterpret_cast<uint64_t*>(tmp),
+                                     nullptr, n, (hipStream_t)stream, dev::kXformSigned, "pa_sort_i64");
 }
 
 int pa_sort_kv_u32_u32(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, size_t n, pa_stream_t stream) {
-  return sort_entry<uint32_t, true>(keys, vals, ktmp, vtmp, n, (hipStream_t)stream, false, "pa_sort_kv_u32_u32");
+  return sort_entry<uint32_t, true>(keys, vals, ktmp, vtmp, n, (hipStream_t)stream, dev::kXformNone, "pa_sort_kv_u32_u32");
 }
 
 int pa_sort_kv_u64_u32(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, size_t n, pa_stream_t stream) {
-  return sort_entry<uint64_t, true>(keys, vals, ktmp, vtmp, n, (hipStream_t)stream, false, "pa_sort_kv_u64_u32");
+  return sort_entry<uint64_t, true>(keys, vals, ktmp, vtmp, n, (hipStream_t)stream, dev::kXformNone, "pa_sort_kv_u64_u32");
 }
 
 int pa_autotune_sort_u32(size_t max_n, pa_stream_t stream) {
@@ -595,7 +679,7 @@ int pa_autotune_sort_u32(size_t max_n, pa_stream_t stream) {
             break;
           }
           (void)hipEventRecord(e0, s);
-          if ((st = enqueue_sort<uint32_t, false>(var, keys, nullptr, tmp, nullptr, n, scratch, s, false)) != PA_OK) break;
+          if ((st = enqueue_sort<uint32_t, false>(var, keys, nullptr, tmp, nullptr, n, scratch, s, dev::kXformNone)) != PA_OK) break;
           (void)hipEventRecord(e1, s);
           if ((e = hipEventSynchronize(e1)) != hipSuccess) {
             fail(e, "hipEventSynchronize");

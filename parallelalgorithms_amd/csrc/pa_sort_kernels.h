@@ -14,7 +14,11 @@ constexpr uint32_t kLbInc = 2u << 30;
 constexpr uint32_t kLbMask = (1u << 30) - 1;
 constexpr size_t kMaxSortN = (size_t(1) << 30) - 1;
 
-enum : uint32_t { kFloatIn = 1u, kFloatOut = 2u };
+// Pass flags: key transform on load / on store (kSignedKeys selects the
+// two's-complement transform, else the IEEE-754 one); kPrefixReady: every
+// tile's inclusive prefix was published before the pass (pass 0 of the plain
+// sort, from the histogram read), so tiles neither publish nor wait.
+enum : uint32_t { kFloatIn = 1u, kFloatOut = 2u, kPrefixReady = 4u, kSignedKeys = 8u };
 
 // Tile-to-array modes of the onesweep pass.
 enum : int {
@@ -32,7 +36,7 @@ constexpr uint32_t kChainShift = 5;  // chain of a key in pass p >= 1: digit p-1
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict__ keys, size_t n,
                                                             uint32_t* __restrict__ counts, uint4* __restrict__ zero_ptr,
-                                                            size_t zero_vec, uint32_t float_keys) {
+                                                            size_t zero_vec, uint32_t xform) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
   __shared__ uint32_t s_hist[D * 256];
@@ -67,9 +71,9 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
       if (!ok[u]) continue;
       K k[PER];
       __builtin_memcpy(k, &x[u], 16);
-      if (sizeof(K) == 4 && float_keys) {
+      if (xform) {
 #pragma unroll
-        for (int j = 0; j < PER; ++j) k[j] = (K)dev::f32_fwd((uint32_t)k[j]);
+        for (int j = 0; j < PER; ++j) k[j] = dev::key_fwd<K>(k[j], xform);
       }
       // Whole-key fast path: every active lane holds PER copies of one key
       // (constant input) -> one lane adds the wave's count to each digit.
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
     for (size_t e = tid; e < extra; e += kSortBlock) {
       size_t idx = e < head ? e : tail_start + (e - head);
       K kk = keys[idx];
-      if (sizeof(K) == 4 && float_keys) kk = (K)dev::f32_fwd((uint32_t)kk);
+      if (xform) kk = dev::key_fwd<K>(kk, xform);
 #pragma unroll
       for (int d = 0; d < D; ++d) atomicAdd(&s_hist[d * 256 + dev::digit8<K>(kk, 8 * d)], 1u);
     }
@@ -215,8 +219,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   // 16-lane units: slot (u, d) is word u * 256 + d, count in the high half.
   uint32_t* const slot16 = reinterpret_cast<uint32_t*>(s_keys);
   uint32_t* const uslot16 = slot16 + unit * 256;
-  const bool float_in = sizeof(K) == 4 && (fmode & kFloatIn);
-  const bool float_out = sizeof(K) == 4 && (fmode & kFloatOut);
+  const bool float_in = (fmode & kFloatIn) != 0;
+  const bool float_out = (fmode & kFloatOut) != 0;
+  const uint32_t xform = (fmode & kSignedKeys) ? dev::kXformSigned : dev::kXformFloat;
 
   // Clear the rank slots while the claim is in flight.
   {
@@ -285,7 +290,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       for (int i = 0; i < ITEMS; ++i) key[i] = src[i * UL];
       if (float_in) {
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) key[i] = (K)dev::f32_fwd((uint32_t)key[i]);
+        for (int i = 0; i < ITEMS; ++i) key[i] = dev::key_fwd<K>(key[i], xform);
       }
     } else {
 #pragma unroll
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
         K v = ~K(0);
         if (first + i * UL < len) {
           v = src[i * UL];
-          if (float_in) v = (K)dev::f32_fwd((uint32_t)v);
+          if (float_in) v = dev::key_fwd<K>(v, xform);
         }
         key[i] = v;
       }
@@ -388,7 +393,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
         cnt += c;
       }
     }
-    dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
+    if (!(fmode & kPrefixReady))
+      dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
     if constexpr (SEG) gval = g[tid];
     cinc = dev::wave_incl_scan(cnt, lane);
     if constexpr (SEG) ginc = dev::wave_incl_scan(gval, lane);
@@ -435,7 +441,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     if (local > 0) {
       if (!polled) poll();
       while (!consume()) poll();
-      dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], kLbInc | (excl + cnt));
+      if (!(fmode & kPrefixReady)) dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], kLbInc | (excl + cnt));
     }
     s_goff[tid] = gstart + excl - tstart;
   };
@@ -503,7 +509,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       for (int i = 0; i < G; ++i) {
         gp[i] = s_goff[dev::digit8<K>(wk[i], shift)] + r * STAGE + tid + (g0 + i) * BLOCK;
         if constexpr (KV) gpos[g0 + i] = gp[i];
-        if constexpr (decltype(fo)::value) wk[i] = (K)dev::f32_inv((uint32_t)wk[i]);
+        if constexpr (decltype(fo)::value) wk[i] = dev::key_inv<K>(wk[i], xform);
       }
       if (valid == (uint32_t)TILE) {
 #pragma unroll
@@ -556,6 +562,79 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i)
       if (tid + i * BLOCK < valid) vdst[gpos[i]] = s_vals[tid + i * BLOCK];
+  }
+}
+
+
+// Histogram read that also prepares pass 0's look-back rows: blocks take
+// whole tiles of `tile` keys (pass 0's tiles), write each tile's digit-0
+// counts to rows[t][256] and accumulate all D digit histograms into counts
+// [D][256].  Also clears the look-back rows of passes 1..D-1 (zero_vec
+// 16-byte words at zero_ptr).
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__ keys, size_t n, uint32_t tile,
+                                                           uint32_t* __restrict__ rows, uint32_t* __restrict__ counts,
+                                                           uint4* __restrict__ zero_ptr, size_t zero_vec,
+                                                           uint32_t xform) {
+  constexpr int D = sizeof(K);
+  constexpr int PER = 16 / sizeof(K);
+  __shared__ uint32_t s_hist[D * 256];  // row 0: digit 0 over the block's finished tiles
+  __shared__ uint32_t s_tile[256];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < D * 256; i += kSortBlock) s_hist[i] = 0;
+  if (tid < 256) s_tile[tid] = 0;
+  const size_t gstride = (size_t)gridDim.x * kSortBlock;
+  for (size_t i = (size_t)blockIdx.x * kSortBlock + tid; i < zero_vec; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  const bool aligned = ((uintptr_t)keys & 15) == 0 && (tile % PER) == 0;
+  const size_t ntiles = (n + tile - 1) / tile;
+  for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const size_t lo = t * tile, hi = min(lo + tile, n);
+    size_t scalar_from = lo;
+    if (aligned) {
+      const size_t nvec = (hi - lo) / PER;
+      const uint4* vk = reinterpret_cast<const uint4*>(keys + lo);
+#pragma unroll 4
+      for (size_t v = tid; v < nvec; v += kSortBlock) {
+        const uint4 x = vk[v];
+        K k[PER];
+        __builtin_memcpy(k, &x, 16);
+        if (xform) {
+#pragma unroll
+          for (int j = 0; j < PER; ++j) k[j] = dev::key_fwd<K>(k[j], xform);
+        }
+        uint32_t dg[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(k[j], 0);
+        dev::lds_count<PER>(s_tile, dg);
+#pragma unroll
+        for (int d = 1; d < D; ++d) {
+#pragma unroll
+          for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(k[j], 8 * d);
+          dev::lds_count<PER>(s_hist + d * 256, dg);
+        }
+      }
+      scalar_from = lo + nvec * PER;
+    }
+    for (size_t i = scalar_from + tid; i < hi; i += kSortBlock) {
+      K kk = keys[i];
+      if (xform) kk = dev::key_fwd<K>(kk, xform);
+      atomicAdd(&s_tile[dev::digit8<K>(kk, 0)], 1u);
+#pragma unroll
+      for (int d = 1; d < D; ++d) atomicAdd(&s_hist[d * 256 + dev::digit8<K>(kk, 8 * d)], 1u);
+    }
+    __syncthreads();
+    if (tid < 256) {
+      const uint32_t c = s_tile[tid];
+      rows[t * 256 + tid] = c;
+      s_hist[tid] += c;
+      s_tile[tid] = 0;
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = tid; i < D * 256; i += kSortBlock) {
+    const uint32_t c = s_hist[i];
+    if (c) atomicAdd(&counts[i], c);
   }
 }
 

@@ -6,7 +6,8 @@ views), so no value conversion happens on the way.
 import numpy as np
 
 _VIEW = {np.dtype(np.uint32): np.int32, np.dtype(np.uint64): np.int64, np.dtype(np.float32): np.float32,
-         np.dtype(np.uint8): np.uint8, np.dtype(np.int32): np.int32}
+         np.dtype(np.uint8): np.uint8, np.dtype(np.int32): np.int32, np.dtype(np.int64): np.int64,
+         np.dtype(np.float64): np.float64}
 
 
 def to_dev(a):
@@ -24,7 +25,8 @@ def to_host(t, dtype):
 def dev_empty(n, dtype):
     import torch
     tmap = {np.dtype(np.uint32): torch.int32, np.dtype(np.uint64): torch.int64, np.dtype(np.float32): torch.float32,
-            np.dtype(np.uint8): torch.uint8}
+            np.dtype(np.uint8): torch.uint8, np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64,
+            np.dtype(np.float64): torch.float64}
     return torch.empty(n, dtype=tmap[np.dtype(dtype)], device="cuda")
 
 

@@ -119,6 +119,31 @@ def test_f32_transform_orders_like_ieee(orc):
     assert nan_neg < f[0] and nan_pos > f[-1]
 
 
+@pytest.mark.parametrize("n", [0, 1, 99, 100, 5000, 100_003])
+def test_sort_signed_matches_numpy(orc, n):
+    a32 = P.gen_u32(41, n).view(np.int32)
+    a64 = P.gen_u64(42, n).view(np.int64)
+    np.testing.assert_array_equal(orc.sort_i32(a32), np.sort(a32))
+    np.testing.assert_array_equal(orc.sort_i64(a64), np.sort(a64))
+
+
+def test_sort_f64_total_order(orc):
+    # Finite values in numpy order; -NaN first, +NaN last, -0 before +0, NaN
+    # payloads kept: the transformed bits' stable order.
+    r = P.gen_u64(43, 50_000)
+    bits = ((r >> np.uint64(11)).astype(np.float64) / float(1 << 53) * 200.0 - 100.0).view(np.uint64).copy()
+    bits[::97] = np.array([0x7FF8000000000001, 0xFFF8000000000002, 0, 1 << 63, 0x7FF0000000000000,
+                           0xFFF0000000000000], dtype=np.uint64)[np.arange(bits[::97].size) % 6]
+    got = orc.sort_f64_bits(bits)
+    f = np.where(bits >> np.uint64(63), ~bits, bits | np.uint64(1 << 63)).astype(np.uint64)
+    np.testing.assert_array_equal(got, bits[np.argsort(f, kind="stable")])
+    v = got.view(np.float64)
+    fin = np.isfinite(v)
+    np.testing.assert_array_equal(v[fin], np.sort(bits.view(np.float64)[np.isfinite(bits.view(np.float64))]))
+    assert got[0] == 0xFFF8000000000002 and got[-1] == 0x7FF8000000000001
+    assert list(got[np.flatnonzero((got == 0) | (got == np.uint64(1 << 63)))][[0, -1]]) == [1 << 63, 0]
+
+
 def test_sort_f32_matches_numpy_on_finite(orc):
     x = (P.gen_u64(1236, 100_000) >> np.uint64(40)).astype(np.float32) - np.float32(1 << 23)
     bits = x.view(np.uint32)
@@ -247,6 +272,72 @@ def test_select_radix_benchmark_k(orc):
     a = gen("uniform", 1_000_000)
     k = a.size - 21
     assert orc.select_radix_u32(a, k)[0] == np.partition(a, k)[k]
+
+
+# ---------------------------------------------------------------- counting sort
+@pytest.mark.parametrize("n", [0, 1, 15, 16, 17, 100_003])
+@pytest.mark.parametrize("kind", ["uniform", "constant", "two"])
+def test_counting_sort_u8_is_np_sort(orc, n, kind):
+    a = (P.gen_u32(51, n) & np.uint32(0xFF)).astype(np.uint8)
+    if kind == "constant":
+        a[:] = 0x5A
+    elif kind == "two":  # most bins empty
+        a = np.where(a & 1, 3, 250).astype(np.uint8)
+    np.testing.assert_array_equal(orc.counting_sort_u8(a), np.sort(a))
+
+
+def test_counting_sort_u8_subrange(orc):
+    a = (P.gen_u32(52, 10_000) & np.uint32(0xFF)).astype(np.uint8)
+    got = orc.counting_sort_u8(a, 100, 9_000, thr=1000)
+    expect = a.copy()
+    expect[100:9_000] = np.sort(a[100:9_000])
+    np.testing.assert_array_equal(got, expect)
+
+
+# ---------------------------------------------------------------- partition
+def _check_partitioned(a, out, ks, start, length):
+    seg, res = a[start:start + length], out[start:start + length]
+    s = np.sort(seg)
+    np.testing.assert_array_equal(np.sort(res), s)  # a permutation
+    np.testing.assert_array_equal(out[:start], a[:start])  # outside untouched
+    np.testing.assert_array_equal(out[start + length:], a[start + length:])
+    for k in ks:
+        r = k - start
+        assert res[r] == s[r]  # nth_element's value at k
+        assert res[:r].max(initial=0) <= res[r] <= res[r + 1:].min(initial=0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "presorted", "constant", "few", "reversed"])
+@pytest.mark.parametrize("kfrac", [0.0, 0.01, 0.5, 0.999, 1.0])
+def test_partition_radix_single_rank(orc, dist, kfrac):
+    n = 60_001
+    a = gen(dist, n, seed=21)
+    k = min(n - 1, int(kfrac * n))
+    _check_partitioned(a, orc.partition_radix_u32(a, [k]), [k], 0, n)
+
+
+def test_partition_radix_subrange_and_ranks(orc):
+    a = gen("uniform", 80_000, seed=22)
+    start, length = 1234, 50_000
+    ks = [start, start + 3, start + 25_000, start + length - 1]
+    _check_partitioned(a, orc.partition_radix_u32(a, ks, start, length), ks, start, length)
+
+
+def test_partition_radix_brute_force_tiny(orc):
+    # Every k of every array of length <= 6 over a 3-letter alphabet.
+    import itertools
+    for n in range(1, 7):
+        for t in itertools.product([0, 0x10000, 0xFFFFFFFF], repeat=n):
+            a = np.array(t, dtype=np.uint32)
+            for k in range(n):
+                _check_partitioned(a, orc.partition_radix_u32(a, [k]), [k], 0, n)
+
+
+def test_partition_radix_benchmark_k(orc):
+    # RadixPartitionBenchmark.cpp:50-66: a[k] against nth_element at k = n - 21.
+    a = gen("uniform", 1_000_000, seed=23)
+    k = a.size - 21
+    assert orc.partition_radix_u32(a, [k])[k] == np.partition(a, k)[k]
 
 
 # ---------------------------------------------------------------- merge
