@@ -143,75 +143,88 @@ def bench_segmented(pa, torch, count=10_000, seed=1239):
 
 
 def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
-    """Served stream: jobs 70% sort / 20% select (k log-uniform 1..n/10) / 10%
-    256-bin histogram of the top byte, sizes log-uniform 4K..64M, keys u32 or
-    u64 (50/50),
+    """Served stream through the job-trace format (parallelalgorithms_amd.trace):
+    jobs 70% sort / 20% select (k log-uniform 1..n/10) / 10% 256-bin histogram
+    of the top byte, sizes log-uniform 4K..64M, keys u32 or u64 (50/50),
     Poisson arrivals at fractions of the measured saturation rate, 64 GB
-    admission budget.  Latency = completion - submission."""
-    import time
-    import numpy as np
-    r = pa.gen_u64(seed, 4 * jobs)
-    sizes = log_uniform(r[:jobs], 4096, 64 << 20)
-    kind_u = r[jobs:2 * jobs] % np.uint64(100)
-    wide = (r[2 * jobs:3 * jobs] & np.uint64(1)).astype(bool)
-    spec = []
-    for i in range(jobs):
-        n = int(sizes[i])
-        kt = pa.KEY_U64 if wide[i] else pa.KEY_U32
-        op = pa.OP_SORT if kind_u[i] < 70 else (pa.OP_SELECT if kind_u[i] < 90 else pa.OP_HIST)
-        u = float(r[3 * jobs + i] >> np.uint64(11)) / float(1 << 53)
-        k = int(np.floor(np.exp(u * np.log(max(2.0, n / 10.0))))) - 1
-        spec.append((op, kt, n, min(max(k, 0), n - 1)))
-
-    def make_inputs(tag):
-        arrs, outs = [], []
-        for i, (op, kt, n, k) in enumerate(spec):
-            t = torch.empty(n, dtype=torch.int64 if kt == pa.KEY_U64 else torch.int32, device="cuda")
-            (pa.fill_random_u64 if kt == pa.KEY_U64 else pa.fill_random_u32)(t, seed * 1000 + i)
-            arrs.append(t)
-            outs.append(torch.empty(256, dtype=torch.int64, device="cuda") if op != pa.OP_SORT else None)
-        torch.cuda.synchronize()
-        return arrs, outs
-
-    def run(arrivals):
-        arrs, outs = make_inputs(0)
-        srv = pa.Server(budget_bytes=64 << 30, streams=4, coalesce_max_keys=1 << 20)
-        try:
-            tickets = []
-            t0 = time.perf_counter()
-            for i, (op, kt, n, k) in enumerate(spec):
-                if arrivals is not None:
-                    due = t0 + arrivals[i]
-                    if due - time.perf_counter() > 2e-3:
-                        time.sleep(due - time.perf_counter() - 1e-3)
-                    while time.perf_counter() < due:
-                        pass
-                tickets.append(srv.submit(op, arrs[i], kt, k=k, shift=56 if kt == pa.KEY_U64 else 24, bits=8,
-                                          out=outs[i]))
-            lat = [srv.wait(t) for t in tickets]
-            span = time.perf_counter() - t0
-            st = srv.stats()
-        finally:
-            srv.close()
-        del arrs, outs
-        torch.cuda.empty_cache()
-        return np.array(lat), span, st
-
-    keys_total = float(sum(n for _, _, n, _ in spec))
-    run(None)  # warm-up (allocations, code objects)
-    lat, span, st = run(None)
-    sat = jobs / span
+    admission budget.  Latency = completion - submission.  Every level replays
+    the same jobs; their output digests must agree across levels."""
+    from parallelalgorithms_amd import trace
+    base = trace.generate(jobs, seed)
+    trace.replay(base, paced=False, digests=False)  # warm-up (allocations, code objects)
+    ref, st = trace.replay(base, paced=False)
+    sat = st["jobs_per_s"]
     levels = []
-    for i, load in enumerate(loads):
-        rate = load * sat
-        gaps = -np.log1p(-((pa.gen_u64(seed + 10 + i, jobs) >> np.uint64(11)).astype(np.float64) / float(1 << 53))) / rate
-        arrivals = np.cumsum(gaps) - gaps[0]
-        lat, span, st = run(arrivals)
-        levels.append({"load": load, "offered_jobs_per_s": round(rate, 1), "achieved_jobs_per_s": round(jobs / span, 1),
-                       "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                       "gkeys_per_s": round(keys_total / span / 1e9, 2), "coalesced_jobs": st["coalesced_jobs"]})
+    same = True
+    for load in loads:
+        res, st = trace.replay(trace.generate(jobs, seed, rate=load * sat))
+        same = same and not trace.compare(ref, res)["outputs_differ"]
+        levels.append({"load": load, "offered_jobs_per_s": round(load * sat, 1),
+                       "achieved_jobs_per_s": st["jobs_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                       "gkeys_per_s": round(st["gkeys_per_s"], 2), "coalesced_jobs": st["coalesced_jobs"]})
     return {"workload": f"{jobs} jobs: 70% sort, 20% select, 10% hist; u32/u64 keys 50/50; sizes log-uniform 4K..64M",
-            "saturation_jobs_per_s": round(sat, 1), "levels": levels}
+            "saturation_jobs_per_s": round(sat, 1), "levels": levels, "outputs_equal_across_levels": same}
+
+
+def bench_primitives(pa, torch, n=100_000_000, seed=1241, reps=5):
+    """Device-resident throughput of the path's other entry points at the
+    sort's size: each is timed with HIP events on the launch stream, best of
+    `reps` calls after one warm-up, inputs regenerated where a call sorts in
+    place.  Keys/s counts input keys (merge: output keys; counting sort: bytes)."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    out = {}
+
+    def timed(name, fn, fresh=None, units=n):
+        fn()
+        best = None
+        for _ in range(reps):
+            if fresh:
+                fresh()
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(st)
+            fn()
+            b.record(st)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b)
+            best = ms if best is None else min(best, ms)
+        out[name] = {"ms": round(best, 4), "gkeys_per_s": round(units / (best / 1e3) / 1e9, 2)}
+
+    u32 = torch.from_numpy(pa.gen_u32(seed, n).view(np.int32)).to(dev)
+    w32 = torch.empty_like(u32)
+    t32 = torch.empty_like(u32)
+    r32 = torch.zeros(1, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(256, dtype=torch.int64, device=dev)
+    timed("hist_u32_top_byte", lambda: pa.hist_u32(u32, 24, 8, cnt))
+    timed("select_u32_median", lambda: pa.select_u32(u32, n // 2, r32))
+    timed("partition_u32_median", lambda: pa.partition_u32(w32, n // 2, r32), fresh=lambda: w32.copy_(u32))
+    half = torch.sort(u32[: n // 2].to(torch.int64) & 0xFFFFFFFF)[0].to(torch.int32)
+    half2 = torch.sort(u32[n // 2:].to(torch.int64) & 0xFFFFFFFF)[0].to(torch.int32)
+    timed("merge_u32_2x50M", lambda: pa.merge_u32(half, half2, w32))
+    del half, half2
+    vals = torch.arange(n, dtype=torch.int32, device=dev)
+    v2, vt = torch.empty_like(vals), torch.empty_like(vals)
+
+    def kv_fresh():
+        w32.copy_(u32)
+        v2.copy_(vals)
+    timed("sort_kv_u32_u32", lambda: pa.sort_kv_u32_u32(w32, v2, t32, vt), fresh=kv_fresh)
+    del vals, v2, vt
+    b8 = (u32 & 0xFF).to(torch.uint8)
+    w8 = torch.empty_like(b8)
+    timed("counting_sort_u8", lambda: pa.counting_sort_u8(w8), fresh=lambda: w8.copy_(b8))
+    del b8, w8, w32, t32
+    u64 = torch.from_numpy(pa.gen_u64(seed + 1, n).view(np.int64)).to(dev)
+    w64, t64 = torch.empty_like(u64), torch.empty_like(u64)
+    timed("sort_u64", lambda: pa.sort_u64(w64, t64), fresh=lambda: w64.copy_(u64))
+    f64 = (u64 >> 11).to(torch.float64) * (2.0 ** -53) * 2e6 - 1e6
+    timed("sort_f64", lambda: pa.sort_f64(w64.view(torch.float64), t64.view(torch.float64)),
+          fresh=lambda: w64.view(torch.float64).copy_(f64))
+    del u64, w64, t64, f64, u32
+    torch.cuda.empty_cache()
+    return out
 
 
 def load_traffic(kernel, workload):
@@ -236,6 +249,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--no-seg", action="store_true", help="skip the segmented-batch leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--no-prims", action="store_true", help="skip the other entry points' throughput leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -321,7 +335,10 @@ def main():
     del bufs, src, tmp
     torch.cuda.empty_cache()
 
-    seg = serve = None
+    seg = serve = prims = None
+    if not args.no_prims:
+        print("[bench] other entry points", file=sys.stderr, flush=True)
+        prims = bench_primitives(pa, torch)
     if not args.no_seg:
         print("[bench] segmented batch vs per-array calls", file=sys.stderr, flush=True)
         seg = bench_segmented(pa, torch)
@@ -355,6 +372,7 @@ def main():
         "whole_sort_hbm_frac": round(whole_sort_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "copy_kernel_GBps": round(copy_gbs, 1),
         "kernels": kernels,
+        "primitives_100M": prims,
         "segmented": seg,
         "serve": serve,
         "sorted_ok": ok,

@@ -28,6 +28,8 @@
 #include <atomic>
 #include <type_traits>
 #include <cstdlib>
+#include <utility>
+#include <vector>
 
 #include "pa_sort_kernels.h"
 
@@ -550,14 +552,82 @@ int launch_hist_digits_u32(const uint32_t* keys, size_t n, uint32_t* counts, hip
   return PA_OK;
 }
 
+// Elementwise key transform (forward or inverse) over a device array.
+template <typename K>
+__global__ __launch_bounds__(256) void k_key_xform(K* __restrict__ keys, size_t n, uint32_t xform, uint32_t inverse) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    keys[i] = inverse ? dev::key_inv<K>(keys[i], xform) : dev::key_fwd<K>(keys[i], xform);
+}
+
+// Arrays of 2^30 keys or more (past the 30-bit count of a look-back status
+// word): sort as few equal chunks of at most 2^30 - 1 keys as cover the array,
+// then merge adjacent sorted runs pairwise with the merge-path kernels until
+// one run is left -- the shape of the reference's radix-leaf merge sort
+// (parallel_merge_sort_hybrid_radix, ParallelMergeSort.h:255-295).
+// Transformed orders (float, signed) are applied once before and undone once
+// after, so chunks sort and merge as plain unsigned keys.  tmp holds n keys.
+constexpr size_t kLargeChunk = kMaxSortN;
+
+inline size_t large_chunk_keys(size_t n) {
+  const size_t chunks = (n + kLargeChunk - 1) / kLargeChunk;
+  return (n + chunks - 1) / chunks;
+}
+
+template <typename K>
+size_t large_sort_scratch_bytes(size_t n) {
+  const size_t c = large_chunk_keys(n);
+  return sort_scratch_bytes<K, false>(pick_variant<K, false>(c), c) + 256 + merge_split_bytes(n);
+}
+
+template <typename K>
+int enqueue_large_sort(K* keys, K* tmp, size_t n, char* scratch, hipStream_t s, uint32_t xform) {
+  const size_t csize = large_chunk_keys(n);
+  const PassVariant& var = pick_variant<K, false>(csize);
+  const size_t sort_bytes = sort_scratch_bytes<K, false>(var, csize);
+  size_t* split = reinterpret_cast<size_t*>(scratch + ((sort_bytes + 255) & ~size_t(255)));
+  const unsigned xgrid = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+  if (xform) hipLaunchKernelGGL(k_key_xform<K>, dim3(xgrid), dim3(256), 0, s, keys, n, xform, 0u);
+  std::vector<std::pair<size_t, size_t>> runs;  // (first key, length)
+  for (size_t off = 0; off < n; off += csize) {
+    const size_t len = std::min(csize, n - off);
+    // An even number of digit passes leaves each chunk's result in keys.
+    PA_TRY((enqueue_sort<K, false>(var, keys + off, nullptr, tmp + off, nullptr, len, scratch, s, dev::kXformNone)));
+    runs.emplace_back(off, len);
+  }
+  K* x = keys;
+  K* y = tmp;
+  while (runs.size() > 1) {
+    std::vector<std::pair<size_t, size_t>> next;
+    for (size_t i = 0; i + 1 < runs.size(); i += 2) {
+      const auto [o1, l1] = runs[i];
+      const auto [o2, l2] = runs[i + 1];
+      PA_TRY(merge_enqueue<K>(x + o1, l1, x + o2, l2, y + o1, split, s));
+      next.emplace_back(o1, l1 + l2);
+    }
+    if (runs.size() % 2) {
+      const auto [o, l] = runs.back();
+      PA_HIP(hipMemcpyAsync(y + o, x + o, l * sizeof(K), hipMemcpyDeviceToDevice, s));
+      next.push_back(runs.back());
+    }
+    runs.swap(next);
+    std::swap(x, y);
+  }
+  if (x != keys) PA_HIP(hipMemcpyAsync(keys, x, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+  if (xform) hipLaunchKernelGGL(k_key_xform<K>, dim3(xgrid), dim3(256), 0, s, keys, n, xform, 1u);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
 // Host entry: classifies pointers, stages host arrays, runs, copies back.
 template <typename K, bool KV>
 int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipStream_t s, uint32_t xform,
                const char* who) {
   if (n == 0) return PA_OK;
   if (!keys || (KV && !vals)) return arg_error(who);
-  if (n > kMaxSortN) {
-    set_last_error(std::string(who) + ": n exceeds 2^30-1");
+  const bool large = n > kMaxSortN;
+  if (KV && large) {
+    set_last_error(std::string(who) + ": key-value sorts take n < 2^30");
     return PA_ERR_TOO_LARGE;
   }
   StreamCtx& ctx = stream_ctx(s);
@@ -566,9 +636,9 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   const bool dv = !KV || is_device_ptr(vals);
   const bool dkt = ktmp && is_device_ptr(ktmp);
   const bool dvt = !KV || (vtmp && is_device_ptr(vtmp));
-  const PassVariant& var = pick_variant<K, KV>(n);
+  const PassVariant& var = pick_variant<K, KV>(large ? large_chunk_keys(n) : n);
   Layout L;
-  const size_t o_sort = L.take(sort_scratch_bytes<K, KV>(var, n));
+  const size_t o_sort = L.take(large ? large_sort_scratch_bytes<K>(n) : sort_scratch_bytes<K, KV>(var, n));
   const size_t o_k = dk ? 0 : L.take(n * sizeof(K));
   const size_t o_kt = dkt ? 0 : L.take(n * sizeof(K));
   const size_t o_v = dv ? 0 : L.take(n * 4);
@@ -582,7 +652,15 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   uint32_t* dvtmp = KV ? (dvt ? vtmp : reinterpret_cast<uint32_t*>(b + o_vt)) : nullptr;
   if (!dk) PA_HIP(hipMemcpyAsync(dkeys, keys, n * sizeof(K), hipMemcpyHostToDevice, s));
   if (KV && !dv) PA_HIP(hipMemcpyAsync(dvals, vals, n * 4, hipMemcpyHostToDevice, s));
-  PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
+  if constexpr (!KV) {
+    if (large) {
+      PA_TRY(enqueue_large_sort<K>(dkeys, dktmp, n, b + o_sort, s, xform));
+    } else {
+      PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
+    }
+  } else {
+    PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
+  }
   if (!dk) PA_HIP(hipMemcpyAsync(keys, dkeys, n * sizeof(K), hipMemcpyDeviceToHost, s));
   if (KV && !dv) PA_HIP(hipMemcpyAsync(vals, dvals, n * 4, hipMemcpyDeviceToHost, s));
   if (!dk || (KV && !dv)) PA_HIP(hipStreamSynchronize(s));

@@ -1,0 +1,69 @@
+"""The job-trace wire format (parallelalgorithms_amd.trace) on the host: job
+lines round-trip, invalid jobs are rejected with the field named, the
+generator is deterministic and follows its stated distribution, and compare /
+latency_stats compute what they say.  No GPU."""
+import json
+
+import numpy as np
+import pytest
+
+from parallelalgorithms_amd import trace
+
+
+def test_job_round_trip(tmp_path):
+    jobs = trace.generate(50, seed=7, rate=1000.0, lo=100, hi=10_000)
+    p = tmp_path / "t.jsonl"
+    trace.write_jobs(p, jobs)
+    back = trace.read_jobs(p)
+    assert back == jobs
+    line = json.loads(p.read_text().splitlines()[0])
+    assert set(line) == {"id", "t_us", "op", "dtype", "n", "k", "shift", "bits", "seed"}
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ({"op": "scan"}, "op"), ({"dtype": "u16"}, "dtype"), ({"n": 0}, "n must"), ({"t_us": -1.0}, "t_us"),
+    ({"op": "select", "k": 10}, "k < n"), ({"op": "hist", "bits": 0}, "bits"),
+    ({"op": "hist", "shift": 28, "bits": 8}, r"shift \+ bits"), ({"extra": 1}, "unknown job fields"),
+])
+def test_job_validation(bad, msg):
+    d = {"id": 0, "t_us": 0.0, "op": "sort", "dtype": "u32", "n": 10, "k": 0, "shift": 0, "bits": 8, "seed": 1}
+    d.update(bad)
+    with pytest.raises(ValueError, match=msg):
+        trace.Job.from_dict(d)
+
+
+def test_duplicate_ids_rejected(tmp_path):
+    p = tmp_path / "t.jsonl"
+    j = trace.Job(0, 0.0, "sort", "u32", 5)
+    p.write_text(j.to_json() + "\n" + j.to_json() + "\n")
+    with pytest.raises(ValueError, match="unique"):
+        trace.read_jobs(p)
+
+
+def test_generate_distribution_and_determinism():
+    a = trace.generate(2000, seed=11, rate=500.0)
+    assert a == trace.generate(2000, seed=11, rate=500.0)
+    # Same jobs at another rate: only arrival times change.
+    b = trace.generate(2000, seed=11, rate=1000.0)
+    assert [(x.op, x.dtype, x.n, x.k, x.seed) for x in a] == [(x.op, x.dtype, x.n, x.k, x.seed) for x in b]
+    assert all(abs(x.t_us - 2 * y.t_us) < 1e-2 for x, y in zip(a, b))
+    n = np.array([x.n for x in a])
+    assert n.min() >= 4096 and n.max() <= 64 << 20
+    ops = [x.op for x in a]
+    assert abs(ops.count("sort") / 2000 - 0.70) < 0.04 and abs(ops.count("hist") / 2000 - 0.10) < 0.03
+    assert all(0 <= x.k < x.n / 10 + 1 for x in a if x.op == "select")
+    t = np.array([x.t_us for x in a])
+    assert t[0] == 0 and np.all(np.diff(t) >= 0)
+    assert abs(np.diff(t).mean() - 2000.0) < 150.0  # mean gap 1/rate = 2 ms
+    for x in a:
+        x.validate()
+
+
+def test_compare_and_latency_stats():
+    ra = [{"id": i, "status": "ok", "latency_ms": float(i + 1), "digest": str(i)} for i in range(100)]
+    rb = [dict(r) for r in ra[1:]]
+    rb[4]["digest"] = "x"
+    rep = trace.compare(ra, rb)
+    assert rep["jobs"] == 99 and rep["outputs_differ"] == [5] and rep["only_in_a"] == [0] and rep["only_in_b"] == []
+    st = trace.latency_stats([r["latency_ms"] for r in ra])
+    assert st["p50_ms"] == 50.5 and st["max_ms"] == 100.0 and st["p99_ms"] == pytest.approx(99.01)
