@@ -61,8 +61,21 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge_tile(const K* __restrict_
   const size_t i0 = split[tile], i1 = split[tile + 1];
   const size_t j0 = d0 - i0, j1 = d1 - i1;
   const uint32_t la = (uint32_t)(i1 - i0), lb = (uint32_t)(j1 - j0);
-  for (uint32_t x = tid; x < la; x += kMergeBlock) s_in[x] = a[i0 + x];
-  for (uint32_t x = tid; x < lb; x += kMergeBlock) s_in[la + x] = b[j0 + x];
+  // All kMergeItems loads of a thread are issued before the first LDS store,
+  // so each thread keeps its whole share of the tile in flight.
+  {
+    K v[kMergeItems];
+#pragma unroll
+    for (int x = 0; x < kMergeItems; ++x) {
+      const uint32_t e = tid + x * kMergeBlock;
+      v[x] = e < la ? a[i0 + e] : (e < la + lb ? b[j0 + (e - la)] : K(0));
+    }
+#pragma unroll
+    for (int x = 0; x < kMergeItems; ++x) {
+      const uint32_t e = tid + x * kMergeBlock;
+      if (e < la + lb) s_in[e] = v[x];
+    }
+  }
   __syncthreads();
   const K* sa = s_in;
   const K* sb = s_in + la;
@@ -84,7 +97,11 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge_tile(const K* __restrict_
   for (int x = 0; x < kMergeItems; ++x)
     if (dt + x < total) s_out[dt + x] = r[x];
   __syncthreads();
-  for (uint32_t x = tid; x < total; x += kMergeBlock) out[d0 + x] = s_out[x];
+#pragma unroll
+  for (int x = 0; x < kMergeItems; ++x) {
+    const uint32_t e = tid + x * kMergeBlock;
+    if (e < total) out[d0 + e] = s_out[e];
+  }
 }
 
 template <typename K>

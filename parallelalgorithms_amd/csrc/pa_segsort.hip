@@ -8,15 +8,18 @@
 //
 // Size classes:
 //   len <= 1              nothing to do.
-//   len <= kSmallTile     one workgroup sorts the segment entirely in LDS
+//   len <= 16384          one workgroup sorts the segment entirely in LDS
 //                         (4 stable 8-bit passes over LDS, one read and one
-//                         write of HBM per key).  One launch for all of them.
+//                         write of HBM per key), in three workgroup sizes
+//                         (256, 512, 1024 threads x 16 keys) so a segment
+//                         pays for at most twice its length in LDS slots.
+//                         One launch per size for all of them.
 //   larger                segmented onesweep: one fused read builds every
 //                         segment's 4 digit histograms, then one onesweep
 //                         launch per digit covers every tile of every large
 //                         segment; each segment has its own look-back chain.
-// So the whole batch costs 7 launches however many segments it holds, where
-// per-array calls cost 6 launches per array.
+// So the whole batch costs at most 9 launches however many segments it holds,
+// where per-array calls cost 6 launches per array.
 #include <algorithm>
 #include <vector>
 
@@ -24,9 +27,12 @@
 
 namespace pa {
 
-constexpr int kSmallBlock = 256;
+// LDS classes: segments of up to kLdsLimit[c] keys are sorted by one
+// workgroup of kLdsLimit[c] / kSmallItems threads.
 constexpr int kSmallItems = 16;
-constexpr uint32_t kSmallTile = kSmallBlock * kSmallItems;  // 4096 keys
+constexpr int kLdsClasses = 3;
+constexpr uint32_t kLdsLimit[kLdsClasses] = {4096, 8192, 16384};
+constexpr uint32_t kLdsMax = kLdsLimit[kLdsClasses - 1];
 
 // Segmented onesweep tile shape (keys only): the plain sort's default pass,
 // 512 x 48 keys staged in two rounds.
@@ -43,14 +49,17 @@ constexpr uint32_t kSegBuckets = 1024;
 
 // Whole-segment LDS sort.  Padding slots hold 0xFFFFFFFF: a stable sort
 // keeps them after every real key, so the first len slots are the answer.
-__global__ __launch_bounds__(kSmallBlock) void k_seg_small(const uint64_t* __restrict__ ptr,
-                                                           const uint32_t* __restrict__ len) {
-  constexpr int WAVES = kSmallBlock / 64;
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_small(const uint64_t* __restrict__ ptr,
+                                                     const uint32_t* __restrict__ len) {
+  constexpr int WAVES = BLOCK / 64;
   constexpr int WAVE_SPAN = 64 * kSmallItems;
-  __shared__ uint32_t s_keys[kSmallTile];
+  constexpr int TILE = BLOCK * kSmallItems;
+  static_assert(BLOCK >= 256, "one thread per digit in the scan");
+  __shared__ uint32_t s_keys[TILE];
   __shared__ uint32_t s_whist[WAVES * 256];
   __shared__ uint64_t s_match[WAVES * 256];
-  __shared__ uint32_t s_wsum[WAVES];
+  __shared__ uint32_t s_wsum[4];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   uint32_t* seg = reinterpret_cast<uint32_t*>(ptr[blockIdx.x]);
   const uint32_t n = len[blockIdx.x];
@@ -63,9 +72,9 @@ __global__ __launch_bounds__(kSmallBlock) void k_seg_small(const uint64_t* __res
   }
   uint32_t* wh = s_whist + wave * 256;
   uint64_t* wm = s_match + wave * 256;
-  for (uint32_t i = tid; i < WAVES * 256; i += kSmallBlock) s_match[i] = 0ull;
+  for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_match[i] = 0ull;
   for (uint32_t shift = 0; shift < 32; shift += 8) {
-    for (uint32_t i = tid; i < WAVES * 256; i += kSmallBlock) s_whist[i] = 0;
+    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_whist[i] = 0;
     __syncthreads();
     // Peers of one digit within the wave: each lane ORs its bit into the
     // wave's LDS slot for its digit; the lowest peer clears the slot.
@@ -84,22 +93,27 @@ __global__ __launch_bounds__(kSmallBlock) void k_seg_small(const uint64_t* __res
       }
     }
     __syncthreads();
-    // Digit totals, wave prefixes and the block-wide digit offsets.
-    uint32_t cnt = 0;
+    // Digit totals, wave prefixes and the block-wide digit offsets (threads
+    // 0..255, one digit each).
+    uint32_t cnt = 0, inc = 0;
+    if (tid < 256) {
 #pragma unroll
-    for (int w = 0; w < WAVES; ++w) {
-      const uint32_t c = s_whist[w * 256 + tid];
-      s_whist[w * 256 + tid] = cnt;
-      cnt += c;
+      for (int w = 0; w < WAVES; ++w) {
+        const uint32_t c = s_whist[w * 256 + tid];
+        s_whist[w * 256 + tid] = cnt;
+        cnt += c;
+      }
+      inc = dev::wave_incl_scan(cnt, lane);
+      if (lane == 63) s_wsum[wave] = inc;
     }
-    const uint32_t inc = dev::wave_incl_scan(cnt, lane);
-    if (lane == 63) s_wsum[wave] = inc;
     __syncthreads();
-    uint32_t pre = 0;
-    for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
-    const uint32_t start = pre + inc - cnt;
+    if (tid < 256) {
+      uint32_t pre = 0;
+      for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
+      const uint32_t start = pre + inc - cnt;
 #pragma unroll
-    for (int w = 0; w < WAVES; ++w) s_whist[w * 256 + tid] += start;
+      for (int w = 0; w < WAVES; ++w) s_whist[w * 256 + tid] += start;
+    }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kSmallItems; ++i) s_keys[wh[(key[i] >> shift) & 0xFFu] + rank[i]] = key[i];
@@ -108,7 +122,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_seg_small(const uint64_t* __res
     for (int i = 0; i < kSmallItems; ++i) key[i] = s_keys[wbase + i * 64];
     __syncthreads();
   }
-  for (uint32_t j = tid; j < n; j += kSmallBlock) seg[j] = s_keys[j];
+  for (uint32_t j = tid; j < n; j += BLOCK) seg[j] = s_keys[j];
 }
 
 __global__ __launch_bounds__(256) void k_seg_prepare(uint32_t* __restrict__ counts, size_t ncounts,
@@ -152,14 +166,16 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restri
 // Sorts the segments described by (address, length) lists; large segments get
 // ping-pong space from `work` (their tmp addresses).  ctx.mu must be held.
 struct SegPlan {
-  std::vector<uint64_t> small_ptr, large_in, large_tmp, tile_info;
-  std::vector<uint32_t> small_len, large_len, large_tiles, large_row;
+  std::vector<uint64_t> small_ptr[kLdsClasses], large_in, large_tmp, tile_info;
+  std::vector<uint32_t> small_len[kLdsClasses], large_len, large_tiles, large_row;
   void add(uint64_t ptr, uint64_t len, uint64_t tmp_ptr) {
     if (len <= 1) return;
-    if (len <= kSmallTile) {
-      small_ptr.push_back(ptr);
-      small_len.push_back((uint32_t)len);
-      return;
+    for (int c = 0; c < kLdsClasses; ++c) {
+      if (len <= kLdsLimit[c]) {
+        small_ptr[c].push_back(ptr);
+        small_len[c].push_back((uint32_t)len);
+        return;
+      }
     }
     large_in.push_back(ptr);
     large_tmp.push_back(tmp_ptr);
@@ -192,16 +208,18 @@ struct SegPlan {
 // tile order) followed by per-segment counts, tile counters, the look-back
 // table and `extra` bytes of ping-pong space for large segments.
 struct PlanLayout {
-  size_t t_sp, t_sl, t_li, t_lt, t_ll, t_lr, t_ti, table_bytes;
+  size_t t_sp[kLdsClasses], t_sl[kLdsClasses], t_li, t_lt, t_ll, t_lr, t_ti, table_bytes;
   size_t o_tab, o_cnt, o_ctr, o_lb, o_work, total;
 };
 
 static PlanLayout plan_layout(const SegPlan& P, size_t extra) {
-  const size_t ns = P.small_ptr.size(), nl = P.large_in.size(), nt = P.tile_info.size();
+  const size_t nl = P.large_in.size(), nt = P.tile_info.size();
   PlanLayout Y;
   Layout T;
-  Y.t_sp = T.take(ns * 8);
-  Y.t_sl = T.take(ns * 4);
+  for (int c = 0; c < kLdsClasses; ++c) {
+    Y.t_sp[c] = T.take(P.small_ptr[c].size() * 8);
+    Y.t_sl[c] = T.take(P.small_len[c].size() * 4);
+  }
   Y.t_li = T.take(nl * 8);
   Y.t_lt = T.take(nl * 8);
   Y.t_ll = T.take(nl * 4);
@@ -228,8 +246,10 @@ static void assign_work(SegPlan& P, char* work) {
 }
 
 static void write_tables(const SegPlan& P, const PlanLayout& Y, char* pin) {
-  std::copy(P.small_ptr.begin(), P.small_ptr.end(), (uint64_t*)(pin + Y.t_sp));
-  std::copy(P.small_len.begin(), P.small_len.end(), (uint32_t*)(pin + Y.t_sl));
+  for (int c = 0; c < kLdsClasses; ++c) {
+    std::copy(P.small_ptr[c].begin(), P.small_ptr[c].end(), (uint64_t*)(pin + Y.t_sp[c]));
+    std::copy(P.small_len[c].begin(), P.small_len[c].end(), (uint32_t*)(pin + Y.t_sl[c]));
+  }
   std::copy(P.large_in.begin(), P.large_in.end(), (uint64_t*)(pin + Y.t_li));
   std::copy(P.large_tmp.begin(), P.large_tmp.end(), (uint64_t*)(pin + Y.t_lt));
   std::copy(P.large_len.begin(), P.large_len.end(), (uint32_t*)(pin + Y.t_ll));
@@ -240,10 +260,8 @@ static void write_tables(const SegPlan& P, const PlanLayout& Y, char* pin) {
 // The kernels of one plan, on device memory laid out by plan_layout.  Only
 // kernels (no memset, no copy), so the sequence can be captured in a graph.
 static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStream_t s) {
-  const size_t ns = P.small_ptr.size(), nl = P.large_in.size(), nt = P.tile_info.size();
+  const size_t nl = P.large_in.size(), nt = P.tile_info.size();
   const char* tab = dev + Y.o_tab;
-  const uint64_t* d_sp = (const uint64_t*)(tab + Y.t_sp);
-  const uint32_t* d_sl = (const uint32_t*)(tab + Y.t_sl);
   const uint64_t* d_li = (const uint64_t*)(tab + Y.t_li);
   const uint64_t* d_lt = (const uint64_t*)(tab + Y.t_lt);
   const uint32_t* d_ll = (const uint32_t*)(tab + Y.t_ll);
@@ -253,9 +271,17 @@ static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStre
   uint32_t* ctr = (uint32_t*)(dev + Y.o_ctr);
   uint32_t* lbp = (uint32_t*)(dev + Y.o_lb);
   const size_t lb_bytes = (size_t)4 * nt * 256 * 4;
-  if (ns) {
+  for (int c = 0; c < kLdsClasses; ++c) {
+    const size_t ns = P.small_ptr[c].size();
+    if (!ns) continue;
+    const uint64_t* d_sp = (const uint64_t*)(tab + Y.t_sp[c]);
+    const uint32_t* d_sl = (const uint32_t*)(tab + Y.t_sl[c]);
     ProfScope ps("k_seg_small", s);
-    hipLaunchKernelGGL(k_seg_small, dim3((unsigned)ns), dim3(kSmallBlock), 0, s, d_sp, d_sl);
+    switch (c) {
+      case 0: hipLaunchKernelGGL(k_seg_small<256>, dim3((unsigned)ns), dim3(256), 0, s, d_sp, d_sl); break;
+      case 1: hipLaunchKernelGGL(k_seg_small<512>, dim3((unsigned)ns), dim3(512), 0, s, d_sp, d_sl); break;
+      default: hipLaunchKernelGGL(k_seg_small<1024>, dim3((unsigned)ns), dim3(1024), 0, s, d_sp, d_sl); break;
+    }
   }
   if (nl) {
     hipLaunchKernelGGL(k_seg_prepare, dim3((unsigned)std::min<size_t>((nl * 1024 + 255) / 256, 4096)), dim3(256), 0, s,
@@ -311,7 +337,7 @@ static int build_ptr_plan(uint32_t* const* keys, const uint64_t* lens, size_t nu
       return PA_ERR_TOO_LARGE;
     }
     P->add((uint64_t)(uintptr_t)keys[i], lens[i], 0);
-    if (lens[i] > kSmallTile) *large_total += lens[i];
+    if (lens[i] > kLdsMax) *large_total += lens[i];
   }
   for (size_t i = 0; i < num_segments; ++i)
     if (lens[i] > 1 && !is_device_ptr(keys[i]))
@@ -346,7 +372,7 @@ int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets
       return PA_ERR_TOO_LARGE;
     }
     P.add((uint64_t)(uintptr_t)(keys + offsets[i]), len, tmp ? (uint64_t)(uintptr_t)(tmp + offsets[i]) : 0);
-    if (len > kSmallTile) large_total += len;
+    if (len > kLdsMax) large_total += len;
   }
   P.finalize();
   StreamCtx& ctx = stream_ctx(s);

@@ -35,7 +35,9 @@ def _check(keys, offs, got, orc=None, oracle_every=0):
 
 
 def test_segmented_edge_classes(pa, orc):
-    lens = [0, 1, 2, 3, 63, 64, 4095, 4096, 4097, 8191, 8192, 8193, 20_000, 0, 1, 100_003, 7]
+    # Around every class edge: LDS 4096 / 8192 / 16384, onesweep tiles of 24576.
+    lens = [0, 1, 2, 3, 63, 64, 4095, 4096, 4097, 8191, 8192, 8193, 16_383, 16_384, 16_385, 20_000, 24_576, 24_577,
+            0, 1, 100_003, 7]
     offs = np.zeros(len(lens) + 1, dtype=np.uint64)
     offs[1:] = np.cumsum(lens)
     import parallelalgorithms_amd as P
