@@ -44,26 +44,44 @@ __device__ __host__ inline size_t part_tiles_per_block(size_t n, size_t nb) {
 __global__ __launch_bounds__(kPartBlock) void k_part_count(const uint32_t* __restrict__ keys, size_t n,
                                                            const uint32_t* __restrict__ pivot,
                                                            uint32_t* __restrict__ counts) {
+  // Counting needs no order, so the block reads its whole key range with
+  // 16-byte loads (four in flight per thread) when the keys are 16-byte
+  // aligned; the range starts on a tile boundary, a multiple of 4 keys.
   __shared__ uint32_t s_c[2];
   const uint32_t tid = threadIdx.x;
   if (tid < 2) s_c[tid] = 0;
   __syncthreads();
   const uint32_t v = *pivot;
   uint32_t lt = 0, eq = 0;
-  const size_t ntiles = (n + kPartTile - 1) / kPartTile;
-  const size_t tpb = part_tiles_per_block(n, gridDim.x);
-  const size_t t1 = std::min<size_t>(ntiles, (blockIdx.x + 1) * tpb);
-  for (size_t t = blockIdx.x * tpb; t < t1; ++t) {
-    const size_t base = t * kPartTile;
+  const size_t span = part_tiles_per_block(n, gridDim.x) * kPartTile;
+  const size_t lo = std::min<size_t>(n, (size_t)blockIdx.x * span), hi = std::min<size_t>(n, lo + span);
+  size_t scalar_from = lo;
+  if (((uintptr_t)keys & 15) == 0) {
+    const uint4* vk = reinterpret_cast<const uint4*>(keys + lo);
+    const size_t nv = (hi - lo) / 4;
+    constexpr int U = 4;
+    for (size_t i0 = tid; i0 < nv; i0 += (size_t)kPartBlock * U) {
+      uint4 x[U];
 #pragma unroll
-    for (int j = 0; j < kPartItems; ++j) {
-      const size_t idx = base + (tid >> 6) * 1024 + j * 64 + (tid & 63);
-      if (idx < n) {
-        const uint32_t k = keys[idx];
-        lt += k < v;
-        eq += k == v;
+      for (int u = 0; u < U; ++u)
+        if (i0 + u * kPartBlock < nv) x[u] = vk[i0 + u * kPartBlock];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (i0 + u * kPartBlock >= nv) continue;
+        const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          lt += w[j] < v;
+          eq += w[j] == v;
+        }
       }
     }
+    scalar_from = lo + nv * 4;
+  }
+  for (size_t i = scalar_from + tid; i < hi; i += kPartBlock) {
+    const uint32_t k = keys[i];
+    lt += k < v;
+    eq += k == v;
   }
   // wave reduction, then one atomic per wave into LDS
 #pragma unroll

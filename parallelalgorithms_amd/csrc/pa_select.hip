@@ -9,18 +9,21 @@
 // same scheme over digits 56 .. 0 (an extension; the reference is 32-bit).
 //
 // GPU form (the caller's array is never modified):
-//   k_select_first  one read of all keys builds the 4 digit histograms.  The
-//                   last workgroup to finish walks the digits from the top:
-//                   while the chosen bin still holds every candidate (constant
-//                   input, shared high bytes) the next digit's histogram over
-//                   the same set is already known, so no further read is needed.
-//   k_select_pass   (launched for each remaining digit; exits at once when the
-//                   answer is known) histograms the next digit over the keys
-//                   matching the fixed high digits.  When the candidate set is
-//                   small enough it is compacted into a buffer in the same
-//                   read, so later passes read only the candidates.
-// Control state lives in device memory; the whole select is enqueued without
-// a host round trip.
+//   k_select_first  one read of all keys builds the D digit histograms.
+//   k_select_walk   (one block) walks the digits from the top: while the
+//                   chosen bin still holds every candidate (constant input,
+//                   shared high bytes) the next digit's histogram over the same
+//                   set is already known, so no further read is needed.
+//   k_select_pass   (each remaining digit; returns at once when the answer is
+//                   known) histograms the next digit over the keys matching the
+//                   fixed high digits.  When the candidates fit the buffer
+//                   (half the input) they are compacted in the same read, so
+//                   later passes read only the candidates.
+//   k_select_pick   (one block) picks that digit's bin.
+// Blocks add their LDS counts to global histograms with atomics and the
+// one-block kernels read them after the launch boundary: no grid-wide fence
+// or last-block election.  Control state lives in device memory; the whole
+// select is enqueued without a host round trip.
 #include <algorithm>
 
 #include "pa_internal.h"
@@ -80,27 +83,21 @@ __device__ void pick_bin(const uint32_t* s_row, uint64_t k, uint32_t* s_wsum, ui
   __syncthreads();
 }
 
-// Last-workgroup detection: returns true in the block that finished last.
-__device__ bool last_block(uint32_t* done_ctr, uint32_t* s_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) *s_flag = (atomicAdd(done_ctr, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (*s_flag) __threadfence();
-  return *s_flag;
+// Adds a block's LDS counts into the global histogram (zero bins skipped).
+__device__ __forceinline__ void flush_counts(const uint32_t* s_hist, uint32_t* ghist, uint32_t nbins) {
+  for (uint32_t i = threadIdx.x; i < nbins; i += kSelBlock)
+    if (s_hist[i]) atomicAdd(&ghist[i], s_hist[i]);
 }
 
+// One read of all keys: the D digit histograms.  Four 16-byte loads per thread
+// are in flight per step.
 template <typename K>
 __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict__ keys, size_t n,
-                                                            SelState* __restrict__ st, uint32_t* __restrict__ ghist,
-                                                            uint32_t* __restrict__ done_ctr) {
+                                                            uint32_t* __restrict__ ghist) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
+  constexpr int U = 4;
   __shared__ uint32_t s_hist[D * 256];
-  __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_pick[1];
-  __shared__ uint64_t s_below;
-  __shared__ uint32_t s_flag;
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < D * 256; i += kSelBlock) s_hist[i] = 0;
   __syncthreads();
@@ -109,16 +106,23 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   const size_t nvec = (n - head) / PER;
   const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
   const size_t stride = (size_t)gridDim.x * kSelBlock;
-  for (size_t v = (size_t)blockIdx.x * kSelBlock + tid; v < nvec; v += stride) {
-    const uint4 x = vk[v];
-    K kk[PER];
-    __builtin_memcpy(kk, &x, 16);
+  for (size_t v0 = (size_t)blockIdx.x * kSelBlock + tid; v0 < nvec; v0 += stride * U) {
+    uint4 x[U];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      uint32_t dg[PER];
+    for (int u = 0; u < U; ++u)
+      if (v0 + u * stride < nvec) x[u] = vk[v0 + u * stride];
 #pragma unroll
-      for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(kk[j], 8 * d);
-      dev::lds_count<PER>(s_hist + d * 256, dg);
+    for (int u = 0; u < U; ++u) {
+      if (v0 + u * stride >= nvec) continue;
+      K kk[PER];
+      __builtin_memcpy(kk, &x[u], 16);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        uint32_t dg[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(kk[j], 8 * d);
+        dev::lds_count<PER>(s_hist + d * 256, dg);
+      }
     }
   }
   if (blockIdx.x == 0) {
@@ -131,13 +135,22 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
     }
   }
   __syncthreads();
-  for (uint32_t i = tid; i < D * 256; i += kSelBlock)
-    if (s_hist[i]) atomicAdd(&ghist[i], s_hist[i]);
-  if (!last_block(done_ctr, &s_flag)) return;
+  flush_counts(s_hist, ghist, D * 256);
+}
 
-  // Last block: walk the digits from the top while the set is not narrowed.
-  for (uint32_t i = tid; i < D * 256; i += kSelBlock) {
-    s_hist[i] = dev::atomic_load_agent(&ghist[i]);
+// After the first read (one block): walk the digits from the top while the
+// chosen bin still holds every candidate (constant input, shared high bytes),
+// so no further read is needed for those digits.  Clears the histograms.
+template <typename K>
+__global__ __launch_bounds__(256) void k_select_walk(SelState* __restrict__ st, uint32_t* __restrict__ ghist) {
+  constexpr int D = sizeof(K);
+  __shared__ uint32_t s_hist[D * 256];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_pick[1];
+  __shared__ uint64_t s_below;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < D * 256; i += 256) {
+    s_hist[i] = ghist[i];
     ghist[i] = 0;
   }
   __syncthreads();
@@ -168,28 +181,27 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
     st->result = prefix;
     st->src = 0;
     st->compact_now = cand <= st->cap;
-    *done_ctr = 0;
   }
 }
 
-// One filtered histogram pass at st->shift over keys matching the fixed
-// digits; optionally compacts the matching keys into buf.
+// One filtered histogram pass at st->shift over the keys matching the fixed
+// digits; when st->compact_now is set, the matching keys are also appended to
+// buf.  Each wave stages its matches in its own LDS slice and flushes the
+// slice with one global atomic when it fills and at the end, so no block-wide
+// barrier sits inside the loop; one wave scan places a step's matches.
 template <typename K>
 __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__ keys, size_t n,
-                                                           K* __restrict__ buf, SelState* __restrict__ st,
-                                                           uint32_t* __restrict__ ghist, uint32_t* __restrict__ done_ctr,
-                                                           uint32_t* __restrict__ buf_ctr) {
-  __shared__ uint32_t s_hist[256];
+                                                           K* __restrict__ buf, const SelState* __restrict__ st,
+                                                           uint32_t* __restrict__ ghist,
+                                                           unsigned long long* __restrict__ buf_ctr) {
   constexpr int PER = 16 / sizeof(K);
-  __shared__ K s_stage[kStage];
-  __shared__ uint32_t s_stage_n;
-  __shared__ uint64_t s_stage_base;
-  __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_pick[1];
-  __shared__ uint64_t s_below;
-  __shared__ uint32_t s_flag;
-  if (st->done) return;  // uniform for the whole grid: nobody counts in
-  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  constexpr int U = 4;
+  constexpr int WAVES = kSelBlock / 64;
+  constexpr uint32_t STAGE = 64 * U * PER;  // one step's worst case per wave
+  __shared__ uint32_t s_hist[256];
+  __shared__ K s_stage[WAVES][STAGE];
+  if (st->done) return;  // uniform for the whole grid
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const K prefix = (K)st->prefix, pmask = (K)st->pmask;
   const uint32_t shift = (uint32_t)st->shift;
   const bool from_buf = st->src != 0;
@@ -197,100 +209,132 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
   const K* src = from_buf ? buf : keys;
   const size_t m = from_buf ? (size_t)st->buf_count : n;
   if (tid < 256) s_hist[tid] = 0;
-  if (tid == 0) s_stage_n = 0;
   __syncthreads();
-
-  // Vector part: every thread runs the same number of iterations so the
-  // compaction flush can synchronise the block.
+  K* stage = s_stage[wave];
+  uint32_t staged = 0;  // wave-uniform
+  auto flush = [&]() {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(buf_ctr, (unsigned long long)staged);
+    base = __shfl(base, 0, 64);
+    for (uint32_t j = lane; j < staged; j += 64) buf[base + j] = stage[j];
+    staged = 0;
+  };
+  // Appends this lane's matches (bit u*PER+j of hits: key j of vector u) to
+  // the wave's slice, flushing first if the slice cannot hold the step.
+  auto take = [&](const uint4 (&x)[U], uint32_t hits) {
+    const uint32_t nm = (uint32_t)__popc(hits);
+    const uint32_t inc = dev::wave_incl_scan(nm, lane);
+    const uint32_t wtot = __shfl(inc, 63, 64);
+    if (wtot == 0) return;
+    if (staged + wtot > STAGE) flush();
+    __builtin_amdgcn_wave_barrier();
+    uint32_t pos = staged + inc - nm;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      K xs[PER];
+      __builtin_memcpy(xs, &x[u], 16);
+#pragma unroll
+      for (int j = 0; j < PER; ++j)
+        if ((hits >> (u * PER + j)) & 1u) stage[pos++] = xs[j];
+    }
+    __builtin_amdgcn_wave_barrier();
+    staged += wtot;
+  };
   size_t head = ((16 - ((uintptr_t)src & 15)) & 15) / sizeof(K);
   if (head > m) head = m;
   const size_t nvec = (m - head) / PER;
   const uint4* vk = reinterpret_cast<const uint4*>(src + head);
   const size_t stride = (size_t)gridDim.x * kSelBlock;
-  const size_t iters = (nvec + stride - 1) / stride;
-  for (size_t it = 0; it < iters; ++it) {
-    const size_t v = it * stride + (size_t)blockIdx.x * kSelBlock + tid;
-    K kk[PER] = {};
-    uint32_t nm = 0;
-    if (v < nvec) {
-      const uint4 x = vk[v];
-      K xs[PER];
-      __builtin_memcpy(xs, &x, 16);
-      bool all = true;
+  // Every lane of a wave runs the same iterations (the bound uses the wave's
+  // first index), so the wave-level staging stays convergent.
+  const size_t wfirst = (size_t)blockIdx.x * kSelBlock + wave * 64;
+  for (size_t w0 = wfirst; w0 < nvec; w0 += stride * U) {
+    const size_t v0 = w0 + lane;
+    uint4 x[U];
+    uint32_t hits = 0;
 #pragma unroll
-      for (int j = 0; j < PER; ++j) {
-        const bool hit = (xs[j] & pmask) == prefix;
-        all = all && hit;
-        if (hit) kk[nm++] = xs[j];
-      }
-      if (all) {  // wave-aggregated counting for runs (presorted, shared digits)
+    for (int u = 0; u < U; ++u)
+      if (v0 + u * stride < nvec) x[u] = vk[v0 + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (v0 + u * stride >= nvec) continue;
+      K xs[PER];
+      __builtin_memcpy(xs, &x[u], 16);
+      uint32_t h = 0;
+#pragma unroll
+      for (int j = 0; j < PER; ++j) h |= (uint32_t)((xs[j] & pmask) == prefix) << j;
+      if (h == (1u << PER) - 1) {  // wave-aggregated counting for runs (presorted, shared digits)
         uint32_t dg[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(xs[j], shift);
         dev::lds_count<PER>(s_hist, dg);
       } else {
-        for (uint32_t j = 0; j < nm; ++j) atomicAdd(&s_hist[dev::digit8<K>(kk[j], shift)], 1u);
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+          if ((h >> j) & 1u) atomicAdd(&s_hist[dev::digit8<K>(xs[j], shift)], 1u);
       }
+      hits |= h << (u * PER);
     }
-    if (compact) {
-      // Reserve LDS stage space per wave, then copy this lane's matches.
-      const uint32_t inc = dev::wave_incl_scan(nm, lane);
-      const uint32_t wtot = __shfl(inc, 63, 64);
-      uint32_t wbase = 0;
-      if (lane == 63 && wtot) wbase = atomicAdd(&s_stage_n, wtot);
-      wbase = __shfl(wbase, 63, 64);
-      for (uint32_t j = 0; j < nm; ++j) s_stage[wbase + inc - nm + j] = kk[j];
-      __syncthreads();
-      if (s_stage_n > kStage - PER * kSelBlock || it + 1 == iters) {
-        if (tid == 0) s_stage_base = atomicAdd(buf_ctr, s_stage_n);
-        __syncthreads();
-        const uint32_t cnt = s_stage_n;
-        for (uint32_t j = tid; j < cnt; j += kSelBlock) buf[s_stage_base + j] = s_stage[j];
-        __syncthreads();
-        if (tid == 0) s_stage_n = 0;
-        __syncthreads();
-      }
-    }
+    if (compact) take(x, hits);
   }
-  // Head and tail elements outside whole vectors (at most 6): block 0.
-  if (blockIdx.x == 0 && tid == 0) {
+  // Head and tail elements outside whole vectors (at most 2 * PER - 2): wave 0 of block 0.
+  if (blockIdx.x == 0 && wave == 0) {
     const size_t tail = head + nvec * PER;
     const size_t extra = head + (m - tail);
-    for (size_t e = 0; e < extra; ++e) {
-      const K key = src[e < head ? e : tail + (e - head)];
-      if ((key & pmask) == prefix) {
-        atomicAdd(&s_hist[dev::digit8<K>(key, shift)], 1u);
-        if (compact) buf[atomicAdd(buf_ctr, 1u)] = key;
+    for (size_t e0 = 0; e0 < extra; e0 += 64) {
+      const size_t e = e0 + lane;
+      uint4 x[U] = {};
+      uint32_t hits = 0;
+      if (e < extra) {
+        const K key = src[e < head ? e : tail + (e - head)];
+        __builtin_memcpy(&x[0], &key, sizeof(K));
+        if ((key & pmask) == prefix) {
+          atomicAdd(&s_hist[dev::digit8<K>(key, shift)], 1u);
+          hits = 1;
+        }
       }
+      if (compact) take(x, hits);
     }
   }
+  if (compact && staged) flush();
   __syncthreads();
-  if (tid < 256 && s_hist[tid]) atomicAdd(&ghist[tid], s_hist[tid]);
-  if (!last_block(done_ctr, &s_flag)) return;
+  flush_counts(s_hist, ghist, 256);
+}
 
-  if (tid < 256) {
-    s_hist[tid] = dev::atomic_load_agent(&ghist[tid]);
-    ghist[tid] = 0;
-  }
+// After a pass (one block): picks the bin of digit st->shift holding rank k,
+// fixes that digit, and switches later passes to the compacted buffer when
+// this pass compacted.  Clears the histogram and the buffer counter.
+template <typename K>
+__global__ __launch_bounds__(256) void k_select_pick(SelState* __restrict__ st, uint32_t* __restrict__ ghist,
+                                                     unsigned long long* __restrict__ buf_ctr) {
+  __shared__ uint32_t s_hist[256];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_pick[1];
+  __shared__ uint64_t s_below;
+  if (st->done) return;
+  const uint32_t tid = threadIdx.x;
+  s_hist[tid] = ghist[tid];
+  ghist[tid] = 0;
   __syncthreads();
   const uint64_t k = st->k;
+  const uint32_t shift = (uint32_t)st->shift;
   pick_bin(s_hist, k, s_wsum, s_pick, &s_below);
   if (tid == 0) {
     const uint32_t b = s_pick[0];
+    const bool compacted = !st->src && st->compact_now;
     st->k = k - s_below;
     st->cand = s_hist[b];
-    st->prefix = (uint64_t)prefix | ((uint64_t)b << shift);
-    st->pmask = (uint64_t)pmask | ((uint64_t)0xFF << shift);
+    st->prefix |= (uint64_t)b << shift;
+    st->pmask |= (uint64_t)0xFF << shift;
     st->shift = (int32_t)shift - 8;
     st->done = shift == 0;
-    st->result = (uint64_t)prefix | ((uint64_t)b << shift);
-    if (compact) {
+    st->result = st->prefix;
+    if (compacted) {
       st->src = 1;
-      st->buf_count = dev::atomic_load_agent(buf_ctr);
+      st->buf_count = *buf_ctr;
       *buf_ctr = 0;
     }
     st->compact_now = !st->src && st->cand <= st->cap;
-    *done_ctr = 0;
   }
 }
 
@@ -304,36 +348,56 @@ __global__ void k_select_result(const SelState* st, K* dst) { *dst = (K)st->resu
 // Select on device keys into caller-provided scratch (no allocation, no
 // locking): used by the partition.  The selected key is written to
 // *value_out (device memory).
+// Zeroed control words: D digit histograms, one pass histogram, and the 64-bit
+// compaction counter.
+template <typename K>
+static constexpr size_t select_zero_words() { return sizeof(K) * 256 + 256 + 2; }
+
 template <typename K>
 static size_t select_bytes(size_t n) {
-  constexpr int D = sizeof(K);
   Layout L;
   L.take(sizeof(SelState));
-  L.take((D * 256 + 256 + 2) * 4);
+  L.take(select_zero_words<K>() * 4);
   L.take(std::max<size_t>(n / 2, 1) * sizeof(K));
   return L.off;
 }
 
+// The launch sequence of one select, all stream-ordered with no host round
+// trip: the first read with the digit walk, then for each remaining digit a
+// filtered pass and its bin pick (both return at once when the answer is
+// known), then the result.
+template <typename K>
+static void launch_select(const K* din, size_t n, size_t k, SelState* st, uint32_t* z, uint32_t nz, K* buf, size_t cap,
+                          hipStream_t s) {
+  constexpr int D = sizeof(K);
+  uint32_t* ghistD = z;
+  uint32_t* ghist = z + D * 256;
+  unsigned long long* buf_ctr = reinterpret_cast<unsigned long long*>(ghist + 256);
+  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
+                     (int32_t)(8 * (D - 1)), z, nz);
+  const size_t nvec = n / (16 / sizeof(K));
+  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + 4 * kSelBlock - 1) / (4 * kSelBlock), 512));
+  {
+    ProfScope ps("k_select_first", s);
+    hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghistD);
+  }
+  hipLaunchKernelGGL(k_select_walk<K>, dim3(1), dim3(256), 0, s, st, ghistD);
+  for (int pass = 0; pass < D - 1; ++pass) {
+    ProfScope ps("k_select_pass", s);
+    hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, st, ghist, buf_ctr);
+    hipLaunchKernelGGL(k_select_pick<K>, dim3(1), dim3(256), 0, s, st, ghist, buf_ctr);
+  }
+}
+
 template <typename K>
 static int select_enqueue(const K* din, size_t n, size_t k, char* b, K* value_out, hipStream_t s) {
-  constexpr int D = sizeof(K);
   const size_t cap = std::max<size_t>(n / 2, 1);
-  const size_t nz = D * 256 + 256 + 2;
+  const size_t nz = select_zero_words<K>();
   Layout L;
   SelState* st = (SelState*)(b + L.take(sizeof(SelState)));
   uint32_t* z = (uint32_t*)(b + L.take(nz * 4));
   K* buf = (K*)(b + L.take(cap * sizeof(K)));
-  uint32_t* ghistD = z;
-  uint32_t* ghist = z + D * 256;
-  uint32_t* done_ctr = ghist + 256;
-  uint32_t* buf_ctr = done_ctr + 1;
-  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
-                     (int32_t)(8 * (D - 1)), z, (uint32_t)nz);
-  const size_t nvec = n / (16 / sizeof(K));
-  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + kSelBlock) / kSelBlock, 1024));
-  hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, st, ghistD, done_ctr);
-  for (int pass = 0; pass < D - 1; ++pass)
-    hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, st, ghist, done_ctr, buf_ctr);
+  launch_select<K>(din, n, k, st, z, (uint32_t)nz, buf, cap, s);
   hipLaunchKernelGGL(k_select_result<K>, dim3(1), dim3(1), 0, s, st, value_out);
   PA_HIP(hipGetLastError());
   return PA_OK;
@@ -346,7 +410,6 @@ int select_enqueue_u32(const uint32_t* din, size_t n, size_t k, char* scratch, u
 
 template <typename K>
 static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, hipStream_t s, const char* who) {
-  constexpr int D = sizeof(K);
   if (!keys || !result) return arg_error(who);
   if (n == 0 || k >= n) return arg_error("select: need 0 <= k < n");
   if (n > 0xFFFFFFFFull) return arg_error("select: n must be < 2^32");
@@ -357,7 +420,7 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
   const size_t cap = std::max<size_t>(n / 2, 1);
   Layout L;
   const size_t o_st = L.take(sizeof(SelState));
-  const size_t nz = D * 256 + 256 + 2;  // ghistD, ghist, done counter, buffer counter
+  const size_t nz = select_zero_words<K>();
   const size_t o_z = L.take(nz * 4);
   const size_t o_res = L.take(sizeof(K));
   const size_t o_buf = L.take(cap * sizeof(K));
@@ -367,10 +430,6 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
   char* b = (char*)base;
   SelState* st = (SelState*)(b + o_st);
   uint32_t* z = (uint32_t*)(b + o_z);
-  uint32_t* ghistD = z;
-  uint32_t* ghist = z + D * 256;
-  uint32_t* done_ctr = ghist + 256;
-  uint32_t* buf_ctr = done_ctr + 1;
   K* dres_tmp = (K*)(b + o_res);
   K* buf = (K*)(b + o_buf);
   const K* din = keys;
@@ -378,22 +437,11 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
     PA_HIP(hipMemcpyAsync(b + o_in, keys, n * sizeof(K), hipMemcpyHostToDevice, s));
     din = (const K*)(b + o_in);
   }
-  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
-                     (int32_t)(8 * (D - 1)), z, (uint32_t)nz);
-  const size_t nvec = n / (16 / sizeof(K));
-  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + kSelBlock) / kSelBlock, 1024));
-  {
-    ProfScope ps("k_select_first", s);
-    hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, st, ghistD, done_ctr);
-  }
-  for (int pass = 0; pass < D - 1; ++pass) {
-    ProfScope ps("k_select_pass", s);
-    hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, st, ghist, done_ctr, buf_ctr);
-  }
+  launch_select<K>(din, n, k, st, z, (uint32_t)nz, buf, cap, s);
   if (place && dk) hipLaunchKernelGGL(k_select_place<K>, dim3(1), dim3(1), 0, s, st, place);
-  hipLaunchKernelGGL(k_select_result<K>, dim3(1), dim3(1), 0, s, st, dres_tmp);
+  hipLaunchKernelGGL(k_select_result<K>, dim3(1), dim3(1), 0, s, st, dres ? result : dres_tmp);
   PA_HIP(hipGetLastError());
-  PA_HIP(hipMemcpyAsync(result, dres_tmp, sizeof(K), dres ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  if (!dres) PA_HIP(hipMemcpyAsync(result, dres_tmp, sizeof(K), hipMemcpyDeviceToHost, s));
   if (!dres || !dk) PA_HIP(hipStreamSynchronize(s));
   if (place && !dk) {  // host keys: store the value into the caller's array
     if (dres)
