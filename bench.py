@@ -193,6 +193,10 @@ def bench_primitives(pa, torch, n=100_000_000, seed=1241, reps=5):
         out[name] = {"ms": round(best, 4), "gkeys_per_s": round(units / (best / 1e3) / 1e9, 2)}
 
     u32 = torch.from_numpy(pa.gen_u32(seed, n).view(np.int32)).to(dev)
+    for m in (10_000, 1_000_000):  # small sorts: launch latency, not bandwidth
+        sm = u32[:m].clone()
+        wm = torch.empty_like(sm)
+        timed(f"sort_u32_{m}", lambda: pa.sort_u32(wm), fresh=lambda: wm.copy_(sm), units=m)
     w32 = torch.empty_like(u32)
     t32 = torch.empty_like(u32)
     r32 = torch.zeros(1, dtype=torch.int32, device=dev)

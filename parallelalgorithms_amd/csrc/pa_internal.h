@@ -101,6 +101,10 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 size_t select_scratch_bytes_u32(size_t n);
 int select_enqueue_u32(const uint32_t* din, size_t n, size_t k, char* scratch, uint32_t* value_out, hipStream_t s);
 int launch_hist_digits_u32(const uint32_t* keys, size_t n, uint32_t* counts, hipStream_t s);
+// Plain sort of one device array of at most small_sort_max_keys() u32 keys in
+// one workgroup's LDS, one launch (pa_segsort.hip).
+int small_sort_enqueue_u32(uint32_t* keys, size_t n, hipStream_t s);
+size_t small_sort_max_keys();
 // Merge of two sorted device runs into a device output (pa_merge.hip), with
 // merge_split_bytes(na + nb) bytes of caller scratch for the split table.
 size_t merge_split_bytes(size_t n);

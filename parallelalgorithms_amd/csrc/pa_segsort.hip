@@ -50,8 +50,7 @@ constexpr uint32_t kSegBuckets = 1024;
 // Whole-segment LDS sort.  Padding slots hold 0xFFFFFFFF: a stable sort
 // keeps them after every real key, so the first len slots are the answer.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_seg_small(const uint64_t* __restrict__ ptr,
-                                                     const uint32_t* __restrict__ len) {
+__device__ __forceinline__ void lds_sort_segment(uint32_t* __restrict__ seg, uint32_t n) {
   constexpr int WAVES = BLOCK / 64;
   constexpr int WAVE_SPAN = 64 * kSmallItems;
   constexpr int TILE = BLOCK * kSmallItems;
@@ -61,8 +60,6 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(const uint64_t* __restrict_
   __shared__ uint64_t s_match[WAVES * 256];
   __shared__ uint32_t s_wsum[4];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint32_t* seg = reinterpret_cast<uint32_t*>(ptr[blockIdx.x]);
-  const uint32_t n = len[blockIdx.x];
   uint32_t key[kSmallItems];
   const uint32_t wbase = wave * WAVE_SPAN + lane;
 #pragma unroll
@@ -123,6 +120,19 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(const uint64_t* __restrict_
     __syncthreads();
   }
   for (uint32_t j = tid; j < n; j += BLOCK) seg[j] = s_keys[j];
+}
+
+// One segment per workgroup, from the plan's (address, length) lists.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_small(const uint64_t* __restrict__ ptr,
+                                                     const uint32_t* __restrict__ len) {
+  lds_sort_segment<BLOCK>(reinterpret_cast<uint32_t*>(ptr[blockIdx.x]), len[blockIdx.x]);
+}
+
+// A single small array (plain sort of up to kLdsMax keys): one launch.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sort_small(uint32_t* __restrict__ keys, uint32_t n) {
+  lds_sort_segment<BLOCK>(keys, n);
 }
 
 __global__ __launch_bounds__(256) void k_seg_prepare(uint32_t* __restrict__ counts, size_t ncounts,
@@ -345,6 +355,22 @@ static int build_ptr_plan(uint32_t* const* keys, const uint64_t* lens, size_t nu
   P->finalize();
   return PA_OK;
 }
+
+// Plain sort of one device array of n <= kLdsMax u32 keys in one launch: the
+// smallest workgroup whose LDS tile holds the array.
+int small_sort_enqueue_u32(uint32_t* keys, size_t n, hipStream_t s) {
+  if (n <= 1) return PA_OK;
+  ProfScope ps("k_sort_small", s);
+  if (n <= kLdsLimit[0])
+    hipLaunchKernelGGL(k_sort_small<256>, dim3(1), dim3(256), 0, s, keys, (uint32_t)n);
+  else if (n <= kLdsLimit[1])
+    hipLaunchKernelGGL(k_sort_small<512>, dim3(1), dim3(512), 0, s, keys, (uint32_t)n);
+  else
+    hipLaunchKernelGGL(k_sort_small<1024>, dim3(1), dim3(1024), 0, s, keys, (uint32_t)n);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+size_t small_sort_max_keys() { return kLdsMax; }
 
 }  // namespace pa
 

@@ -367,8 +367,7 @@ struct Variants<uint32_t, false> {
       PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 32),  PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 1, true, 2, 32),
       PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2, 32), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2, 32),
       PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2, 16), PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 16),
-      PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 16),
-  };
+      PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 16)};
   static constexpr int by_class[kClasses] = {2, 2, 6, 6};
 };
 template <>
@@ -383,7 +382,7 @@ struct Variants<uint64_t, false> {
                                           PA_VARIANT(uint64_t, false, 512, 16, 8, true, 6, 0, true, 2, 32),
                                           PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true, 1, 32),
                                           PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true, 1, 32)};
-  static constexpr int by_class[kClasses] = {0, 0, 0, 0};
+  static constexpr int by_class[kClasses] = {0, 0, 0, 2};  // 512 x 12 above 2^24 keys (100M: 4.77 -> 4.13 ms)
 };
 template <>
 struct Variants<uint64_t, true> {
@@ -653,7 +652,10 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   if (!dk) PA_HIP(hipMemcpyAsync(dkeys, keys, n * sizeof(K), hipMemcpyHostToDevice, s));
   if (KV && !dv) PA_HIP(hipMemcpyAsync(dvals, vals, n * 4, hipMemcpyHostToDevice, s));
   if constexpr (!KV) {
-    if (large) {
+    if (std::is_same<K, uint32_t>::value && xform == dev::kXformNone && n <= small_sort_max_keys()) {
+      // One workgroup sorts the whole array in LDS: one launch.
+      PA_TRY(small_sort_enqueue_u32(reinterpret_cast<uint32_t*>(dkeys), n, s));
+    } else if (large) {
       PA_TRY(enqueue_large_sort<K>(dkeys, dktmp, n, b + o_sort, s, xform));
     } else {
       PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
