@@ -35,7 +35,7 @@ constexpr uint32_t kLdsLimit[kLdsClasses] = {4096, 8192, 16384};
 constexpr uint32_t kLdsMax = kLdsLimit[kLdsClasses - 1];
 
 // Segmented onesweep tile shape (keys only): the plain sort's default pass,
-// 512 x 48 keys staged in two rounds.
+// 512 x 48 keys staged in two rounds, ranked in 16-lane units.
 constexpr int kSegBlock = 512;
 constexpr int kSegItems = 48;
 constexpr int kSegLbw = 8;
@@ -305,7 +305,7 @@ static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStre
       const uint64_t* tin = (p & 1) ? d_lt : d_li;
       const uint64_t* tout = (p & 1) ? d_li : d_lt;
       ProfScope ps("k_onesweep_seg", s);
-      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, kModeSeg, true, 4, true, kSegSplit>),
+      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, kModeSeg, true, 4, true, kSegSplit, 16>),
                          dim3((unsigned)nt), dim3(kSegBlock), 0, s, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                          (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u, counts + p * 256,
                          lbp + (size_t)p * nt * 256, ctr + p, (uint32_t)nt, (uint32_t)(8 * p), 0u, d_ti, tin, tout, d_ll,
