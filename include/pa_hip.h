@@ -299,6 +299,10 @@ int pa_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, pa_stream_t stream
 /* Per-kernel timing with HIP events recorded on the launch stream around each
  * kernel the library launches.  enable=1 starts recording, 0 stops. */
 int pa_profile_enable(int enable);
+/* Restricts kernel timing to kernels whose name starts with `prefix` (NULL or
+ * "" = every kernel), so that a benchmark times its dominant kernel without
+ * event records around every other launch. */
+int pa_profile_only(const char* prefix);
 /* Totals for kernels whose name starts with `prefix` since the last reset;
  * waits for the recorded events. */
 int pa_profile_query(const char* prefix, double* total_ms, uint64_t* launches);

@@ -74,6 +74,7 @@ _SIGS = {
     "pa_server_get_stats": (_i, [_vp, _vp]),
     "pa_server_destroy": (None, [_vp]),
     "pa_profile_enable": (_i, [_i]),
+    "pa_profile_only": (_i, [ctypes.c_char_p]),
     "pa_profile_query": (_i, [ctypes.c_char_p, _vp, _vp]),
     "pa_profile_reset": (_i, []),
     "pa_gen_mt19937_u32": (None, [_u64, _vp, _sz]),
@@ -388,6 +389,10 @@ def synchronize(stream=None):
 
 def profile_enable(on=True):
     call("pa_profile_enable", 1 if on else 0)
+
+
+def profile_only(prefix=None):
+    call("pa_profile_only", prefix.encode() if prefix else None)
 
 
 def profile_reset():

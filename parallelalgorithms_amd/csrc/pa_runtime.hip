@@ -136,17 +136,43 @@ struct ProfRec {
 static std::atomic<bool> g_prof_on{false};
 static std::mutex g_prof_mu;
 static std::vector<ProfRec> g_prof;
+static std::vector<hipEvent_t> g_prof_pool;  // events of reset records, reused
+static std::string g_prof_only;               // time only names with this prefix ("" = all)
+
+// An event from the pool, or a new one; nullptr on failure.
+static hipEvent_t prof_event() {
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof_pool.empty()) {
+      hipEvent_t e = g_prof_pool.back();
+      g_prof_pool.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
 
 ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
   if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof_only.empty() && std::strncmp(n, g_prof_only.c_str(), g_prof_only.size()) != 0) return;
+  }
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
     (void)hipGetLastError();  // launches being captured into a graph are not timed
     return;
   }
-  if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
-    (void)hipGetLastError();
-    start = stop = nullptr;
+  start = prof_event();
+  stop = start ? prof_event() : nullptr;
+  if (!stop) {
+    if (start) (void)hipEventDestroy(start);
+    start = nullptr;
     return;
   }
   (void)hipEventRecord(start, stream);
@@ -247,6 +273,12 @@ int pa_profile_enable(int enable) {
   return PA_OK;
 }
 
+int pa_profile_only(const char* prefix) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_only = prefix ? prefix : "";
+  return PA_OK;
+}
+
 int pa_profile_query(const char* prefix, double* total_ms, uint64_t* launches) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   double tot = 0.0;
@@ -269,8 +301,8 @@ int pa_profile_reset(void) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   for (auto& r : g_prof) {
     (void)hipEventSynchronize(r.stop);
-    (void)hipEventDestroy(r.start);
-    (void)hipEventDestroy(r.stop);
+    g_prof_pool.push_back(r.start);
+    g_prof_pool.push_back(r.stop);
   }
   g_prof.clear();
   return PA_OK;
