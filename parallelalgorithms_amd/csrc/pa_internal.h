@@ -65,6 +65,14 @@ int ensure_pinned(StreamCtx& ctx, size_t bytes, void** out);
 int acquire_pinned(StreamCtx& ctx, size_t bytes, void** out, int* slot);
 int release_pinned(StreamCtx& ctx, int slot);
 
+// Copies between pageable host memory and device memory on ctx's stream
+// through the context's pinned ring: chunks are copied into pinned slots by
+// several host threads while DMA moves the previous chunk.  to_device returns
+// once every chunk is enqueued (the caller's buffer may be reused); to_host
+// returns with the data in place.  ctx.mu must be held.
+int stage_to_device(StreamCtx& ctx, void* dst, const void* src, size_t bytes);
+int stage_to_host(StreamCtx& ctx, void* dst, const void* src, size_t bytes);
+
 // True when p is device-accessible memory (hipMalloc / managed / pinned).
 bool is_device_ptr(const void* p);
 

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <map>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "pa_internal.h"
@@ -111,6 +112,99 @@ int acquire_pinned(StreamCtx& ctx, size_t bytes, void** out, int* slot) {
 
 int release_pinned(StreamCtx& ctx, int slot) {
   PA_HIP(hipEventRecord(ctx.ring[slot].done, ctx.stream));
+  return PA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined staging of pageable host memory
+// ---------------------------------------------------------------------------
+constexpr size_t kStageChunk = size_t(64) << 20;   // bytes per pinned slot use
+constexpr size_t kStageDirect = size_t(16) << 20;  // below this, one plain copy
+constexpr int kStageThreads = 8;
+
+// memcpy split over host threads (large pieces only).
+static void par_memcpy(void* dst, const void* src, size_t bytes) {
+  if (bytes < (size_t(4) << 20)) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t per = ((bytes + kStageThreads - 1) / kStageThreads + 63) & ~size_t(63);
+  std::vector<std::thread> th;
+  th.reserve(kStageThreads);
+  for (int t = 0; t < kStageThreads; ++t) {
+    const size_t lo = std::min(bytes, t * per), hi = std::min(bytes, lo + per);
+    if (hi > lo)
+      th.emplace_back([=] { std::memcpy((char*)dst + lo, (const char*)src + lo, hi - lo); });
+  }
+  for (auto& x : th) x.join();
+}
+
+// Slot k of the ring with at least `bytes` of pinned memory and its event,
+// after its previous copy has finished.
+static int ring_slot(StreamCtx& ctx, int k, size_t bytes, void** out) {
+  PinnedSlot& r = ctx.ring[k];
+  if (r.done) PA_HIP(hipEventSynchronize(r.done));
+  else PA_HIP(hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
+  if (r.bytes < bytes) {
+    if (r.ptr) PA_HIP(hipHostFree(r.ptr));
+    r.ptr = nullptr;
+    r.bytes = 0;
+    PA_HIP(hipHostMalloc(&r.ptr, align_up(bytes, 4096), hipHostMallocDefault));
+    r.bytes = align_up(bytes, 4096);
+  }
+  *out = r.ptr;
+  return PA_OK;
+}
+
+int stage_to_device(StreamCtx& ctx, void* dst, const void* src, size_t bytes) {
+  hipStream_t s = ctx.stream;
+  if (bytes < kStageDirect) {
+    PA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return PA_OK;
+  }
+  const int nslots = (int)(sizeof(ctx.ring) / sizeof(ctx.ring[0]));
+  int k = 0;
+  for (size_t off = 0; off < bytes; off += kStageChunk, k = (k + 1) % nslots) {
+    const size_t len = std::min(kStageChunk, bytes - off);
+    void* p;
+    PA_TRY(ring_slot(ctx, k, kStageChunk, &p));  // waits for this slot's last DMA
+    par_memcpy(p, (const char*)src + off, len);
+    PA_HIP(hipMemcpyAsync((char*)dst + off, p, len, hipMemcpyHostToDevice, s));
+    PA_HIP(hipEventRecord(ctx.ring[k].done, s));
+  }
+  return PA_OK;
+}
+
+int stage_to_host(StreamCtx& ctx, void* dst, const void* src, size_t bytes) {
+  hipStream_t s = ctx.stream;
+  if (bytes < kStageDirect) {
+    PA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    PA_HIP(hipStreamSynchronize(s));
+    return PA_OK;
+  }
+  const int nslots = (int)(sizeof(ctx.ring) / sizeof(ctx.ring[0]));
+  const size_t nchunks = (bytes + kStageChunk - 1) / kStageChunk;
+  void* p[sizeof(ctx.ring) / sizeof(ctx.ring[0])];
+  // Chunk i uses slot i % nslots; up to nslots DMA copies run ahead of the
+  // host copies out of the pinned slots.
+  auto issue = [&](size_t i) -> int {
+    const int k = (int)(i % nslots);
+    const size_t off = i * kStageChunk, len = std::min(kStageChunk, bytes - off);
+    PA_HIP(hipMemcpyAsync(p[k], (const char*)src + off, len, hipMemcpyDeviceToHost, s));
+    PA_HIP(hipEventRecord(ctx.ring[k].done, s));
+    return PA_OK;
+  };
+  for (size_t i = 0; i < nchunks && i < (size_t)nslots; ++i) {
+    PA_TRY(ring_slot(ctx, (int)i, kStageChunk, &p[i]));
+    PA_TRY(issue(i));
+  }
+  for (size_t i = 0; i < nchunks; ++i) {
+    const int k = (int)(i % nslots);
+    const size_t off = i * kStageChunk, len = std::min(kStageChunk, bytes - off);
+    PA_HIP(hipEventSynchronize(ctx.ring[k].done));
+    par_memcpy((char*)dst + off, p[k], len);
+    if (i + nslots < nchunks) PA_TRY(issue(i + nslots));
+  }
   return PA_OK;
 }
 

@@ -649,8 +649,8 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   K* dktmp = dkt ? ktmp : reinterpret_cast<K*>(b + o_kt);
   uint32_t* dvals = KV ? (dv ? vals : reinterpret_cast<uint32_t*>(b + o_v)) : nullptr;
   uint32_t* dvtmp = KV ? (dvt ? vtmp : reinterpret_cast<uint32_t*>(b + o_vt)) : nullptr;
-  if (!dk) PA_HIP(hipMemcpyAsync(dkeys, keys, n * sizeof(K), hipMemcpyHostToDevice, s));
-  if (KV && !dv) PA_HIP(hipMemcpyAsync(dvals, vals, n * 4, hipMemcpyHostToDevice, s));
+  if (!dk) PA_TRY(stage_to_device(ctx, dkeys, keys, n * sizeof(K)));
+  if (KV && !dv) PA_TRY(stage_to_device(ctx, dvals, vals, n * 4));
   if constexpr (!KV) {
     if (std::is_same<K, uint32_t>::value && xform == dev::kXformNone && n <= small_sort_max_keys()) {
       // One workgroup sorts the whole array in LDS: one launch.
@@ -663,9 +663,8 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   } else {
     PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
   }
-  if (!dk) PA_HIP(hipMemcpyAsync(keys, dkeys, n * sizeof(K), hipMemcpyDeviceToHost, s));
-  if (KV && !dv) PA_HIP(hipMemcpyAsync(vals, dvals, n * 4, hipMemcpyDeviceToHost, s));
-  if (!dk || (KV && !dv)) PA_HIP(hipStreamSynchronize(s));
+  if (!dk) PA_TRY(stage_to_host(ctx, keys, dkeys, n * sizeof(K)));
+  if (KV && !dv) PA_TRY(stage_to_host(ctx, vals, dvals, n * 4));
   return PA_OK;
 }
 

@@ -50,6 +50,24 @@ def test_sort_u32_tmp_null_and_host_pointers(pa):
     np.testing.assert_array_equal(h, expect)
 
 
+@pytest.mark.parametrize("n", [(4 << 20) + 1, (16 << 20) + 5, 5 * (16 << 20) + 3])
+def test_sort_host_memory_staging(pa, n):
+    """Host arrays above 16 MB go through the pipelined pinned staging: one
+    chunk, two chunks, and more chunks than pinned slots (slot reuse both ways)."""
+    import parallelalgorithms_amd as P
+    a = P.gen_u32(n, n)
+    h = a.copy()
+    pa.sort_u32(h)
+    np.testing.assert_array_equal(h, np.sort(a))
+    if n < (64 << 20):
+        k = P.gen_u64(n + 1, n)
+        v = np.arange(n, dtype=np.uint32)
+        pa.sort_kv_u64_u32(k, v)  # keys and payload both staged
+        order = np.argsort(P.gen_u64(n + 1, n), kind="stable")
+        np.testing.assert_array_equal(v, order.astype(np.uint32))
+        np.testing.assert_array_equal(k, np.sort(P.gen_u64(n + 1, n)))
+
+
 def test_two_phase_result_in_both(pa):
     n = 123_457
     a = make_u32("uniform", n, seed=11)
