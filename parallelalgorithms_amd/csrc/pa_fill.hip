@@ -71,8 +71,7 @@ int pa_fill(void* a, size_t elem_bytes, const void* value, size_t n, pa_stream_t
   }
   PA_HIP(hipGetLastError());
   if (!dev) {
-    PA_HIP(hipMemcpyAsync(a, dst, nbytes, hipMemcpyDeviceToHost, s));
-    PA_HIP(hipStreamSynchronize(s));
+    PA_TRY(stage_to_host(ctx, a, dst, nbytes));
   }
   return PA_OK;
 }

@@ -232,7 +232,7 @@ static int stage_in(const void* p, size_t bytes, char* scratch_slot, bool is_dev
     *out = p;
     return PA_OK;
   }
-  PA_HIP(hipMemcpyAsync(scratch_slot, p, bytes, hipMemcpyHostToDevice, s));
+  PA_TRY(stage_to_device(stream_ctx(s), scratch_slot, p, bytes));
   *out = scratch_slot;
   return PA_OK;
 }
@@ -366,7 +366,7 @@ int pa_counting_sort_u8(uint8_t* bytes, size_t n, pa_stream_t stream) {
   // A host array is staged with the same 16-byte phase, so the device copy is
   // laid out like the caller's.
   uint8_t* d = dk ? bytes : (uint8_t*)(b + o_in + ((uintptr_t)bytes & 15));
-  if (!dk) PA_HIP(hipMemcpyAsync(d, bytes, n, hipMemcpyHostToDevice, s));
+  if (!dk) PA_TRY(stage_to_device(ctx, d, bytes, n));
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, c32, (size_t)256);
   {
     ProfScope ps("k_hist_u8", s);
@@ -377,10 +377,7 @@ int pa_counting_sort_u8(uint8_t* bytes, size_t n, pa_stream_t stream) {
     hipLaunchKernelGGL(k_counting_fill, dim3(hist_grid(n / 16 + 1)), dim3(kHistBlock), 0, s, d, n, c32);
   }
   PA_HIP(hipGetLastError());
-  if (!dk) {
-    PA_HIP(hipMemcpyAsync(bytes, d, n, hipMemcpyDeviceToHost, s));
-    PA_HIP(hipStreamSynchronize(s));
-  }
+  if (!dk) PA_TRY(stage_to_host(ctx, bytes, d, n));
   return PA_OK;
 }
 
@@ -435,7 +432,7 @@ int pa_hist_quanta_u32(const uint32_t* keys, size_t first, size_t last, size_t q
   uint64_t* c64 = dout ? counts : (uint64_t*)(b + o_c64);
   const void* din = keys;
   if (!dk) {
-    PA_HIP(hipMemcpyAsync(b + o_in + first * 4, keys + first, (last - first) * 4, hipMemcpyHostToDevice, s));
+    PA_TRY(stage_to_device(ctx, b + o_in + first * 4, keys + first, (last - first) * 4));
     din = b + o_in;
   }
   PA_TRY(quanta_counts((const uint32_t*)din, first, last, quantum, nq, which_byte, c32, s));

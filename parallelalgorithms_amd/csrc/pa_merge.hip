@@ -128,11 +128,11 @@ int merge_entry(const K* a, size_t na, const K* b, size_t nb, K* out, hipStream_
   const K* dB = b;
   K* dO = dout ? out : (K*)(p + o_o);
   if (!da) {
-    PA_HIP(hipMemcpyAsync(p + o_a, a, na * sizeof(K), hipMemcpyHostToDevice, s));
+    PA_TRY(stage_to_device(ctx, p + o_a, a, na * sizeof(K)));
     dA = (const K*)(p + o_a);
   }
   if (!db) {
-    PA_HIP(hipMemcpyAsync(p + o_b, b, nb * sizeof(K), hipMemcpyHostToDevice, s));
+    PA_TRY(stage_to_device(ctx, p + o_b, b, nb * sizeof(K)));
     dB = (const K*)(p + o_b);
   }
   {
@@ -146,7 +146,7 @@ int merge_entry(const K* a, size_t na, const K* b, size_t nb, K* out, hipStream_
   }
   PA_HIP(hipGetLastError());
   if (!dout) {
-    PA_HIP(hipMemcpyAsync(out, dO, n * sizeof(K), hipMemcpyDeviceToHost, s));
+    PA_TRY(stage_to_host(ctx, out, dO, n * sizeof(K)));
   }
   if (!da || !db || !dout) PA_HIP(hipStreamSynchronize(s));
   return PA_OK;

@@ -237,7 +237,7 @@ int pa_partition_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_st
   uint32_t* din = keys;
   if (!dk) {
     din = (uint32_t*)(b + o_in);
-    PA_HIP(hipMemcpyAsync(din, keys, n * 4, hipMemcpyHostToDevice, s));
+    PA_TRY(stage_to_device(ctx, din, keys, n * 4));
   }
   PA_TRY(select_enqueue_u32(din, n, k, b + o_sel, pivot, s));
   {
@@ -251,7 +251,10 @@ int pa_partition_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_st
     hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(kPartBlock), 0, s, din, n, pivot, offsets, tmp);
   }
   PA_HIP(hipGetLastError());
-  PA_HIP(hipMemcpyAsync(keys, tmp, n * 4, dk ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  if (dk)
+    PA_HIP(hipMemcpyAsync(keys, tmp, n * 4, hipMemcpyDeviceToDevice, s));
+  else
+    PA_TRY(stage_to_host(ctx, keys, tmp, n * 4));
   PA_HIP(hipMemcpyAsync(result, pivot, 4, dres ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   if (!dk || !dres) PA_HIP(hipStreamSynchronize(s));
   return PA_OK;

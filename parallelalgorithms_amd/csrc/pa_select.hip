@@ -434,7 +434,7 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
   K* buf = (K*)(b + o_buf);
   const K* din = keys;
   if (!dk) {
-    PA_HIP(hipMemcpyAsync(b + o_in, keys, n * sizeof(K), hipMemcpyHostToDevice, s));
+    PA_TRY(stage_to_device(ctx, b + o_in, keys, n * sizeof(K)));
     din = (const K*)(b + o_in);
   }
   launch_select<K>(din, n, k, st, z, (uint32_t)nz, buf, cap, s);
