@@ -227,6 +227,19 @@ def bench_primitives(pa, torch, n=100_000_000, seed=1241, reps=5):
     timed("sort_f64", lambda: pa.sort_f64(w64.view(torch.float64), t64.view(torch.float64)),
           fresh=lambda: w64.view(torch.float64).copy_(f64))
     del u64, w64, t64, f64
+    # Key-value at the configuration's size: 200M u64 keys, u32 payload = index.
+    m = 2 * n
+    k64 = torch.from_numpy(pa.gen_u64(seed + 2, m).view(np.int64)).to(dev)
+    kw = torch.empty_like(k64)
+    v0 = torch.arange(m, dtype=torch.int32, device=dev)
+    vw = torch.empty_like(v0)
+
+    def kv64_fresh():
+        kw.copy_(k64)
+        vw.copy_(v0)
+    timed("sort_kv_u64_u32_200M", lambda: pa.sort_kv_u64_u32(kw, vw), fresh=kv64_fresh, units=m)
+    del k64, kw, v0, vw
+    torch.cuda.empty_cache()
     # The drop-in headers' usual case: keys in ordinary (pageable) host memory,
     # staged to the device and back inside the call.  Wall clock, PCIe included.
     import time
