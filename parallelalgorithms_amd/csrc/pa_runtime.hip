@@ -131,11 +131,16 @@ static void par_memcpy(void* dst, const void* src, size_t bytes) {
   const size_t per = ((bytes + kStageThreads - 1) / kStageThreads + 63) & ~size_t(63);
   std::vector<std::thread> th;
   th.reserve(kStageThreads);
-  for (int t = 0; t < kStageThreads; ++t) {
-    const size_t lo = std::min(bytes, t * per), hi = std::min(bytes, lo + per);
-    if (hi > lo)
-      th.emplace_back([=] { std::memcpy((char*)dst + lo, (const char*)src + lo, hi - lo); });
+  size_t done = 0;  // bytes from 0 handed to threads; the rest is copied here
+  try {
+    for (int t = 0; t < kStageThreads; ++t) {
+      const size_t lo = std::min(bytes, t * per), hi = std::min(bytes, lo + per);
+      if (hi > lo) th.emplace_back([=] { std::memcpy((char*)dst + lo, (const char*)src + lo, hi - lo); });
+      done = hi;
+    }
+  } catch (...) {  // no thread could be started: this thread copies the rest
   }
+  if (done < bytes) std::memcpy((char*)dst + done, (const char*)src + done, bytes - done);
   for (auto& x : th) x.join();
 }
 
