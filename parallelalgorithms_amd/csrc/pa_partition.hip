@@ -12,13 +12,14 @@
 //   radix select of v = the k-th smallest (the select kernels, one read of the
 //   keys plus passes over a shrinking candidate set), then one stable
 //   three-way partition of the keys into (< v | == v | > v):
-//     k_part_count    per block: keys < v and == v over its tiles
+//     k_part_count    per block: keys < v and == v over its tiles; also
+//                     copies the keys into scratch
 //     k_part_offsets  one block: each block's start in each of the 3 groups
-//     k_part_scatter  per block, same tiles in the same order: each key goes
-//                     to its group's next slot (wave ballots give the stable
-//                     order inside a tile), into a scratch copy
-//   and the scratch copy goes back over the keys.  Since count(< v) <= k <
-//   count(<= v), slot k holds v.
+//     k_part_scatter  per block, same tiles in the same order, reading the
+//                     scratch copy: each key goes to its group's next slot
+//                     (wave ballots give the stable order inside a tile),
+//                     written straight over the keys
+//   Since count(< v) <= k < count(<= v), slot k holds v.
 // Several ranks: the keys are sorted (a sorted array satisfies every rank's
 // partition at once).
 #include <algorithm>
@@ -43,10 +44,13 @@ __device__ __host__ inline size_t part_tiles_per_block(size_t n, size_t nb) {
 
 __global__ __launch_bounds__(kPartBlock) void k_part_count(const uint32_t* __restrict__ keys, size_t n,
                                                            const uint32_t* __restrict__ pivot,
-                                                           uint32_t* __restrict__ counts) {
+                                                           uint32_t* __restrict__ counts,
+                                                           uint32_t* __restrict__ copy_out) {
   // Counting needs no order, so the block reads its whole key range with
   // 16-byte loads (four in flight per thread) when the keys are 16-byte
-  // aligned; the range starts on a tile boundary, a multiple of 4 keys.
+  // aligned; the range starts on a tile boundary, a multiple of 4 keys.  The
+  // keys are also written to copy_out, the scatter's source, so the scatter
+  // can write the result over the keys without a copy back.
   __shared__ uint32_t s_c[2];
   const uint32_t tid = threadIdx.x;
   if (tid < 2) s_c[tid] = 0;
@@ -56,8 +60,9 @@ __global__ __launch_bounds__(kPartBlock) void k_part_count(const uint32_t* __res
   const size_t span = part_tiles_per_block(n, gridDim.x) * kPartTile;
   const size_t lo = std::min<size_t>(n, (size_t)blockIdx.x * span), hi = std::min<size_t>(n, lo + span);
   size_t scalar_from = lo;
-  if (((uintptr_t)keys & 15) == 0) {
+  if ((((uintptr_t)keys | (uintptr_t)copy_out) & 15) == 0) {
     const uint4* vk = reinterpret_cast<const uint4*>(keys + lo);
+    uint4* vo = reinterpret_cast<uint4*>(copy_out + lo);
     const size_t nv = (hi - lo) / 4;
     constexpr int U = 4;
     for (size_t i0 = tid; i0 < nv; i0 += (size_t)kPartBlock * U) {
@@ -68,6 +73,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_count(const uint32_t* __res
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (i0 + u * kPartBlock >= nv) continue;
+        vo[i0 + u * kPartBlock] = x[u];
         const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -80,6 +86,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_count(const uint32_t* __res
   }
   for (size_t i = scalar_from + tid; i < hi; i += kPartBlock) {
     const uint32_t k = keys[i];
+    copy_out[i] = k;
     lt += k < v;
     eq += k == v;
   }
@@ -242,19 +249,16 @@ int pa_partition_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_st
   PA_TRY(select_enqueue_u32(din, n, k, b + o_sel, pivot, s));
   {
     ProfScope ps("k_part_count", s);
-    hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(kPartBlock), 0, s, din, n, pivot, counts);
+    hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(kPartBlock), 0, s, din, n, pivot, counts, tmp);
   }
   hipLaunchKernelGGL(k_part_block_items, dim3((nb + 255) / 256), dim3(256), 0, s, nb, n, items);
   hipLaunchKernelGGL(k_part_offsets, dim3(1), dim3(1024), 0, s, counts, nb, n, items, offsets);
   {
     ProfScope ps("k_part_scatter", s);
-    hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(kPartBlock), 0, s, din, n, pivot, offsets, tmp);
+    hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(kPartBlock), 0, s, tmp, n, pivot, offsets, din);
   }
   PA_HIP(hipGetLastError());
-  if (dk)
-    PA_HIP(hipMemcpyAsync(keys, tmp, n * 4, hipMemcpyDeviceToDevice, s));
-  else
-    PA_TRY(stage_to_host(ctx, keys, tmp, n * 4));
+  if (!dk) PA_TRY(stage_to_host(ctx, keys, din, n * 4));
   PA_HIP(hipMemcpyAsync(result, pivot, 4, dres ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   if (!dk || !dres) PA_HIP(hipStreamSynchronize(s));
   return PA_OK;
