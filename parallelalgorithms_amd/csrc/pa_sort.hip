@@ -379,12 +379,11 @@ struct Variants<uint32_t, true> {
 template <>
 struct Variants<uint64_t, false> {
   static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 0, true, 1, 32),
-                                          PA_VARIANT(uint64_t, false, 512, 16, 8, true, 6, 0, true, 2, 32),
                                           PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true, 1, 32),
                                           PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true, 1, 32),
                                           PA_VARIANT(uint64_t, false, 512, 16, 8, true, 4, 0, true, 1, 16),
                                           PA_VARIANT(uint64_t, false, 512, 24, 8, true, 4, 0, true, 2, 16)};
-  static constexpr int by_class[kClasses] = {0, 0, 0, 5};  // 512 x 24 split, 16-lane units above 2^24 keys (100M: 4.77 -> 3.57 ms)
+  static constexpr int by_class[kClasses] = {0, 0, 0, 4};  // 512 x 24 split, 16-lane units above 2^24 keys (100M: 4.77 -> 3.57 ms)
 };
 template <>
 struct Variants<uint64_t, true> {

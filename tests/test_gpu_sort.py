@@ -182,6 +182,33 @@ def test_sort_signed(pa, orc, n):
     np.testing.assert_array_equal(h, np.sort(a64))
 
 
+def test_profile_filter_times_only_the_named_kernels(pa):
+    """pa_profile_only restricts event timing to kernels whose name starts with
+    the prefix: one 4-pass sort records four k_onesweep launches and nothing
+    else; without a filter the histogram step is timed too."""
+    import parallelalgorithms_amd as P
+    n = 3_000_017
+    d = to_dev(make_u32("uniform", n, seed=5))
+    P.profile_reset()
+    P.profile_only("k_onesweep")
+    P.profile_enable(True)
+    try:
+        pa.sort_u32(d)
+    finally:
+        P.profile_enable(False)
+        P.profile_only(None)
+    assert P.profile_query("k_onesweep")[1] == 4
+    assert P.profile_query("k_hist")[1] == 0
+    P.profile_reset()
+    P.profile_enable(True)
+    try:
+        pa.sort_u32(d)
+    finally:
+        P.profile_enable(False)
+    assert P.profile_query("k_onesweep")[1] == 4 and P.profile_query("k_hist")[1] >= 1
+    P.profile_reset()
+
+
 def test_autotune_then_sort(pa):
     """After the tuner records a pass shape per size class, sorts in every
     class still match np.sort; the choice is a valid candidate index."""
