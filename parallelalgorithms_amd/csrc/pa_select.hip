@@ -10,20 +10,24 @@
 //
 // GPU form (the caller's array is never modified):
 //   k_select_first  one read of all keys builds the D digit histograms.
-//   k_select_walk   (one block) walks the digits from the top: while the
-//                   chosen bin still holds every candidate (constant input,
-//                   shared high bytes) the next digit's histogram over the same
-//                   set is already known, so no further read is needed.
-//   k_select_pass   (each remaining digit; returns at once when the answer is
-//                   known) histograms the next digit over the keys matching the
-//                   fixed high digits.  When the candidates fit the buffer
-//                   (half the input) they are compacted in the same read, so
-//                   later passes read only the candidates.
-//   k_select_pick   (one block) picks that digit's bin.
-// Blocks add their LDS counts to global histograms with atomics and the
-// one-block kernels read them after the launch boundary: no grid-wide fence
-// or last-block election.  Control state lives in device memory; the whole
-// select is enqueued without a host round trip.
+//   k_select_pass   (launch i of D - 1; returns at once when the answer is
+//                   known) first derives its state from the counts the
+//                   previous launch left: launch 0 walks the digits from the
+//                   top (while the chosen bin still holds every candidate --
+//                   constant input, shared high bytes -- the next digit's
+//                   histogram over the same set is already known, so no read
+//                   is needed for it), launch i > 0 picks the bin of the
+//                   previous pass's digit.  Every block computes the same
+//                   state from the same counts.  Then it histograms the next
+//                   digit over the keys matching the fixed high digits; when
+//                   the candidates fit the buffer (half the input) they are
+//                   compacted in the same read, so later passes read only the
+//                   candidates.
+//   k_select_final  (one block) picks the last digit and writes the value.
+// Blocks add their LDS counts to global histograms (one per launch) with
+// atomics and the next launch reads them after the launch boundary: no
+// grid-wide fence or last-block election.  Control state lives in device
+// memory; the whole select is enqueued without a host round trip.
 #include <algorithm>
 
 #include "pa_internal.h"
@@ -31,7 +35,6 @@
 namespace pa {
 
 constexpr int kSelBlock = 512;
-constexpr uint32_t kStage = 4096;  // LDS staging for compaction (keys)
 
 struct SelState {
   uint64_t k;          // rank within the current candidate set
@@ -138,76 +141,127 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   flush_counts(s_hist, ghist, D * 256);
 }
 
-// After the first read (one block): walk the digits from the top while the
-// chosen bin still holds every candidate (constant input, shared high bytes),
-// so no further read is needed for those digits.  Clears the histograms.
-template <typename K>
-__global__ __launch_bounds__(256) void k_select_walk(SelState* __restrict__ st, uint32_t* __restrict__ ghist) {
-  constexpr int D = sizeof(K);
-  __shared__ uint32_t s_hist[D * 256];
-  __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_pick[1];
-  __shared__ uint64_t s_below;
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < D * 256; i += 256) {
-    s_hist[i] = ghist[i];
-    ghist[i] = 0;
-  }
-  __syncthreads();
-  uint64_t k = st->k, cand = st->cand;
-  uint64_t prefix = 0, pmask = 0;
+// The state a pass works with, computed by every block of that pass from the
+// histogram the previous step left (every block reads the same counts and
+// picks the same bin, so no separate pick launch is needed):
+//   step 0: walk the first read's digit histograms from the top while the
+//           chosen bin still holds every candidate (constant input, shared
+//           high bytes), so no further read is needed for those digits;
+//   step i: pick the bin of the previous pass's digit.
+// Block 0 stores the result in S[i + 1] for the next step.  Every thread of
+// the block must call these (they synchronise the block).
+struct SelShared {
+  uint32_t wsum[4];
+  uint32_t pick[1];
+  uint64_t below;
+};
+
+__device__ void walk_state(const uint32_t* s_hist, int D, SelState& t, SelShared& sh) {
+  uint64_t k = t.k, cand = t.cand, prefix = 0, pmask = 0;
   int shift = 8 * (D - 1);
   while (shift >= 0) {
     const int d = shift / 8;
-    pick_bin(s_hist + d * 256, k, s_wsum, s_pick, &s_below);
-    const uint32_t b = s_pick[0];
+    pick_bin(s_hist + d * 256, k, sh.wsum, sh.pick, &sh.below);
+    const uint32_t b = sh.pick[0];
     const uint64_t c = s_hist[d * 256 + b];
     prefix |= (uint64_t)b << shift;
     pmask |= (uint64_t)0xFF << shift;
-    k -= s_below;
+    k -= sh.below;
     shift -= 8;
+    __syncthreads();  // sh is reused by the next digit
     if (c < cand) {
       cand = c;
       break;
     }
   }
-  if (tid == 0) {
-    st->k = k;
-    st->cand = cand;
-    st->prefix = prefix;
-    st->pmask = pmask;
-    st->shift = shift;
-    st->done = shift < 0;
-    st->result = prefix;
-    st->src = 0;
-    st->compact_now = cand <= st->cap;
-  }
+  t.k = k;
+  t.cand = cand;
+  t.prefix = prefix;
+  t.pmask = pmask;
+  t.shift = shift;
+  t.done = shift < 0;
+  t.result = prefix;
+  t.src = 0;
+  t.compact_now = cand <= t.cap;
 }
 
-// One filtered histogram pass at st->shift over the keys matching the fixed
-// digits; when st->compact_now is set, the matching keys are also appended to
-// buf.  Each wave stages its matches in its own LDS slice and flushes the
+// prev: the state the previous pass used; hist: that pass's histogram of
+// digit prev.shift; compacted: that pass compacted the candidates, buf_count
+// of them.
+__device__ void pick_state(const uint32_t* s_row, SelState& t, bool compacted, uint64_t buf_count, SelShared& sh) {
+  const uint32_t shift = (uint32_t)t.shift;
+  pick_bin(s_row, t.k, sh.wsum, sh.pick, &sh.below);
+  const uint32_t b = sh.pick[0];
+  t.k -= sh.below;
+  t.cand = s_row[b];
+  t.prefix |= (uint64_t)b << shift;
+  t.pmask |= (uint64_t)0xFF << shift;
+  t.shift = (int32_t)shift - 8;
+  t.done = shift == 0;
+  t.result = t.prefix;
+  if (compacted) {
+    t.src = 1;
+    t.buf_count = buf_count;
+  }
+  t.compact_now = !t.src && t.cand <= t.cap;
+  __syncthreads();  // sh is reused by the caller
+}
+
+// The state of step `step` (see above), from S[step] and the histogram the
+// previous step left; block 0 stores it in S[step + 1].
+template <typename K>
+__device__ SelState step_state(int step, const SelState* __restrict__ S, const uint32_t* __restrict__ ghist_first,
+                               const uint32_t* __restrict__ ghist_prev,
+                               const unsigned long long* __restrict__ ctr_prev, SelState* __restrict__ S_next,
+                               uint32_t* s_hist, SelShared& sh) {
+  constexpr int D = sizeof(K);
+  SelState t = S[step];
+  if (step == 0) {
+    for (uint32_t i = threadIdx.x; i < D * 256; i += blockDim.x) s_hist[i] = ghist_first[i];
+    __syncthreads();
+    walk_state(s_hist, D, t, sh);
+  } else if (!t.done) {
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_hist[i] = ghist_prev[i];
+    __syncthreads();
+    const bool compacted = !t.src && t.compact_now;
+    pick_state(s_hist, t, compacted, compacted ? *ctr_prev : 0, sh);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *S_next = t;
+  return t;
+}
+
+// One filtered histogram pass (launch `step`) over the keys matching the
+// fixed digits; when the state says so, the matching keys are also appended
+// to buf.  Each wave stages its matches in its own LDS slice and flushes the
 // slice with one global atomic when it fills and at the end, so no block-wide
 // barrier sits inside the loop; one wave scan places a step's matches.
 template <typename K>
 __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__ keys, size_t n,
-                                                           K* __restrict__ buf, const SelState* __restrict__ st,
+                                                           K* __restrict__ buf, int step, SelState* __restrict__ S,
+                                                           const uint32_t* __restrict__ ghist_first,
                                                            uint32_t* __restrict__ ghist,
-                                                           unsigned long long* __restrict__ buf_ctr) {
+                                                           unsigned long long* __restrict__ ctr) {
+  constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
   constexpr int U = 4;
   constexpr int WAVES = kSelBlock / 64;
   constexpr uint32_t STAGE = 64 * U * PER;  // one step's worst case per wave
-  __shared__ uint32_t s_hist[256];
+  __shared__ uint32_t s_hist[D * 256];
   __shared__ K s_stage[WAVES][STAGE];
-  if (st->done) return;  // uniform for the whole grid
+  __shared__ SelShared sh;
+  const SelState t = step_state<K>(step, S, ghist_first, step ? ghist + (step - 1) * 256 : nullptr,
+                                   step ? ctr + (step - 1) : nullptr, S + step + 1, s_hist, sh);
+  if (t.done) return;  // uniform for the whole grid
+  uint32_t* const gh = ghist + step * 256;
+  unsigned long long* const buf_ctr = ctr + step;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const K prefix = (K)st->prefix, pmask = (K)st->pmask;
-  const uint32_t shift = (uint32_t)st->shift;
-  const bool from_buf = st->src != 0;
-  const bool compact = !from_buf && st->compact_now;
+  const K prefix = (K)t.prefix, pmask = (K)t.pmask;
+  const uint32_t shift = (uint32_t)t.shift;
+  const bool from_buf = t.src != 0;
+  const bool compact = !from_buf && t.compact_now;
   const K* src = from_buf ? buf : keys;
-  const size_t m = from_buf ? (size_t)st->buf_count : n;
+  const size_t m = from_buf ? (size_t)t.buf_count : n;
+  __syncthreads();  // s_hist held the previous histogram until here
   if (tid < 256) s_hist[tid] = 0;
   __syncthreads();
   K* stage = s_stage[wave];
@@ -298,95 +352,77 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
   }
   if (compact && staged) flush();
   __syncthreads();
-  flush_counts(s_hist, ghist, 256);
+  flush_counts(s_hist, gh, 256);
 }
 
-// After a pass (one block): picks the bin of digit st->shift holding rank k,
-// fixes that digit, and switches later passes to the compacted buffer when
-// this pass compacted.  Clears the histogram and the buffer counter.
+// The selected key from the last step's state (picking from the last
+// pass's histogram when the walk or the passes have not finished), written to
+// dst and, when given, to place (a[k], the reference's post-condition,
+// RadixSelect.h:149).  One block of 256 threads.
 template <typename K>
-__global__ __launch_bounds__(256) void k_select_pick(SelState* __restrict__ st, uint32_t* __restrict__ ghist,
-                                                     unsigned long long* __restrict__ buf_ctr) {
+__global__ __launch_bounds__(256) void k_select_final(const SelState* __restrict__ S_last,
+                                                      const uint32_t* __restrict__ ghist_last,
+                                                      const unsigned long long* __restrict__ ctr_last, K* dst,
+                                                      K* place) {
   __shared__ uint32_t s_hist[256];
-  __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_pick[1];
-  __shared__ uint64_t s_below;
-  if (st->done) return;
-  const uint32_t tid = threadIdx.x;
-  s_hist[tid] = ghist[tid];
-  ghist[tid] = 0;
-  __syncthreads();
-  const uint64_t k = st->k;
-  const uint32_t shift = (uint32_t)st->shift;
-  pick_bin(s_hist, k, s_wsum, s_pick, &s_below);
-  if (tid == 0) {
-    const uint32_t b = s_pick[0];
-    const bool compacted = !st->src && st->compact_now;
-    st->k = k - s_below;
-    st->cand = s_hist[b];
-    st->prefix |= (uint64_t)b << shift;
-    st->pmask |= (uint64_t)0xFF << shift;
-    st->shift = (int32_t)shift - 8;
-    st->done = shift == 0;
-    st->result = st->prefix;
-    if (compacted) {
-      st->src = 1;
-      st->buf_count = *buf_ctr;
-      *buf_ctr = 0;
-    }
-    st->compact_now = !st->src && st->cand <= st->cap;
+  __shared__ SelShared sh;
+  SelState t = *S_last;
+  if (!t.done) {
+    s_hist[threadIdx.x] = ghist_last[threadIdx.x];
+    __syncthreads();
+    const bool compacted = !t.src && t.compact_now;
+    pick_state(s_hist, t, compacted, compacted ? *ctr_last : 0, sh);
+  }
+  if (threadIdx.x == 0) {
+    *dst = (K)t.result;
+    if (place) *place = (K)t.result;
   }
 }
 
-// a[k] = the selected value (the reference's post-condition, RadixSelect.h:149).
+// Scratch: the D per-step states, then the zeroed control words (the D digit
+// histograms of the first read, one histogram per pass, one 64-bit compaction
+// counter per pass), then the compaction buffer.
 template <typename K>
-__global__ void k_select_place(const SelState* st, K* dst) { *dst = (K)st->result; }
-
-template <typename K>
-__global__ void k_select_result(const SelState* st, K* dst) { *dst = (K)st->result; }
-
-// Select on device keys into caller-provided scratch (no allocation, no
-// locking): used by the partition.  The selected key is written to
-// *value_out (device memory).
-// Zeroed control words: D digit histograms, one pass histogram, and the 64-bit
-// compaction counter.
-template <typename K>
-static constexpr size_t select_zero_words() { return sizeof(K) * 256 + 256 + 2; }
+static constexpr size_t select_zero_words() {
+  return sizeof(K) * 256 + (sizeof(K) - 1) * 256 + 2 * (sizeof(K) - 1);
+}
 
 template <typename K>
 static size_t select_bytes(size_t n) {
   Layout L;
-  L.take(sizeof(SelState));
+  L.take(sizeof(K) * sizeof(SelState));
   L.take(select_zero_words<K>() * 4);
   L.take(std::max<size_t>(n / 2, 1) * sizeof(K));
   return L.off;
 }
 
 // The launch sequence of one select, all stream-ordered with no host round
-// trip: the first read with the digit walk, then for each remaining digit a
-// filtered pass and its bin pick (both return at once when the answer is
-// known), then the result.
+// trip: init, the first read, D - 1 filtered passes (each deriving its state
+// from the previous launch's counts, returning at once when the answer is
+// known), then the final pick, which writes the value to dst (device memory)
+// and, when given, to place.
 template <typename K>
-static void launch_select(const K* din, size_t n, size_t k, SelState* st, uint32_t* z, uint32_t nz, K* buf, size_t cap,
-                          hipStream_t s) {
+static void launch_select(const K* din, size_t n, size_t k, SelState* S, uint32_t* z, uint32_t nz, K* buf, size_t cap,
+                          K* dst, K* place, hipStream_t s) {
   constexpr int D = sizeof(K);
-  uint32_t* ghistD = z;
+  uint32_t* ghist_first = z;
   uint32_t* ghist = z + D * 256;
-  unsigned long long* buf_ctr = reinterpret_cast<unsigned long long*>(ghist + 256);
-  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ghist + (D - 1) * 256);
+  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, S, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
                      (int32_t)(8 * (D - 1)), z, nz);
   const size_t nvec = n / (16 / sizeof(K));
   const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + 4 * kSelBlock - 1) / (4 * kSelBlock), 512));
   {
     ProfScope ps("k_select_first", s);
-    hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghistD);
+    hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghist_first);
   }
-  hipLaunchKernelGGL(k_select_walk<K>, dim3(1), dim3(256), 0, s, st, ghistD);
-  for (int pass = 0; pass < D - 1; ++pass) {
+  for (int step = 0; step < D - 1; ++step) {
     ProfScope ps("k_select_pass", s);
-    hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, st, ghist, buf_ctr);
-    hipLaunchKernelGGL(k_select_pick<K>, dim3(1), dim3(256), 0, s, st, ghist, buf_ctr);
+    hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, step, S, ghist_first, ghist,
+                       ctr);
   }
+  hipLaunchKernelGGL(k_select_final<K>, dim3(1), dim3(256), 0, s, S + (D - 1), ghist + (D - 2) * 256, ctr + (D - 2),
+                     dst, place);
 }
 
 template <typename K>
@@ -394,11 +430,10 @@ static int select_enqueue(const K* din, size_t n, size_t k, char* b, K* value_ou
   const size_t cap = std::max<size_t>(n / 2, 1);
   const size_t nz = select_zero_words<K>();
   Layout L;
-  SelState* st = (SelState*)(b + L.take(sizeof(SelState)));
+  SelState* S = (SelState*)(b + L.take(sizeof(K) * sizeof(SelState)));
   uint32_t* z = (uint32_t*)(b + L.take(nz * 4));
   K* buf = (K*)(b + L.take(cap * sizeof(K)));
-  launch_select<K>(din, n, k, st, z, (uint32_t)nz, buf, cap, s);
-  hipLaunchKernelGGL(k_select_result<K>, dim3(1), dim3(1), 0, s, st, value_out);
+  launch_select<K>(din, n, k, S, z, (uint32_t)nz, buf, cap, value_out, nullptr, s);
   PA_HIP(hipGetLastError());
   return PA_OK;
 }
@@ -419,7 +454,7 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
   const bool dres = is_device_ptr(result);
   const size_t cap = std::max<size_t>(n / 2, 1);
   Layout L;
-  const size_t o_st = L.take(sizeof(SelState));
+  const size_t o_st = L.take(sizeof(K) * sizeof(SelState));
   const size_t nz = select_zero_words<K>();
   const size_t o_z = L.take(nz * 4);
   const size_t o_res = L.take(sizeof(K));
@@ -428,7 +463,7 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
   void* base;
   PA_TRY(ensure_scratch(ctx, L.off, &base));
   char* b = (char*)base;
-  SelState* st = (SelState*)(b + o_st);
+  SelState* S = (SelState*)(b + o_st);
   uint32_t* z = (uint32_t*)(b + o_z);
   K* dres_tmp = (K*)(b + o_res);
   K* buf = (K*)(b + o_buf);
@@ -437,9 +472,7 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
     PA_TRY(stage_to_device(ctx, b + o_in, keys, n * sizeof(K)));
     din = (const K*)(b + o_in);
   }
-  launch_select<K>(din, n, k, st, z, (uint32_t)nz, buf, cap, s);
-  if (place && dk) hipLaunchKernelGGL(k_select_place<K>, dim3(1), dim3(1), 0, s, st, place);
-  hipLaunchKernelGGL(k_select_result<K>, dim3(1), dim3(1), 0, s, st, dres ? result : dres_tmp);
+  launch_select<K>(din, n, k, S, z, (uint32_t)nz, buf, cap, dres ? result : dres_tmp, place && dk ? place : nullptr, s);
   PA_HIP(hipGetLastError());
   if (!dres) PA_HIP(hipMemcpyAsync(result, dres_tmp, sizeof(K), hipMemcpyDeviceToHost, s));
   if (!dres || !dk) PA_HIP(hipStreamSynchronize(s));
