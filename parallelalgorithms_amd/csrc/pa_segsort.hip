@@ -14,12 +14,19 @@
 //                         (256, 512, 1024 threads x 16 keys) so a segment
 //                         pays for at most twice its length in LDS slots.
 //                         One launch per size for all of them.
+//   len <= 2^20           a persistent kernel: workgroups pull whole segments
+//                         (longest first) from a device counter and sort each
+//                         alone, streaming it through LDS in 8192-key chunks
+//                         with running digit offsets (no look-back), when the
+//                         batch holds enough of them to balance the
+//                         workgroups; the longest ones otherwise go to the
+//                         onesweep class.
 //   larger                segmented onesweep: one fused read builds every
 //                         segment's 4 digit histograms, then one onesweep
 //                         launch per digit covers every tile of every large
 //                         segment; each segment has its own look-back chain.
-// So the whole batch costs at most 9 launches however many segments it holds,
-// where per-array calls cost 6 launches per array.
+// So the whole batch costs at most 10 launches however many segments it
+// holds, where per-array calls cost 6 launches per array.
 #include <algorithm>
 #include <vector>
 
@@ -33,6 +40,7 @@ constexpr int kSmallItems = 16;
 constexpr int kLdsClasses = 3;
 constexpr uint32_t kLdsLimit[kLdsClasses] = {4096, 8192, 16384};
 constexpr uint32_t kLdsMax = kLdsLimit[kLdsClasses - 1];
+constexpr uint64_t kQueueDefaultMax = 1u << 20;
 
 // Segmented onesweep tile shape (keys only): the plain sort's default pass,
 // 512 x 48 keys staged in two rounds, ranked in 16-lane units.
@@ -46,6 +54,60 @@ constexpr uint32_t kSegTile = kSegBlock * kSegItems;
 // tiles are spread over the whole launch and a tile's predecessor in its
 // segment was claimed well before it.
 constexpr uint32_t kSegBuckets = 1024;
+
+// Stable ranks within a wave: rank[i] = the wave's earlier keys with the same
+// digit (earlier items, then lower lanes of the same item) plus wh[digit] on
+// entry; wh[d] is advanced by the wave's count of d.  Peers of one digit are
+// found by every lane ORing its bit into the wave's LDS slot for the digit;
+// the lowest peer clears the slot, so wm is all zero on entry and on exit.
+template <int ITEMS>
+__device__ __forceinline__ void wave_rank(const uint32_t (&key)[ITEMS], uint32_t shift, uint32_t* __restrict__ wh,
+                                          uint64_t* __restrict__ wm, uint32_t lane, uint32_t (&rank)[ITEMS]) {
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t d = (key[i] >> shift) & 0xFFu;
+    atomicOr(reinterpret_cast<unsigned long long*>(&wm[d]), 1ull << lane);
+    const uint64_t peers = wm[d];
+    const uint32_t below = dev::count_below(peers);
+    const uint32_t base = wh[d];
+    rank[i] = base + below;
+    if (below == 0) {
+      wh[d] = base + (uint32_t)__popcll(peers);
+      wm[d] = 0ull;
+    }
+  }
+}
+
+// Turns the per-wave digit counts s_whist[w * 256 + d] into the start of
+// (wave w, digit d) in the block's digit-sorted order; when s_dstart is given
+// it receives each digit's start.  Threads 0..255 take one digit each.  Call
+// after a barrier that follows the counting; ends with a barrier.
+template <int WAVES>
+__device__ __forceinline__ void block_digit_starts(uint32_t* __restrict__ s_whist, uint32_t* __restrict__ s_wsum,
+                                                   uint32_t* __restrict__ s_dstart) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t cnt = 0, inc = 0;
+  if (tid < 256) {
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = s_whist[w * 256 + tid];
+      s_whist[w * 256 + tid] = cnt;
+      cnt += c;
+    }
+    inc = dev::wave_incl_scan(cnt, lane);
+    if (lane == 63) s_wsum[wave] = inc;
+  }
+  __syncthreads();
+  if (tid < 256) {
+    uint32_t pre = 0;
+    for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
+    const uint32_t start = pre + inc - cnt;
+    if (s_dstart) s_dstart[tid] = start;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) s_whist[w * 256 + tid] += start;
+  }
+  __syncthreads();
+}
 
 // Whole-segment LDS sort.  Padding slots hold 0xFFFFFFFF: a stable sort
 // keeps them after every real key, so the first len slots are the answer.
@@ -73,45 +135,10 @@ __device__ __forceinline__ void lds_sort_segment(uint32_t* __restrict__ seg, uin
   for (uint32_t shift = 0; shift < 32; shift += 8) {
     for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_whist[i] = 0;
     __syncthreads();
-    // Peers of one digit within the wave: each lane ORs its bit into the
-    // wave's LDS slot for its digit; the lowest peer clears the slot.
     uint32_t rank[kSmallItems];
-#pragma unroll
-    for (int i = 0; i < kSmallItems; ++i) {
-      const uint32_t d = (key[i] >> shift) & 0xFFu;
-      atomicOr(reinterpret_cast<unsigned long long*>(&wm[d]), 1ull << lane);
-      const uint64_t peers = wm[d];
-      const uint32_t below = dev::count_below(peers);
-      const uint32_t base = wh[d];
-      rank[i] = base + below;
-      if (below == 0) {
-        wh[d] = base + (uint32_t)__popcll(peers);
-        wm[d] = 0ull;
-      }
-    }
+    wave_rank<kSmallItems>(key, shift, wh, wm, lane, rank);
     __syncthreads();
-    // Digit totals, wave prefixes and the block-wide digit offsets (threads
-    // 0..255, one digit each).
-    uint32_t cnt = 0, inc = 0;
-    if (tid < 256) {
-#pragma unroll
-      for (int w = 0; w < WAVES; ++w) {
-        const uint32_t c = s_whist[w * 256 + tid];
-        s_whist[w * 256 + tid] = cnt;
-        cnt += c;
-      }
-      inc = dev::wave_incl_scan(cnt, lane);
-      if (lane == 63) s_wsum[wave] = inc;
-    }
-    __syncthreads();
-    if (tid < 256) {
-      uint32_t pre = 0;
-      for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
-      const uint32_t start = pre + inc - cnt;
-#pragma unroll
-      for (int w = 0; w < WAVES; ++w) s_whist[w * 256 + tid] += start;
-    }
-    __syncthreads();
+    block_digit_starts<WAVES>(s_whist, s_wsum, nullptr);
 #pragma unroll
     for (int i = 0; i < kSmallItems; ++i) s_keys[wh[(key[i] >> shift) & 0xFFu] + rank[i]] = key[i];
     __syncthreads();
@@ -120,6 +147,117 @@ __device__ __forceinline__ void lds_sort_segment(uint32_t* __restrict__ seg, uin
     __syncthreads();
   }
   for (uint32_t j = tid; j < n; j += BLOCK) seg[j] = s_keys[j];
+}
+
+// Mid-size class: a persistent kernel.  Each workgroup pulls the next segment
+// from a device counter over a list ordered by length (longest first) and
+// sorts it alone: one read builds the segment's 4 digit histograms, then each
+// of the 4 passes streams the segment in chunks of kQueueChunk keys in input
+// order.  A chunk is ranked by the pass's digit in LDS, staged in digit order
+// and written to the running offset of each digit, which the workgroup keeps
+// in LDS, so the passes need no cross-workgroup prefix at all.  Ping-pong
+// between the segment and its tmp; after 4 passes the result is back in the
+// segment.  Every workgroup leaves once the counter passes the list.
+constexpr int kQueueBlock = 512;
+constexpr uint32_t kQueueChunk = kQueueBlock * kSmallItems;
+constexpr size_t kQueueGrid = 512;  // two workgroups per CU on the 256 CUs of an MI355X
+
+// Longest segment served by the queue.  PA_SEG_QUEUE_MAX overrides it (read
+// per plan, so sweeps can change it between calls); at most kLdsMax disables
+// the class.
+static uint64_t seg_queue_max() {
+  const char* e = getenv("PA_SEG_QUEUE_MAX");
+  return e ? strtoull(e, nullptr, 10) : (uint64_t)kQueueDefaultMax;
+}
+
+__global__ __launch_bounds__(kQueueBlock) void k_seg_queue(const uint64_t* __restrict__ seg_ptr,
+                                                           const uint64_t* __restrict__ seg_tmp,
+                                                           const uint32_t* __restrict__ seg_len, uint32_t nseg,
+                                                           uint32_t* __restrict__ next) {
+  constexpr int WAVES = kQueueBlock / 64;
+  constexpr int WAVE_SPAN = 64 * kSmallItems;
+  __shared__ uint32_t s_keys[kQueueChunk];
+  __shared__ uint32_t s_whist[WAVES * 256];
+  __shared__ uint64_t s_match[WAVES * 256];
+  __shared__ uint32_t s_cnt[4 * 256];
+  __shared__ uint32_t s_off[256];
+  __shared__ uint32_t s_dstart[256];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_seg;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t* wh = s_whist + wave * 256;
+  uint64_t* wm = s_match + wave * 256;
+  for (uint32_t i = tid; i < WAVES * 256; i += kQueueBlock) s_match[i] = 0ull;
+  for (;;) {
+    if (tid == 0) s_seg = atomicAdd(next, 1u);
+    for (uint32_t i = tid; i < 4 * 256; i += kQueueBlock) s_cnt[i] = 0;
+    __syncthreads();
+    const uint32_t g = s_seg;
+    if (g >= nseg) break;  // uniform: every thread read the same s_seg
+    uint32_t* src = reinterpret_cast<uint32_t*>(seg_ptr[g]);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(seg_tmp[g]);
+    const uint32_t m = seg_len[g];
+    // The segment's 4 digit histograms in one read.
+    for (uint32_t i0 = 0; i0 < m; i0 += 4 * kQueueBlock) {
+      uint32_t x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t idx = i0 + u * kQueueBlock + tid;
+        x[u] = idx < m ? src[idx] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (i0 + u * kQueueBlock + tid >= m) continue;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) atomicAdd(&s_cnt[d * 256 + ((x[u] >> (8 * d)) & 0xFFu)], 1u);
+      }
+    }
+    __syncthreads();
+    for (uint32_t shift = 0; shift < 32; shift += 8) {
+      // Running output offset of every digit: exclusive scan of its counts.
+      if (tid < 256) {
+        const uint32_t c = s_cnt[(shift / 8) * 256 + tid];
+        const uint32_t inc = dev::wave_incl_scan(c, lane);
+        if (lane == 63) s_wsum[wave] = inc;
+        s_off[tid] = inc - c;
+      }
+      __syncthreads();
+      if (tid < 256)
+        for (uint32_t w = 0; w < wave; ++w) s_off[tid] += s_wsum[w];
+      for (uint32_t c0 = 0; c0 < m; c0 += kQueueChunk) {
+        const uint32_t cl = min(kQueueChunk, m - c0);
+        uint32_t key[kSmallItems];
+        const uint32_t wbase = wave * WAVE_SPAN + lane;
+#pragma unroll
+        for (int i = 0; i < kSmallItems; ++i) {
+          const uint32_t idx = wbase + i * 64;
+          key[i] = idx < cl ? src[c0 + idx] : 0xFFFFFFFFu;
+        }
+        for (uint32_t i = tid; i < WAVES * 256; i += kQueueBlock) s_whist[i] = 0;
+        __syncthreads();
+        uint32_t rank[kSmallItems];
+        wave_rank<kSmallItems>(key, shift, wh, wm, lane, rank);
+        __syncthreads();
+        block_digit_starts<WAVES>(s_whist, s_wsum, s_dstart);
+#pragma unroll
+        for (int i = 0; i < kSmallItems; ++i) s_keys[wh[(key[i] >> shift) & 0xFFu] + rank[i]] = key[i];
+        __syncthreads();
+        // Padding keys (digit 255, after every real key) fill chunk slots
+        // [cl, kQueueChunk), so j < cl writes exactly the real keys.
+        for (uint32_t j = tid; j < cl; j += kQueueBlock) {
+          const uint32_t k = s_keys[j];
+          const uint32_t d = (k >> shift) & 0xFFu;
+          dst[s_off[d] + (j - s_dstart[d])] = k;
+        }
+        __syncthreads();
+        if (tid < 256) s_off[tid] += (tid < 255 ? s_dstart[tid + 1] : kQueueChunk) - s_dstart[tid];
+      }
+      __syncthreads();
+      uint32_t* t = src;
+      src = dst;
+      dst = t;
+    }
+  }
 }
 
 // One segment per workgroup, from the plan's (address, length) lists.
@@ -139,7 +277,7 @@ __global__ __launch_bounds__(256) void k_seg_prepare(uint32_t* __restrict__ coun
                                                      uint32_t* __restrict__ counters) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncounts; i += stride) counts[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
 }
 
 // One tile per block: 4 digit histograms of the tile's keys into its
@@ -176,8 +314,9 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restri
 // Sorts the segments described by (address, length) lists; large segments get
 // ping-pong space from `work` (their tmp addresses).  ctx.mu must be held.
 struct SegPlan {
-  std::vector<uint64_t> small_ptr[kLdsClasses], large_in, large_tmp, tile_info;
-  std::vector<uint32_t> small_len[kLdsClasses], large_len, large_tiles, large_row;
+  std::vector<uint64_t> small_ptr[kLdsClasses], queue_ptr, queue_tmp, large_in, large_tmp, tile_info;
+  std::vector<uint32_t> small_len[kLdsClasses], queue_len, large_len, large_tiles, large_row;
+  const uint64_t queue_max = seg_queue_max();
   void add(uint64_t ptr, uint64_t len, uint64_t tmp_ptr) {
     if (len <= 1) return;
     for (int c = 0; c < kLdsClasses; ++c) {
@@ -187,6 +326,12 @@ struct SegPlan {
         return;
       }
     }
+    if (len <= queue_max) {
+      queue_ptr.push_back(ptr);
+      queue_tmp.push_back(tmp_ptr);
+      queue_len.push_back((uint32_t)len);
+      return;
+    }
     large_in.push_back(ptr);
     large_tmp.push_back(tmp_ptr);
     large_len.push_back((uint32_t)len);
@@ -195,6 +340,35 @@ struct SegPlan {
   // Look-back rows (contiguous per segment) and the interleaved claim order:
   // a counting sort of the tiles by bucket, stable in (segment, tile).
   void finalize() {
+    // The queue is served longest first, so the last segments taken are short.
+    std::vector<uint32_t> order(queue_len.size());
+    for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return queue_len[a] > queue_len[b]; });
+    std::vector<uint64_t> qp(order.size()), qt(order.size());
+    std::vector<uint32_t> ql(order.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+      qp[i] = queue_ptr[order[i]];
+      qt[i] = queue_tmp[order[i]];
+      ql[i] = queue_len[order[i]];
+    }
+    // A workgroup sorts a queued segment alone, so the queue only pays when
+    // the longest segment is at most twice the average load of a workgroup;
+    // the longest segments that break this go to the onesweep class instead.
+    size_t first = qp.size();
+    uint64_t tail = 0;
+    for (size_t i = qp.size(); i-- > 0;) {
+      tail += ql[i];
+      if ((uint64_t)ql[i] * kQueueGrid <= 2 * tail) first = i;
+    }
+    for (size_t i = 0; i < first; ++i) {
+      large_in.push_back(qp[i]);
+      large_tmp.push_back(qt[i]);
+      large_len.push_back(ql[i]);
+      large_tiles.push_back((uint32_t)((ql[i] + kSegTile - 1) / kSegTile));
+    }
+    queue_ptr.assign(qp.begin() + first, qp.end());
+    queue_tmp.assign(qt.begin() + first, qt.end());
+    queue_len.assign(ql.begin() + first, ql.end());
     const size_t nl = large_tiles.size();
     large_row.resize(nl);
     uint64_t rows = 0;
@@ -218,7 +392,7 @@ struct SegPlan {
 // tile order) followed by per-segment counts, tile counters, the look-back
 // table and `extra` bytes of ping-pong space for large segments.
 struct PlanLayout {
-  size_t t_sp[kLdsClasses], t_sl[kLdsClasses], t_li, t_lt, t_ll, t_lr, t_ti, table_bytes;
+  size_t t_sp[kLdsClasses], t_sl[kLdsClasses], t_qp, t_qt, t_ql, t_li, t_lt, t_ll, t_lr, t_ti, table_bytes;
   size_t o_tab, o_cnt, o_ctr, o_lb, o_work, total;
 };
 
@@ -230,6 +404,10 @@ static PlanLayout plan_layout(const SegPlan& P, size_t extra) {
     Y.t_sp[c] = T.take(P.small_ptr[c].size() * 8);
     Y.t_sl[c] = T.take(P.small_len[c].size() * 4);
   }
+  const size_t nq = P.queue_len.size();
+  Y.t_qp = T.take(nq * 8);
+  Y.t_qt = T.take(nq * 8);
+  Y.t_ql = T.take(nq * 4);
   Y.t_li = T.take(nl * 8);
   Y.t_lt = T.take(nl * 8);
   Y.t_ll = T.take(nl * 4);
@@ -239,16 +417,21 @@ static PlanLayout plan_layout(const SegPlan& P, size_t extra) {
   Layout L;
   Y.o_tab = L.take(Y.table_bytes);
   Y.o_cnt = L.take(nl * 1024 * 4);
-  Y.o_ctr = L.take(16);
+  Y.o_ctr = L.take(32);
   Y.o_lb = L.take((size_t)4 * nt * 256 * 4);
   Y.o_work = L.take(extra);
   Y.total = L.off;
   return Y;
 }
 
-// Large segments without a caller tmp get consecutive ping-pong space.
+// Queue and large segments without a caller tmp get consecutive ping-pong
+// space.
 static void assign_work(SegPlan& P, char* work) {
   uint64_t off = 0;
+  for (size_t i = 0; i < P.queue_tmp.size(); ++i) {
+    P.queue_tmp[i] = (uint64_t)(uintptr_t)work + off * 4;
+    off += P.queue_len[i];
+  }
   for (size_t i = 0; i < P.large_tmp.size(); ++i) {
     P.large_tmp[i] = (uint64_t)(uintptr_t)work + off * 4;
     off += P.large_len[i];
@@ -260,6 +443,9 @@ static void write_tables(const SegPlan& P, const PlanLayout& Y, char* pin) {
     std::copy(P.small_ptr[c].begin(), P.small_ptr[c].end(), (uint64_t*)(pin + Y.t_sp[c]));
     std::copy(P.small_len[c].begin(), P.small_len[c].end(), (uint32_t*)(pin + Y.t_sl[c]));
   }
+  std::copy(P.queue_ptr.begin(), P.queue_ptr.end(), (uint64_t*)(pin + Y.t_qp));
+  std::copy(P.queue_tmp.begin(), P.queue_tmp.end(), (uint64_t*)(pin + Y.t_qt));
+  std::copy(P.queue_len.begin(), P.queue_len.end(), (uint32_t*)(pin + Y.t_ql));
   std::copy(P.large_in.begin(), P.large_in.end(), (uint64_t*)(pin + Y.t_li));
   std::copy(P.large_tmp.begin(), P.large_tmp.end(), (uint64_t*)(pin + Y.t_lt));
   std::copy(P.large_len.begin(), P.large_len.end(), (uint32_t*)(pin + Y.t_ll));
@@ -293,9 +479,17 @@ static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStre
       default: hipLaunchKernelGGL(k_seg_small<1024>, dim3((unsigned)ns), dim3(1024), 0, s, d_sp, d_sl); break;
     }
   }
+  const size_t nq = P.queue_len.size();
+  if (nl || nq)
+    hipLaunchKernelGGL(k_seg_prepare, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((nl * 1024 + 255) / 256, 4096))),
+                       dim3(256), 0, s, counts, nl * 1024, ctr);
+  if (nq) {
+    ProfScope ps("k_seg_queue", s);
+    hipLaunchKernelGGL(k_seg_queue, dim3((unsigned)std::min<size_t>(nq, kQueueGrid)), dim3(kQueueBlock), 0, s,
+                       (const uint64_t*)(tab + Y.t_qp), (const uint64_t*)(tab + Y.t_qt), (const uint32_t*)(tab + Y.t_ql),
+                       (uint32_t)nq, ctr + 4);
+  }
   if (nl) {
-    hipLaunchKernelGGL(k_seg_prepare, dim3((unsigned)std::min<size_t>((nl * 1024 + 255) / 256, 4096)), dim3(256), 0, s,
-                       counts, nl * 1024, ctr);
     {
       ProfScope ps("k_seg_hist", s);
       hipLaunchKernelGGL(k_seg_hist, dim3((unsigned)nt), dim3(kSegBlock), 0, s, d_ti, d_li, d_ll, counts, (uint4*)lbp,

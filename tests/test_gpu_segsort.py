@@ -1,7 +1,8 @@
 """GPU parity of the segmented (batched) sort: every segment must equal the
 oracle's SortRadixPar of that segment and the plain per-array GPU call, bit
 for bit.  Covers every size class (empty, one key, LDS-resident small
-segments, single- and multi-tile onesweep segments) and presorted segments."""
+segments, the persistent queue's chunked segments, single- and multi-tile
+onesweep segments) and presorted segments."""
 import numpy as np
 import pytest
 
@@ -46,6 +47,65 @@ def test_segmented_edge_classes(pa, orc):
     d = to_dev(keys)
     pa.segmented_sort_u32(d, dev_empty(keys.size, np.uint32), offs)
     _check(keys, offs, to_host(d, np.uint32), orc, oracle_every=1)
+
+
+def _launches(P, fn, *prefixes):
+    """Launch counts of the named kernels during fn(), from the library's
+    event profiler."""
+    P.profile_reset()
+    P.profile_only(None)
+    P.profile_enable(True)
+    try:
+        fn()
+    finally:
+        P.profile_enable(False)
+    n = {p: P.profile_query(p)[1] for p in prefixes}
+    P.profile_reset()
+    return n
+
+
+def test_segmented_queue_class(pa, orc):
+    # The persistent queue sorts segments of 16,384 < len <= 2^20 keys in
+    # 8,192-key chunks when the batch balances over its 512 workgroups (the
+    # longest queued segment at most twice the average load).  Edge lengths:
+    # chunk edges, a ragged last chunk, 131,072 +- 1, with constant, presorted,
+    # descending and shared-digit segments (passes where one digit holds
+    # every key), among 1,600 fillers of 20K-30K keys so the queue is used;
+    # a 600K segment breaks the balance and goes to the onesweep class.
+    import parallelalgorithms_amd as P
+    edges = [16_385, 24_576, 32_767, 32_768, 32_769, 57_345, 131_071, 131_072, 131_073, 140_000,
+             40_000, 40_000, 40_000, 40_000, 16_385, 90_000, 600_000]
+    r = P.gen_u64(80, 1600)
+    fill = (20_000 + (r % np.uint64(10_000))).astype(np.int64).tolist()
+    lens = edges[:8] + fill[:800] + edges[8:] + fill[800:]
+    offs = np.zeros(len(lens) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    keys = P.gen_u32(79, int(offs[-1]))
+    seg = lambda i: slice(int(offs[i]), int(offs[i + 1]))
+    c = 808  # index of edges[8] in lens
+    keys[seg(c + 10)] = 0xA5A5A5A5                                       # constant
+    keys[seg(c + 11)] = np.sort(keys[seg(c + 11)])                       # presorted
+    keys[seg(c + 12)] = np.sort(keys[seg(c + 12)])[::-1]                 # descending
+    keys[seg(c + 13)] = (keys[seg(c + 13)] & 0xFF00FF00) | 0x00120034    # digits 0 and 2 constant
+    keys[seg(c + 14)] = keys[seg(c + 14)] & 0x3                          # many equal keys
+    for tmp in (dev_empty(keys.size, np.uint32), None):
+        d = to_dev(keys)
+        counts = _launches(P, lambda: pa.segmented_sort_u32(d, tmp, offs), "k_seg_queue", "k_onesweep_seg")
+        assert counts == {"k_seg_queue": 1, "k_onesweep_seg": 4}, counts
+        _check(keys, offs, to_host(d, np.uint32), orc, oracle_every=97)
+
+
+def test_segmented_queue_needs_balance(pa):
+    # Too few mid-size segments to balance the workgroups: all of them go to
+    # the onesweep class, and the result is the same.
+    import parallelalgorithms_amd as P
+    lens = [20_000, 131_072, 70_001, 3]
+    offs = np.zeros(len(lens) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    keys = P.gen_u32(81, int(offs[-1]))
+    d = to_dev(keys)
+    assert _launches(P, lambda: pa.segmented_sort_u32(d, None, offs), "k_seg_queue") == {"k_seg_queue": 0}
+    _check(keys, offs, to_host(d, np.uint32))
 
 
 def test_segmented_tmp_null_and_offset_base(pa):
