@@ -14,7 +14,7 @@
 //                         (256, 512, 1024 threads x 16 keys) so a segment
 //                         pays for at most twice its length in LDS slots.
 //                         One launch per size for all of them.
-//   len <= 2^20           a persistent kernel: workgroups pull whole segments
+//   len <= 65536          a persistent kernel: workgroups pull whole segments
 //                         (longest first) from a device counter and sort each
 //                         alone, streaming it through LDS in 8192-key chunks
 //                         with running digit offsets (no look-back), when the
@@ -24,7 +24,8 @@
 //   larger                segmented onesweep: one fused read builds every
 //                         segment's 4 digit histograms, then one onesweep
 //                         launch per digit covers every tile of every large
-//                         segment; each segment has its own look-back chain.
+//                         segment; each segment has its own look-back chain,
+//                         and tiles are claimed in groups of a few segments.
 // So the whole batch costs at most 10 launches however many segments it
 // holds, where per-array calls cost 6 launches per array.
 #include <algorithm>
@@ -40,7 +41,7 @@ constexpr int kSmallItems = 16;
 constexpr int kLdsClasses = 3;
 constexpr uint32_t kLdsLimit[kLdsClasses] = {4096, 8192, 16384};
 constexpr uint32_t kLdsMax = kLdsLimit[kLdsClasses - 1];
-constexpr uint64_t kQueueDefaultMax = 1u << 20;
+constexpr uint64_t kQueueDefaultMax = 65536;
 
 // Segmented onesweep tile shape (keys only): the plain sort's default pass,
 // 512 x 48 keys staged in two rounds, ranked in 16-lane units.
@@ -49,11 +50,18 @@ constexpr int kSegItems = 48;
 constexpr int kSegLbw = 8;
 constexpr int kSegSplit = 2;
 constexpr uint32_t kSegTile = kSegBlock * kSegItems;
-// Claim-order resolution: tile k of a segment with T tiles goes to bucket
-// floor((2k + 1) * kSegBuckets / (2T)), buckets in order, so every segment's
-// tiles are spread over the whole launch and a tile's predecessor in its
-// segment was claimed well before it.
+// Claim order: the large segments are cut into consecutive groups of at least
+// kSegGroupTiles tiles.  Groups are claimed in order; inside a group, tile k
+// of a segment with T tiles goes to bucket floor((2k + 1) * kSegBuckets / (2T)),
+// buckets in order, so a short segment's tiles are spread over its group and
+// a tile's predecessor in its segment was claimed well before it, while the
+// tiles in flight at any time come from a few groups only (spreading every
+// segment over the whole launch keeps tiles of hundreds of segments in flight
+// and was measured 25% slower on 10,000 arrays).  A segment of
+// kSegGroupTiles tiles or more is a group of its own and is claimed in order,
+// as the plain sort claims its tiles.
 constexpr uint32_t kSegBuckets = 1024;
+constexpr uint64_t kSegGroupTiles = 64;
 
 // Stable ranks within a wave: rank[i] = the wave's earlier keys with the same
 // digit (earlier items, then lower lanes of the same item) plus wh[digit] on
@@ -372,18 +380,33 @@ struct SegPlan {
     const size_t nl = large_tiles.size();
     large_row.resize(nl);
     uint64_t rows = 0;
-    std::vector<uint32_t> count(kSegBuckets + 1, 0);
     for (size_t g = 0; g < nl; ++g) {
       large_row[g] = (uint32_t)rows;
-      const uint64_t T = large_tiles[g];
-      rows += T;
-      for (uint64_t k = 0; k < T; ++k) ++count[(2 * k + 1) * kSegBuckets / (2 * T) + 1];
+      rows += large_tiles[g];
     }
-    for (uint32_t b = 0; b < kSegBuckets; ++b) count[b + 1] += count[b];
     tile_info.assign(rows, 0);
-    for (size_t g = 0; g < nl; ++g) {
-      const uint64_t T = large_tiles[g];
-      for (uint64_t k = 0; k < T; ++k) tile_info[count[(2 * k + 1) * kSegBuckets / (2 * T)]++] = ((uint64_t)g << 32) | k;
+    // PA_SEG_GROUP overrides the group size in tiles, for sweeps (0: one
+    // group holding every segment).
+    const char* ge = getenv("PA_SEG_GROUP");
+    uint64_t budget = ge ? strtoull(ge, nullptr, 10) : kSegGroupTiles;
+    if (budget == 0) budget = ~0ull;
+    size_t pos = 0;
+    std::vector<uint32_t> count(kSegBuckets + 1);
+    for (size_t g0 = 0, g1; g0 < nl; g0 = g1) {
+      uint64_t tiles = 0;
+      for (g1 = g0; g1 < nl && tiles < budget; ++g1) tiles += large_tiles[g1];
+      std::fill(count.begin(), count.end(), 0u);
+      for (size_t g = g0; g < g1; ++g) {
+        const uint64_t T = large_tiles[g];
+        for (uint64_t k = 0; k < T; ++k) ++count[(2 * k + 1) * kSegBuckets / (2 * T) + 1];
+      }
+      count[0] = (uint32_t)pos;
+      for (uint32_t b = 0; b < kSegBuckets; ++b) count[b + 1] += count[b];
+      for (size_t g = g0; g < g1; ++g) {
+        const uint64_t T = large_tiles[g];
+        for (uint64_t k = 0; k < T; ++k) tile_info[count[(2 * k + 1) * kSegBuckets / (2 * T)]++] = ((uint64_t)g << 32) | k;
+      }
+      pos = count[kSegBuckets - 1];
     }
   }
 };

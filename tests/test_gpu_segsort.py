@@ -65,16 +65,16 @@ def _launches(P, fn, *prefixes):
 
 
 def test_segmented_queue_class(pa, orc):
-    # The persistent queue sorts segments of 16,384 < len <= 2^20 keys in
+    # The persistent queue sorts segments of 16,384 < len <= 65,536 keys in
     # 8,192-key chunks when the batch balances over its 512 workgroups (the
     # longest queued segment at most twice the average load).  Edge lengths:
-    # chunk edges, a ragged last chunk, 131,072 +- 1, with constant, presorted,
+    # chunk edges, a ragged last chunk, 65,536 +- 1, with constant, presorted,
     # descending and shared-digit segments (passes where one digit holds
     # every key), among 1,600 fillers of 20K-30K keys so the queue is used;
-    # a 600K segment breaks the balance and goes to the onesweep class.
+    # the longer ones go to the onesweep class.
     import parallelalgorithms_amd as P
-    edges = [16_385, 24_576, 32_767, 32_768, 32_769, 57_345, 131_071, 131_072, 131_073, 140_000,
-             40_000, 40_000, 40_000, 40_000, 16_385, 90_000, 600_000]
+    edges = [16_385, 24_576, 32_767, 32_768, 32_769, 57_345, 65_535, 65_536, 65_537, 140_000,
+             40_000, 40_000, 40_000, 40_000, 16_385, 60_000, 600_000]
     r = P.gen_u64(80, 1600)
     fill = (20_000 + (r % np.uint64(10_000))).astype(np.int64).tolist()
     lens = edges[:8] + fill[:800] + edges[8:] + fill[800:]
