@@ -179,3 +179,39 @@ def test_segmented_plan_graph_replays(pa):
                 np.testing.assert_array_equal(got, to_host(c, np.uint32))
     finally:
         plan.close()
+
+
+def test_segmented_plan_graph_replays_queue(pa):
+    """A captured plan whose segments go through the persistent queue (1,200
+    arrays of 20K-40K keys, enough to balance its workgroups): the queue's
+    counter is cleared by a kernel inside the graph, so every replay sorts the
+    arrays' current contents.  Three replays with fresh keys written eagerly
+    between them, each checked against np.sort and the one-shot batched call."""
+    import torch
+    import parallelalgorithms_amd as P
+    r = P.gen_u64(82, 1200)
+    lens = (20_000 + (r % np.uint64(20_000))).astype(np.int64)
+    offs = np.zeros(len(lens) + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(lens)
+    buf = torch.empty(int(offs[-1]), dtype=torch.int32, device="cuda")
+    devs = [buf[int(offs[i]):int(offs[i + 1])] for i in range(len(lens))]
+    plan = P.SegmentedPlan(devs)
+    try:
+        for it in range(3):
+            P.fill_random_u32(buf, 500 + it)
+            if it == 2:
+                devs[5].copy_(torch.arange(int(lens[5]), dtype=torch.int32, device="cuda"))  # presorted
+            torch.cuda.synchronize()
+            before = to_host(buf, np.uint32).copy()
+            copy = buf.clone()
+            plan.run()
+            torch.cuda.synchronize()
+            pa.segmented_sort_ptrs_u32([copy[int(offs[i]):int(offs[i + 1])] for i in range(len(lens))])
+            torch.cuda.synchronize()
+            got = to_host(buf, np.uint32)
+            np.testing.assert_array_equal(got, to_host(copy, np.uint32))
+            for i in range(len(lens)):
+                a, b = int(offs[i]), int(offs[i + 1])
+                np.testing.assert_array_equal(got[a:b], np.sort(before[a:b]), err_msg=f"replay {it} segment {i}")
+    finally:
+        plan.close()
