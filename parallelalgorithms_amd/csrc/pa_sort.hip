@@ -368,7 +368,8 @@ struct Variants<uint32_t, false> {
       PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2, 32), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2, 32),
       PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2, 16), PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 16),
       PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 16)};
-  static constexpr int by_class[kClasses] = {2, 2, 6, 6};
+  // 512 x 16 up to 2^24 keys (4M: 0.209 -> 0.180 ms against 512 x 48; even at 16M), 512 x 48 split above.
+  static constexpr int by_class[kClasses] = {2, 2, 7, 6};
 };
 template <>
 struct Variants<uint32_t, true> {
