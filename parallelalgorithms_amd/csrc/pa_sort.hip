@@ -384,14 +384,18 @@ struct Variants<uint64_t, false> {
                                           PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true, 1, 32),
                                           PA_VARIANT(uint64_t, false, 512, 16, 8, true, 4, 0, true, 1, 16),
                                           PA_VARIANT(uint64_t, false, 512, 24, 8, true, 4, 0, true, 2, 16)};
-  static constexpr int by_class[kClasses] = {0, 0, 0, 4};  // 512 x 24 split, 16-lane units above 2^24 keys (100M: 4.77 -> 3.57 ms)
+  // 512 x 16 with 16-lane units up to 2^20 keys (1M: 0.155 -> 0.149 ms), the 512 x 24 split with
+  // 16-lane units above (4M: 0.331 -> 0.254 ms, 16M: 0.798 -> 0.632 ms, 100M: 4.77 -> 3.57 ms).
+  static constexpr int by_class[kClasses] = {0, 3, 4, 4};
 };
 template <>
 struct Variants<uint64_t, true> {
   static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 0, true, 1, 32),
                                           PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 1, true, 1, 32),
                                           PA_VARIANT(uint64_t, true, 512, 16, 8, true, 4, 0, true, 1, 16)};
-  static constexpr int by_class[kClasses] = {0, 0, 0, 2};  // 512 x 16, 16-lane units above 2^24 keys (100M: 7.01 -> 5.72 ms)
+  // 512 x 16 with 16-lane units above 2^20 keys (4M: 0.358 -> 0.284 ms, 16M: 1.10 -> 0.96 ms,
+  // 100M: 7.01 -> 5.72 ms).
+  static constexpr int by_class[kClasses] = {0, 0, 2, 2};
 };
 
 // Choices recorded by the autotuner (-1: none) for u32 keys-only sorts.
