@@ -35,9 +35,14 @@ def test_server_mix_equals_alone(pa, orc):
     jobs = _jobs(1240, 60)
     srv = P.Server(budget_bytes=2 << 30, streams=4, coalesce_max_keys=1 << 16)
     try:
-        dev, outs, tickets = [], [], []
-        for kind, keys, k in jobs:
-            d = to_dev(keys)
+        # Device inputs first, then every job submitted in one burst, so the
+        # small sorts are waiting together while the large jobs run (the
+        # uploads would otherwise space the submissions out).
+        import torch
+        dev = [to_dev(keys) for _, keys, _ in jobs]
+        torch.cuda.synchronize()
+        outs, tickets = [], []
+        for (kind, keys, k), d in zip(jobs, dev):
             out = None
             if kind == "select":
                 out = dev_empty(1, np.uint32)
@@ -53,7 +58,6 @@ def test_server_mix_equals_alone(pa, orc):
                 t = srv.submit(P.OP_HIST, d, P.KEY_U64, shift=56, bits=8, out=out)
             else:
                 t = srv.submit(P.OP_SORT, d, P.KEY_U64 if kind == "sort64" else P.KEY_U32)
-            dev.append(d)
             outs.append(out)
             tickets.append(t)
         lat = [srv.wait(t) for t in tickets]
