@@ -512,7 +512,14 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
     // pass 0 runs without look-back waits.
     const size_t pass_rows_bytes = rows * 256 * 4;
     const size_t zero_bytes = lb_bytes - pass_rows_bytes;
-    const size_t grid = std::max<size_t>(1, std::min<size_t>(rows, 2048));
+    // Tiles per block: about n / 512 keys per block (two blocks per CU), at
+    // least one tile and at most 49,152 keys, so a small block's fixed costs
+    // (clearing its bins, one global flush) are spread over several tiles.
+    // Measured per histogram read: 16M keys in 8K-key tiles 48 -> 31 us,
+    // 100M keys in 24K-key tiles 126 -> 122 us, 1M keys unchanged.
+    const size_t target = std::min<size_t>(std::max<size_t>(n / 512, tile), 49152);
+    const size_t tpb = std::max<size_t>(1, (target + tile / 2) / tile);
+    const size_t grid = std::max<size_t>(1, (rows + tpb - 1) / tpb);
     {
       ProfScope ps("k_hist_digits", s);
       hipLaunchKernelGGL(k_hist_tiles<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, (uint32_t)tile, lb,
