@@ -306,12 +306,24 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restri
   const uint32_t* k = reinterpret_cast<const uint32_t*>(seg_in[seg]);
   const uint32_t n = seg_len[seg];
   const uint32_t lo = local * kSegTile, hi = min(lo + kSegTile, n);
-  for (uint32_t idx = lo + tid; idx < hi; idx += kSegBlock) {
-    const uint32_t key = k[idx];
+  // Eight loads in flight per thread before any counting (segments start at
+  // any offset, so the loads are 4-byte, coalesced across the wave).
+  constexpr int U = 8;
+  for (uint32_t i0 = lo + tid; i0 < hi; i0 += U * kSegBlock) {
+    uint32_t x[U];
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      const uint32_t dg[1] = {(key >> (8 * d)) & 0xFFu};
-      dev::lds_count<1>(s_hist + d * 256, dg);
+    for (int u = 0; u < U; ++u) {
+      const uint32_t idx = i0 + u * kSegBlock;
+      x[u] = idx < hi ? k[idx] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i0 + u * kSegBlock >= hi) break;  // the indices grow with u
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const uint32_t dg[1] = {(x[u] >> (8 * d)) & 0xFFu};
+        dev::lds_count<1>(s_hist + d * 256, dg);
+      }
     }
   }
   __syncthreads();
