@@ -9,14 +9,15 @@
 // same scheme over digits 56 .. 0 (an extension; the reference is 32-bit).
 //
 // GPU form (the caller's array is never modified):
-//   k_select_first  one read of all keys builds the D digit histograms.
+//   k_select_first  one read of all keys builds the top digit's histogram
+//                   and the keys' minimum and maximum.
 //   k_select_pass   (launch i of D - 1; returns at once when the answer is
 //                   known) first derives its state from the counts the
-//                   previous launch left: launch 0 walks the digits from the
-//                   top (while the chosen bin still holds every candidate --
-//                   constant input, shared high bytes -- the next digit's
-//                   histogram over the same set is already known, so no read
-//                   is needed for it), launch i > 0 picks the bin of the
+//                   previous launch left: launch 0 fixes the high bytes that
+//                   the minimum and maximum share (every key has them; equal
+//                   minimum and maximum end the select there) and, when the
+//                   top byte differs, picks the top digit's bin from the
+//                   histogram; launch i > 0 picks the bin of the
 //                   previous pass's digit.  Every block computes the same
 //                   state from the same counts.  Then it histograms the next
 //                   digit over the keys matching the fixed high digits; when
@@ -92,18 +93,24 @@ __device__ __forceinline__ void flush_counts(const uint32_t* s_hist, uint32_t* g
     if (s_hist[i]) atomicAdd(&ghist[i], s_hist[i]);
 }
 
-// One read of all keys: the D digit histograms.  Four 16-byte loads per thread
-// are in flight per step.
+template <typename K>
+__device__ __forceinline__ K umax(K a, K b) { return a > b ? a : b; }
+
+// One read of all keys: the top digit's histogram, and each block's minimum
+// and maximum key (kept as max(~key) and max(key)) in mm[2 * block] and
+// mm[2 * block + 1].  Four 16-byte loads per thread are in flight per step.
 template <typename K>
 __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict__ keys, size_t n,
-                                                            uint32_t* __restrict__ ghist) {
+                                                            uint32_t* __restrict__ ghist, K* __restrict__ mm) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
   constexpr int U = 4;
-  __shared__ uint32_t s_hist[D * 256];
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < D * 256; i += kSelBlock) s_hist[i] = 0;
+  __shared__ uint32_t s_hist[256];
+  __shared__ K s_mm[kSelBlock / 64][2];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < 256) s_hist[tid] = 0;
   __syncthreads();
+  K lo_c = 0, hi = 0;  // max(~key), max(key) over this thread's keys
   size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
   if (head > n) head = n;
   const size_t nvec = (n - head) / PER;
@@ -119,13 +126,14 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
       if (v0 + u * stride >= nvec) continue;
       K kk[PER];
       __builtin_memcpy(kk, &x[u], 16);
+      uint32_t dg[PER];
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        uint32_t dg[PER];
-#pragma unroll
-        for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(kk[j], 8 * d);
-        dev::lds_count<PER>(s_hist + d * 256, dg);
+      for (int j = 0; j < PER; ++j) {
+        dg[j] = dev::digit8<K>(kk[j], 8 * (D - 1));
+        lo_c = umax<K>(lo_c, (K)~kk[j]);
+        hi = umax<K>(hi, kk[j]);
       }
+      dev::lds_count<PER>(s_hist, dg);
     }
   }
   if (blockIdx.x == 0) {
@@ -133,12 +141,32 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
     const size_t extra = head + (n - tail);
     for (size_t e = tid; e < extra; e += kSelBlock) {
       const K key = keys[e < head ? e : tail + (e - head)];
-#pragma unroll
-      for (int d = 0; d < D; ++d) atomicAdd(&s_hist[d * 256 + dev::digit8<K>(key, 8 * d)], 1u);
+      atomicAdd(&s_hist[dev::digit8<K>(key, 8 * (D - 1))], 1u);
+      lo_c = umax<K>(lo_c, (K)~key);
+      hi = umax<K>(hi, key);
     }
   }
+  // Wave maxima, then the block's, stored without atomics (the next launch
+  // reduces the blocks' values).
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo_c = umax<K>(lo_c, (K)__shfl_xor(lo_c, off, 64));
+    hi = umax<K>(hi, (K)__shfl_xor(hi, off, 64));
+  }
+  if (lane == 0) {
+    s_mm[wave][0] = lo_c;
+    s_mm[wave][1] = hi;
+  }
   __syncthreads();
-  flush_counts(s_hist, ghist, D * 256);
+  if (tid == 0) {
+    for (int w = 1; w < kSelBlock / 64; ++w) {
+      lo_c = umax<K>(lo_c, s_mm[w][0]);
+      hi = umax<K>(hi, s_mm[w][1]);
+    }
+    mm[2 * blockIdx.x] = lo_c;
+    mm[2 * blockIdx.x + 1] = hi;
+  }
+  flush_counts(s_hist, ghist, 256);
 }
 
 // The state a pass works with, computed by every block of that pass from the
@@ -154,35 +182,51 @@ struct SelShared {
   uint32_t wsum[4];
   uint32_t pick[1];
   uint64_t below;
+  uint64_t mm[kSelBlock / 64][2];
 };
 
-__device__ void walk_state(const uint32_t* s_hist, int D, SelState& t, SelShared& sh) {
-  uint64_t k = t.k, cand = t.cand, prefix = 0, pmask = 0;
-  int shift = 8 * (D - 1);
-  while (shift >= 0) {
-    const int d = shift / 8;
-    pick_bin(s_hist + d * 256, k, sh.wsum, sh.pick, &sh.below);
-    const uint32_t b = sh.pick[0];
-    const uint64_t c = s_hist[d * 256 + b];
-    prefix |= (uint64_t)b << shift;
-    pmask |= (uint64_t)0xFF << shift;
-    k -= sh.below;
-    shift -= 8;
-    __syncthreads();  // sh is reused by the next digit
-    if (c < cand) {
-      cand = c;
+// Step 0: the high bytes shared by the minimum mn and maximum mx are the
+// answer's (every key lies between them); if they are equal the select is
+// over.  When the top byte differs, its bin is picked from the first read's
+// histogram (the candidates are all keys), else the passes start at the
+// first differing byte.
+template <typename K>
+__device__ void walk_state(const uint32_t* s_top, K mn, K mx, SelState& t, SelShared& sh) {
+  constexpr int D = sizeof(K);
+  const uint64_t diff = (uint64_t)(mn ^ mx);
+  int hd = -1;  // highest differing byte
+  for (int d = D - 1; d >= 0; --d)
+    if ((diff >> (8 * d)) & 0xFF) {
+      hd = d;
       break;
     }
-  }
-  t.k = k;
-  t.cand = cand;
-  t.prefix = prefix;
-  t.pmask = pmask;
-  t.shift = shift;
-  t.done = shift < 0;
-  t.result = prefix;
   t.src = 0;
-  t.compact_now = cand <= t.cap;
+  if (hd < 0) {  // constant input
+    t.prefix = t.result = (uint64_t)mn;
+    t.pmask = ~0ull;
+    t.shift = -8;
+    t.done = 1;
+    t.compact_now = 0;
+    return;
+  }
+  const uint64_t above = hd == 7 ? 0 : (~0ull << (8 * (hd + 1)));
+  t.prefix = (uint64_t)mn & above;
+  t.pmask = above;
+  if (hd == D - 1) {
+    pick_bin(s_top, t.k, sh.wsum, sh.pick, &sh.below);
+    const uint32_t b = sh.pick[0];
+    t.k -= sh.below;
+    t.cand = s_top[b];
+    t.prefix |= (uint64_t)b << (8 * hd);
+    t.pmask |= (uint64_t)0xFF << (8 * hd);
+    t.shift = 8 * hd - 8;
+    __syncthreads();  // sh is reused by the caller
+  } else {
+    t.shift = 8 * hd;
+  }
+  t.done = 0;  // the differing byte hd (or the bytes below the top pick) remain
+  t.result = t.prefix;
+  t.compact_now = t.cand <= t.cap;
 }
 
 // prev: the state the previous pass used; hist: that pass's histogram of
@@ -214,12 +258,34 @@ __device__ SelState step_state(int step, const SelState* __restrict__ S, const u
                                const uint32_t* __restrict__ ghist_prev,
                                const unsigned long long* __restrict__ ctr_prev, SelState* __restrict__ S_next,
                                uint32_t* s_hist, SelShared& sh) {
-  constexpr int D = sizeof(K);
   SelState t = S[step];
   if (step == 0) {
-    for (uint32_t i = threadIdx.x; i < D * 256; i += blockDim.x) s_hist[i] = ghist_first[i];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_hist[i] = ghist_first[i];
     __syncthreads();
-    walk_state(s_hist, D, t, sh);
+    // Minimum and maximum over the first read's per-block values (that
+    // launch had as many blocks as this one).
+    const K* mm = reinterpret_cast<const K*>(ghist_first + 256);
+    K lo_c = 0, hi = 0;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+      lo_c = umax<K>(lo_c, mm[2 * i]);
+      hi = umax<K>(hi, mm[2 * i + 1]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      lo_c = umax<K>(lo_c, (K)__shfl_xor(lo_c, off, 64));
+      hi = umax<K>(hi, (K)__shfl_xor(hi, off, 64));
+    }
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+      sh.mm[wave][0] = lo_c;
+      sh.mm[wave][1] = hi;
+    }
+    __syncthreads();
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+      lo_c = umax<K>(lo_c, (K)sh.mm[w][0]);
+      hi = umax<K>(hi, (K)sh.mm[w][1]);
+    }
+    walk_state<K>(s_hist, (K)~lo_c, hi, t, sh);
   } else if (!t.done) {
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_hist[i] = ghist_prev[i];
     __syncthreads();
@@ -241,12 +307,11 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
                                                            const uint32_t* __restrict__ ghist_first,
                                                            uint32_t* __restrict__ ghist,
                                                            unsigned long long* __restrict__ ctr) {
-  constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
   constexpr int U = 4;
   constexpr int WAVES = kSelBlock / 64;
   constexpr uint32_t STAGE = 64 * U * PER;  // one step's worst case per wave
-  __shared__ uint32_t s_hist[D * 256];
+  __shared__ uint32_t s_hist[256];
   __shared__ K s_stage[WAVES][STAGE];
   __shared__ SelShared sh;
   const SelState t = step_state<K>(step, S, ghist_first, step ? ghist + (step - 1) * 256 : nullptr,
@@ -379,12 +444,15 @@ __global__ __launch_bounds__(256) void k_select_final(const SelState* __restrict
   }
 }
 
-// Scratch: the D per-step states, then the zeroed control words (the D digit
-// histograms of the first read, one histogram per pass, one 64-bit compaction
-// counter per pass), then the compaction buffer.
+// Scratch: the D per-step states, then the zeroed control words (the top
+// digit's histogram from the first read, the per-block minima and maxima,
+// one histogram per pass, one 64-bit compaction counter per pass), then the
+// compaction buffer.
+constexpr unsigned kSelMaxGrid = 512;  // blocks of the first read and the passes
+constexpr size_t kSelMinMaxWords = 2 * kSelMaxGrid * 2;  // two 64-bit words per block
 template <typename K>
 static constexpr size_t select_zero_words() {
-  return sizeof(K) * 256 + (sizeof(K) - 1) * 256 + 2 * (sizeof(K) - 1);
+  return 256 + kSelMinMaxWords + (sizeof(K) - 1) * 256 + 2 * (sizeof(K) - 1);
 }
 
 template <typename K>
@@ -405,16 +473,17 @@ template <typename K>
 static void launch_select(const K* din, size_t n, size_t k, SelState* S, uint32_t* z, uint32_t nz, K* buf, size_t cap,
                           K* dst, K* place, hipStream_t s) {
   constexpr int D = sizeof(K);
-  uint32_t* ghist_first = z;
-  uint32_t* ghist = z + D * 256;
+  uint32_t* ghist_first = z;  // followed by the minimum and maximum words
+  uint32_t* ghist = z + 256 + kSelMinMaxWords;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ghist + (D - 1) * 256);
   hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, S, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
                      (int32_t)(8 * (D - 1)), z, nz);
   const size_t nvec = n / (16 / sizeof(K));
-  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + 4 * kSelBlock - 1) / (4 * kSelBlock), 512));
+  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + 4 * kSelBlock - 1) / (4 * kSelBlock), kSelMaxGrid));
   {
     ProfScope ps("k_select_first", s);
-    hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghist_first);
+    hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghist_first,
+                       reinterpret_cast<K*>(ghist_first + 256));
   }
   for (int step = 0; step < D - 1; ++step) {
     ProfScope ps("k_select_pass", s);

@@ -26,6 +26,23 @@ def test_select_matches_oracle(pa, orc, dist, n):
     np.testing.assert_array_equal(to_host(d, np.uint32), a)  # input untouched
 
 
+@pytest.mark.parametrize("low_bits", [1, 8, 12, 16, 24, 31])
+def test_select_shared_high_bytes(pa, orc, low_bits):
+    """Keys that share their high bytes (only the low `low_bits` vary): the
+    select fixes the shared bytes from the keys' minimum and maximum and
+    starts its passes at the first byte that differs."""
+    import parallelalgorithms_amd as P
+    n = 200_003
+    r = P.gen_u32(31 + low_bits, n)
+    a = (np.uint32(0xA5C3E1F7) & ~np.uint32((1 << low_bits) - 1)) | (r & np.uint32((1 << low_bits) - 1))
+    d = to_dev(a)
+    for k in sorted({0, n // 100, n // 2, n - 21, n - 1}):
+        expect, _ = orc.select_radix_u32(a, k)
+        res = np.zeros(1, dtype=np.uint32)
+        pa.select_u32(d, k, res)
+        assert int(res[0]) == expect, (low_bits, k)
+
+
 def test_select_device_result_and_host_keys(pa, orc):
     n = 300_001
     a = make_u32("uniform", n, seed=21)
