@@ -33,6 +33,7 @@ constexpr int kPartBlock = 256;
 constexpr int kPartItems = 16;
 constexpr uint32_t kPartTile = kPartBlock * kPartItems;  // 4096 keys
 constexpr unsigned kPartGrid = 1024;
+static_assert(kPartGrid <= 1024, "k_part_offsets gives each block one thread of its single 1024-thread workgroup");
 
 // Block b owns the contiguous tiles [b * tpb, (b + 1) * tpb), tpb =
 // ceil(tiles / blocks), and item j of lane l in wave w of a tile is key
