@@ -172,9 +172,8 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
 // The state a pass works with, computed by every block of that pass from the
 // histogram the previous step left (every block reads the same counts and
 // picks the same bin, so no separate pick launch is needed):
-//   step 0: walk the first read's digit histograms from the top while the
-//           chosen bin still holds every candidate (constant input, shared
-//           high bytes), so no further read is needed for those digits;
+//   step 0: fix the high bytes the keys' minimum and maximum share, then
+//           pick the top digit's bin if the top byte differs (walk_state);
 //   step i: pick the bin of the previous pass's digit.
 // Block 0 stores the result in S[i + 1] for the next step.  Every thread of
 // the block must call these (they synchronise the block).
