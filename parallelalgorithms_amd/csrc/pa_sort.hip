@@ -41,21 +41,6 @@ __global__ __launch_bounds__(256) void k_sort_prepare(uint32_t* __restrict__ cou
   for (uint32_t i = threadIdx.x; i < ncounters; i += blockDim.x) counters[i] = 0;
 }
 
-// Per pass p (one block each): base[p][d] = sum_{d' < d} counts[p][d'], the
-// first output slot of digit d.
-__global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* __restrict__ counts,
-                                                     uint32_t* __restrict__ base) {
-  __shared__ uint32_t s_wsum[4];
-  const uint32_t d = threadIdx.x, lane = d & 63, wave = d >> 6;
-  const uint32_t c = counts[blockIdx.x * 256 + d];
-  const uint32_t inc = dev::wave_incl_scan(c, lane);
-  if (lane == 63) s_wsum[wave] = inc;
-  __syncthreads();
-  uint32_t pre = 0;
-  for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
-  base[blockIdx.x * 256 + d] = pre + inc - c;
-}
-
 // Pass-0 prefixes.  rows[t][d] holds digit d's count in pass-0 tile t; the
 // two kernels below turn every row into the inclusive status word of its tile
 // (count of digit d in tiles 0..t), as if every tile had already resolved its
@@ -65,11 +50,33 @@ __global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* __restrict_
 // loads in flight instead of walking the rows one after another.
 constexpr unsigned kPrefixThreads = 1024;
 
-// Step 1: partial[g][d] = count of digit d over group g's tiles.
+// Step 1: partial[g][d] = count of digit d over group g's tiles (blocks
+// g < groups).  The D blocks after them write the digit bases of pass p =
+// blockIdx.x - groups: base[p][d] = sum_{d' < d} counts[p][d'], the first
+// output slot of digit d.
 __global__ __launch_bounds__(kPrefixThreads) void k_tile_sums(const uint32_t* __restrict__ rows, uint32_t ntiles,
-                                                              uint32_t per, uint32_t* __restrict__ partial) {
+                                                              uint32_t per, uint32_t* __restrict__ partial,
+                                                              uint32_t groups, const uint32_t* __restrict__ counts,
+                                                              uint32_t* __restrict__ base) {
   __shared__ uint32_t s_q[4][256];
   const uint32_t d = threadIdx.x & 255, q = threadIdx.x >> 8;
+  if (blockIdx.x >= groups) {
+    // Threads 0..255 (waves 0..3) scan the pass's 256 counts.
+    const uint32_t p = blockIdx.x - groups, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t c = 0, inc = 0;
+    if (q == 0) {
+      c = counts[p * 256 + d];
+      inc = dev::wave_incl_scan(c, lane);
+      if (lane == 63) s_q[0][wave] = inc;
+    }
+    __syncthreads();
+    if (q == 0) {
+      uint32_t pre = 0;
+      for (uint32_t w = 0; w < wave; ++w) pre += s_q[0][w];
+      base[p * 256 + d] = pre + inc - c;
+    }
+    return;
+  }
   const uint32_t t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
   uint32_t acc = 0;
 #pragma unroll 8
@@ -509,7 +516,8 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
     // One read: all digit histograms, pass 0's per-tile digit counts (into
     // pass 0's look-back rows) and the clearing of the other passes' rows.
     // Two small kernels then turn pass 0's rows into inclusive prefixes, so
-    // pass 0 runs without look-back waits.
+    // pass 0 runs without look-back waits; the first also writes the digit
+    // bases.
     const size_t pass_rows_bytes = rows * 256 * 4;
     const size_t zero_bytes = lb_bytes - pass_rows_bytes;
     // Tiles per block: about n / 512 keys per block (two blocks per CU), at
@@ -525,11 +533,11 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
       hipLaunchKernelGGL(k_hist_tiles<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, (uint32_t)tile, lb,
                          counts, reinterpret_cast<uint4*>(lb + rows * 256), zero_bytes / 16, xform);
     }
-    hipLaunchKernelGGL(k_digit_bases, dim3(D), dim3(256), 0, s, counts, bases);
     const uint32_t groups = (uint32_t)std::min<size_t>(rows, kPrefixGroups);
     const uint32_t per = (uint32_t)((rows + groups - 1) / groups);
     ProfScope ps("k_tile_prefix", s);
-    hipLaunchKernelGGL(k_tile_sums, dim3(groups), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per, prefix_partial);
+    hipLaunchKernelGGL(k_tile_sums, dim3(groups + D), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per, prefix_partial,
+                       groups, counts, bases);
     hipLaunchKernelGGL(k_tile_scan, dim3(groups), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per, prefix_partial);
   }
   K* kin = keys;
