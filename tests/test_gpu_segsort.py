@@ -44,6 +44,8 @@ def test_segmented_edge_classes(pa, orc):
     import parallelalgorithms_amd as P
     keys = P.gen_u32(77, int(offs[-1]))
     keys[offs[5]:offs[6]] = 0x5A5A5A5A  # a constant segment
+    keys[offs[9]:offs[10]] |= 0xFFFFFF00  # keys sharing the padding's high digits (8,191 keys, LDS class)
+    keys[offs[17]:offs[18]] = 0xFFFFFFFF  # every key equal to the padding (24,577 keys: a full and a one-key onesweep tile)
     d = to_dev(keys)
     pa.segmented_sort_u32(d, dev_empty(keys.size, np.uint32), offs)
     _check(keys, offs, to_host(d, np.uint32), orc, oracle_every=1)
@@ -82,12 +84,14 @@ def test_segmented_queue_class(pa, orc):
     offs[1:] = np.cumsum(lens)
     keys = P.gen_u32(79, int(offs[-1]))
     seg = lambda i: slice(int(offs[i]), int(offs[i + 1]))
-    c = 808  # index of edges[8] in lens
+    c = 808  # index of edges[8] in lens; c + 9 onwards are queued fillers
     keys[seg(c + 10)] = 0xA5A5A5A5                                       # constant
     keys[seg(c + 11)] = np.sort(keys[seg(c + 11)])                       # presorted
     keys[seg(c + 12)] = np.sort(keys[seg(c + 12)])[::-1]                 # descending
     keys[seg(c + 13)] = (keys[seg(c + 13)] & 0xFF00FF00) | 0x00120034    # digits 0 and 2 constant
     keys[seg(c + 14)] = keys[seg(c + 14)] & 0x3                          # many equal keys
+    keys[seg(c + 15)] = keys[seg(c + 15)] | 0xFFFFFF00                   # real keys equal to the padding's digits
+    keys[seg(c + 16)] = 0xFFFFFFFF                                       # every key equal to the padding
     for tmp in (dev_empty(keys.size, np.uint32), None):
         d = to_dev(keys)
         counts = _launches(P, lambda: pa.segmented_sort_u32(d, tmp, offs), "k_seg_queue", "k_onesweep_seg")
