@@ -40,14 +40,23 @@ __global__ __launch_bounds__(kHistBlock) void k_hist_key(const K* __restrict__ k
   const size_t nvec = (n - head) / PER;
   const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
   const size_t stride = (size_t)gridDim.x * kHistBlock;
-  for (size_t v = (size_t)blockIdx.x * kHistBlock + tid; v < nvec; v += stride) {
-    const uint4 x = vk[v];
-    K kk[PER];
-    __builtin_memcpy(kk, &x, 16);
-    uint32_t dg[PER];
+  // Four 16-byte loads per thread in flight before any counting.
+  constexpr int U = 4;
+  for (size_t v0 = (size_t)blockIdx.x * kHistBlock + tid; v0 < nvec; v0 += stride * U) {
+    uint4 x[U];
 #pragma unroll
-    for (int j = 0; j < PER; ++j) dg[j] = (uint32_t)(kk[j] >> shift) & mask;
-    lds_count<PER>(s_bins, dg);
+    for (int u = 0; u < U; ++u)
+      if (v0 + u * stride < nvec) x[u] = vk[v0 + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (v0 + u * stride >= nvec) continue;
+      K kk[PER];
+      __builtin_memcpy(kk, &x[u], 16);
+      uint32_t dg[PER];
+#pragma unroll
+      for (int j = 0; j < PER; ++j) dg[j] = (uint32_t)(kk[j] >> shift) & mask;
+      lds_count<PER>(s_bins, dg);
+    }
   }
   if (blockIdx.x == 0) {
     const size_t tail = head + nvec * PER;
