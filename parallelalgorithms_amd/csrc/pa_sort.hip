@@ -381,8 +381,10 @@ struct Variants<uint32_t, false> {
 template <>
 struct Variants<uint32_t, true> {
   static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 0, true, 1, 32),
-                                          PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 1, true, 1, 32)};
-  static constexpr int by_class[kClasses] = {0, 0, 0, 0};
+                                          PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 1, true, 1, 32),
+                                          PA_VARIANT(uint32_t, true, 512, 20, 8, true, 4, 0, true, 1, 32)};
+  // 512 x 20 above 2^24 keys (100M: 2.157 -> 2.117 ms; at 16M it is 6% slower than 512 x 16).
+  static constexpr int by_class[kClasses] = {0, 0, 0, 2};
 };
 template <>
 struct Variants<uint64_t, false> {

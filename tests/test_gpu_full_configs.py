@@ -71,6 +71,21 @@ def test_config3_kv_u64_200M(pa):
     assert torch.equal(v.to(torch.int64), ref.indices)  # stable: ties keep input order
 
 
+def test_kv_u32_100M_stable(pa):
+    """Key-value u32 sort at 100M keys (the large-array pass shape), payload =
+    original index: keys equal torch.sort's, payload equals torch's stable
+    sort indices."""
+    import torch
+    import parallelalgorithms_amd as P
+    k = torch.from_numpy(P.gen_u32(1239, N).view(np.int32)).cuda()
+    k &= 0x7FFFFF0F  # many ties, so stability is exercised
+    ref = k.sort(stable=True)  # non-negative int32: signed order is the unsigned order
+    v = torch.arange(N, dtype=torch.int32, device="cuda")
+    pa.sort_kv_u32_u32(k, v)
+    assert torch.equal(k, ref.values)
+    assert torch.equal(v.to(torch.int64), ref.indices)
+
+
 def test_config3_f32_specials_100M(pa):
     import torch
     import parallelalgorithms_amd as P
