@@ -144,6 +144,11 @@ __global__ __launch_bounds__(kHistBlock) void k_counting_fill(uint8_t* __restric
   for (size_t w = (size_t)blockIdx.x * kHistBlock + tid; w < nvec; w += stride) {
     const size_t i0 = head + w * 16;
     uint32_t b = bin_of(i0);
+    if (s_end[b] > i0 + 15) {  // the usual case: all 16 bytes in one bin
+      const uint32_t r = b * 0x01010101u;
+      v[w] = make_uint4(r, r, r, r);
+      continue;
+    }
     uint32_t word[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
