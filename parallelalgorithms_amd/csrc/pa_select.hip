@@ -414,7 +414,24 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
       if (compact) take(x, hits);
     }
   }
-  if (compact && staged) flush();
+  if (compact) {
+    // The waves' last slices go out under one counter reservation per block:
+    // every wave reaches this point at once, and thousands of same-address
+    // atomics at the end of the launch would serialise.
+    __shared__ uint32_t s_wcnt[WAVES];
+    __shared__ unsigned long long s_bbase;
+    if (lane == 0) s_wcnt[wave] = staged;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t total = 0;
+      for (int w = 0; w < WAVES; ++w) total += s_wcnt[w];
+      s_bbase = total ? atomicAdd(buf_ctr, (unsigned long long)total) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long base = s_bbase;
+    for (uint32_t w = 0; w < wave; ++w) base += s_wcnt[w];
+    for (uint32_t j = lane; j < staged; j += 64) buf[base + j] = stage[j];
+  }
   __syncthreads();
   flush_counts(s_hist, gh, 256);
 }
