@@ -20,7 +20,7 @@
 namespace pa {
 
 constexpr int kHistBlock = 512;
-constexpr unsigned kHistMaxGrid = 2048;
+constexpr unsigned kHistMaxGrid = 512;
 
 using dev::lds_count;
 
