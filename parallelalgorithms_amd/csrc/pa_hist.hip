@@ -73,36 +73,50 @@ __global__ __launch_bounds__(kHistBlock) void k_hist_key(const K* __restrict__ k
   }
 }
 
-// 256-bin histogram of a byte array (16 bytes per lane per load).
+// 256-bin histogram of a byte array: 16 bytes per lane per load, four loads
+// in flight per thread, counted into lane-private sub-histograms (32 copies
+// of 32-bit counters, 32 KB of LDS) so the 16 adds per load never conflict.
 __global__ __launch_bounds__(kHistBlock) void k_hist_u8(const uint8_t* __restrict__ data, size_t n,
                                                         uint32_t* __restrict__ counts) {
-  __shared__ uint32_t s_bins[256];
-  const uint32_t tid = threadIdx.x;
-  if (tid < 256) s_bins[tid] = 0;
+  using H = dev::SubHist<32, false>;
+  __shared__ __align__(16) uint32_t s_h[H::words(256)];
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  for (uint32_t i = tid; i < H::words(256) / 4; i += kHistBlock) reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
   size_t head = (16 - ((uintptr_t)data & 15)) & 15;
   if (head > n) head = n;
   const size_t nvec = (n - head) / 16;
   const uint4* vk = reinterpret_cast<const uint4*>(data + head);
   const size_t stride = (size_t)gridDim.x * kHistBlock;
-  for (size_t v = (size_t)blockIdx.x * kHistBlock + tid; v < nvec; v += stride) {
-    uint4 x = vk[v];
-    uint32_t w[4] = {x.x, x.y, x.z, x.w};
-    uint32_t dg[16];
+  constexpr int U = 4;
+  for (size_t v0 = (size_t)blockIdx.x * kHistBlock + tid; v0 < nvec; v0 += stride * U) {
+    uint4 x[U];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) dg[j] = (w[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-    lds_count<16>(s_bins, dg);
+    for (int u = 0; u < U; ++u)
+      if (v0 + u * stride < nvec) x[u] = vk[v0 + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (v0 + u * stride >= nvec) continue;
+      const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+      uint32_t dg[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) dg[j] = (w[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+      H::add_all<16>(s_h, dg, lane);
+    }
   }
   if (blockIdx.x == 0) {
     const size_t tail = head + nvec * 16;
     const size_t extra = head + (n - tail);
     for (size_t e = tid; e < extra; e += kHistBlock) {
       size_t idx = e < head ? e : tail + (e - head);
-      atomicAdd(&s_bins[data[idx]], 1u);
+      H::add(s_h, data[idx], lane, 1u);
     }
   }
   __syncthreads();
-  if (tid < 256 && s_bins[tid]) atomicAdd(&counts[tid], s_bins[tid]);
+  if (tid < 256) {
+    const uint32_t c = H::sum(s_h, tid);
+    if (c) atomicAdd(&counts[tid], c);
+  }
 }
 
 // Counting sort of bytes, the fill half (CountingSortParallel.h:30-76: after

@@ -76,6 +76,9 @@ int stage_to_host(StreamCtx& ctx, void* dst, const void* src, size_t bytes);
 // True when p is device-accessible memory (hipMalloc / managed / pinned).
 bool is_device_ptr(const void* p);
 
+// Compute units of the current device (queried once per device).
+int device_cu_count();
+
 // Bump layout inside a scratch block (256-byte aligned slices).
 struct Layout {
   size_t off = 0;
@@ -89,12 +92,17 @@ struct Layout {
 // ---------------------------------------------------------------------------
 // Kernel timing hooks (pa_profile_*): records a pair of events around a launch.
 // ---------------------------------------------------------------------------
+// attach = true: the events are not recorded around the launch; the caller
+// passes start/stop to hipExtLaunchKernelGGL, which stamps them in the
+// dispatch itself (no extra queue packets), and sets `attached` once it did.
 struct ProfScope {
-  ProfScope(const char* name, hipStream_t s);
+  ProfScope(const char* name, hipStream_t s, bool attach = false);
   ~ProfScope();
   const char* name;
   hipStream_t stream;
   hipEvent_t start = nullptr, stop = nullptr;
+  bool attach = false;
+  bool attached = false;
 };
 
 // ---------------------------------------------------------------------------
@@ -190,6 +198,65 @@ __device__ __forceinline__ void lds_count(uint32_t* __restrict__ bins, const uin
   } else {
 #pragma unroll
     for (int j = 0; j < PER; ++j) atomicAdd(&bins[dg[j]], 1u);
+  }
+}
+
+// Lane-private LDS sub-histograms.  C copies (32 or 16) of the bins; copy
+// lane % C of bin b lives in word (b / 2) * C + lane % C with 16-bit counters
+// packed two per word (half b % 2) in the packed form, or in word b * C +
+// lane % C with 32-bit counters in the wide form.  With C = 32 the 32 lanes
+// of a half-wave always address 32 different banks, so an add costs the
+// LDS's minimum for a wave whatever the bins are (random keys, one repeated
+// key or a presorted run); a single shared histogram serialises on bank
+// conflicts instead.  Only lanes l and l + 32 can meet on one word, which the
+// atomic resolves.  A packed counter holds at most 65535: one copy of one bin
+// may count at most that many keys between flushes.
+template <int C, bool PACKED>
+struct SubHist {
+  static constexpr uint32_t words(uint32_t nbins) { return PACKED ? nbins / 2 * C : nbins * C; }
+  __device__ static __forceinline__ void add(uint32_t* h, uint32_t bin, uint32_t lane, uint32_t inc) {
+    if constexpr (PACKED)
+      atomicAdd(&h[(bin >> 1) * C + lane % C], inc << ((bin & 1) * 16));
+    else
+      atomicAdd(&h[bin * C + lane % C], inc);
+  }
+  // Adds PER bins held by one lane; PER equal bins take one add.
+  template <int PER>
+  __device__ static __forceinline__ void add_all(uint32_t* h, const uint32_t (&bins)[PER], uint32_t lane) {
+    bool same = true;
+#pragma unroll
+    for (int j = 1; j < PER; ++j) same = same && bins[j] == bins[0];
+    if (same) {
+      add(h, bins[0], lane, PER);
+    } else {
+#pragma unroll
+      for (int j = 0; j < PER; ++j) add(h, bins[j], lane, 1u);
+    }
+  }
+  // Sum over the copies of bin b.  Copy reads start at a bin-dependent offset,
+  // so the threads of a wave (consecutive bins) read different banks.
+  __device__ static __forceinline__ uint32_t sum(const uint32_t* h, uint32_t b) {
+    const uint32_t r = PACKED ? b >> 1 : b;
+    const uint32_t sh = PACKED ? (b & 1) * 16 : 0;
+    uint32_t s = 0;
+#pragma unroll 8
+    for (uint32_t c = 0; c < (uint32_t)C; ++c) {
+      const uint32_t v = h[r * C + ((c + r) % C)] >> sh;
+      s += PACKED ? (v & 0xFFFFu) : v;
+    }
+    return s;
+  }
+};
+
+// Buffer load of one 4- or 8-byte key at byte offset off of resource rs.
+template <typename K>
+__device__ __forceinline__ K buffer_load(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  if constexpr (sizeof(K) == 8) {
+    const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+    const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4, 0, 0);
+    return (K)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (K)__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
   }
 }
 

@@ -213,6 +213,23 @@ int stage_to_host(StreamCtx& ctx, void* dst, const void* src, size_t bytes) {
   return PA_OK;
 }
 
+int device_cu_count() {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<int> cached[kMaxDevices];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= kMaxDevices) dev = 0;
+  int c = cached[dev].load(std::memory_order_relaxed);
+  if (c <= 0) {
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) {
+      (void)hipGetLastError();
+      c = 256;
+    }
+    cached[dev].store(c, std::memory_order_relaxed);
+  }
+  return c;
+}
+
 bool is_device_ptr(const void* p) {
   if (p == nullptr) return false;
   hipPointerAttribute_t attr;
@@ -256,7 +273,7 @@ static hipEvent_t prof_event() {
   return e;
 }
 
-ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
+ProfScope::ProfScope(const char* n, hipStream_t s, bool att) : name(n), stream(s), attach(att) {
   if (!g_prof_on.load(std::memory_order_relaxed)) return;
   {
     std::lock_guard<std::mutex> lk(g_prof_mu);
@@ -274,12 +291,18 @@ ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
     start = nullptr;
     return;
   }
-  (void)hipEventRecord(start, stream);
+  if (!attach) (void)hipEventRecord(start, stream);
 }
 
 ProfScope::~ProfScope() {
   if (!start) return;
-  (void)hipEventRecord(stop, stream);
+  if (attach && !attached) {  // the launch did not take the events
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_pool.push_back(start);
+    g_prof_pool.push_back(stop);
+    return;
+  }
+  if (!attach) (void)hipEventRecord(stop, stream);
   std::lock_guard<std::mutex> lk(g_prof_mu);
   g_prof.push_back({name, start, stop});
 }

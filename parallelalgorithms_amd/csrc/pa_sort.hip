@@ -31,6 +31,8 @@
 #include <utility>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "pa_sort_kernels.h"
 
 namespace pa {
@@ -345,10 +347,20 @@ struct PassVariant {
 template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool LATE, int MINW, int MODE, bool UNI, int SPLIT,
           int UL>
 int launch_pass(hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI, SPLIT, UL>), dim3(a.ntiles), dim3(BLOCK),
-                     0, s,
-                     static_cast<const K*>(a.kin), static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr,
-                     a.ntiles, a.shift, a.fmode, nullptr, nullptr, nullptr, nullptr, 0u, a.chain_info, a.chain_tiles);
+  // Timed launches carry their events in the dispatch (no extra packets).
+  ProfScope ps("k_onesweep", s, true);
+  auto kern = k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI, SPLIT, UL>;
+  if (ps.start) {
+    hipExtLaunchKernelGGL(kern, dim3(a.ntiles), dim3(BLOCK), 0, s, ps.start, ps.stop, 0, static_cast<const K*>(a.kin),
+                          static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr, a.ntiles, a.shift, a.fmode,
+                          (const uint64_t*)nullptr, (const uint64_t*)nullptr, (const uint64_t*)nullptr,
+                          (const uint32_t*)nullptr, 0u, a.chain_info, a.chain_tiles);
+    ps.attached = true;
+  } else {
+    hipLaunchKernelGGL(kern, dim3(a.ntiles), dim3(BLOCK), 0, s, static_cast<const K*>(a.kin), static_cast<K*>(a.kout),
+                       a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr, a.ntiles, a.shift, a.fmode, nullptr, nullptr, nullptr,
+                       nullptr, 0u, a.chain_info, a.chain_tiles);
+  }
   PA_HIP(hipGetLastError());
   return PA_OK;
 }
@@ -374,9 +386,11 @@ struct Variants<uint32_t, false> {
       PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 32),  PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 1, true, 2, 32),
       PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2, 32), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2, 32),
       PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2, 16), PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 16),
-      PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 16)};
-  // 512 x 16 up to 2^24 keys (4M: 0.209 -> 0.180 ms against 512 x 48; even at 16M), 512 x 48 split above.
-  static constexpr int by_class[kClasses] = {2, 2, 7, 6};
+      PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 16), PA_VARIANT(uint32_t, false, 768, 32, 8, true, 6, 0, true, 2, 16)};
+  // 512 x 16 up to 2^24 keys (4M: 0.209 -> 0.180 ms against 512 x 48; even at 16M); above, the 24,576-key
+  // split tile as 768 x 32 (12 waves per workgroup, two workgroups per CU: 100M passes 188-227 us against
+  // 194-233 us for 512 x 48).
+  static constexpr int by_class[kClasses] = {2, 2, 7, 9};
 };
 template <>
 struct Variants<uint32_t, true> {
@@ -528,7 +542,7 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
     // Measured per histogram read: 16M keys in 8K-key tiles 48 -> 31 us,
     // 100M keys in 24K-key tiles 126 -> 122 us, 1M keys unchanged.
     const size_t target = std::min<size_t>(std::max<size_t>(n / 512, tile), 49152);
-    const size_t tpb = std::max<size_t>(1, (target + tile / 2) / tile);
+    const size_t tpb = std::max<size_t>(1, std::min<size_t>((target + tile / 2) / tile, kHistTilesMaxKeys / tile));
     const size_t grid = std::max<size_t>(1, (rows + tpb - 1) / tpb);
     {
       ProfScope ps("k_hist_digits", s);
@@ -556,7 +570,6 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
                lb + (size_t)p * rows * 256, ctr + p, (uint32_t)rows, (uint32_t)(8 * p), fmode,
                var.chained ? chain_info + (size_t)p * 36 : nullptr,
                var.chained ? chain_tiles + (size_t)p * rows : nullptr};
-    ProfScope ps("k_onesweep", s);
     PA_TRY(var.launch(s, a));
     std::swap(kin, kout);
     std::swap(vin, vout);

@@ -280,39 +280,49 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   const uint32_t tile_first = local * (uint32_t)TILE;
   const uint32_t valid = min((uint32_t)TILE, len - tile_first);
 
-  // Keys into registers, all-ones padding past the array's end.
+  // Keys into registers through a buffer resource over the tile (one offset
+  // register, immediate offsets per item).  Slots past the array's end read 0
+  // from the range check and become padding: all-ones once transformed, so
+  // they rank after every real key.
   K key[ITEMS];
   {
-    const uint32_t first = tile_first + unit * USPAN + ul;
-    const K* src = in + first;
-    if (valid == (uint32_t)TILE) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<K*>(in) + tile_first, (short)0,
+                                                                         (int)(valid * sizeof(K)), 0x00020000);
+    const uint32_t voff = (unit * USPAN + ul) * (uint32_t)sizeof(K);
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) key[i] = src[i * UL];
-      if (float_in) {
+    for (int i = 0; i < ITEMS; ++i) key[i] = dev::buffer_load<K>(rs, voff + i * UL * (uint32_t)sizeof(K));
+    if (valid < (uint32_t)TILE) {
+      const K pad = float_in ? dev::key_inv<K>(~K(0), xform) : ~K(0);
+      // Item i is padding when i * UL >= lim (one register, not one per item).
+      int lim = (int)valid - (int)(unit * USPAN + ul);
+      asm volatile("" : "+v"(lim));
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) key[i] = dev::key_fwd<K>(key[i], xform);
-      }
-    } else {
+      for (int i = 0; i < ITEMS; ++i)
+        if (i * UL >= lim) key[i] = pad;
+    }
+    if (float_in) {
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) {
-        K v = ~K(0);
-        if (first + i * UL < len) {
-          v = src[i * UL];
-          if (float_in) v = dev::key_fwd<K>(v, xform);
-        }
-        key[i] = v;
-      }
+      for (int i = 0; i < ITEMS; ++i) key[i] = dev::key_fwd<K>(key[i], xform);
     }
   }
   uint32_t val[KV ? ITEMS : 1];
   if constexpr (KV) {
-    const uint32_t first = tile_first + unit * USPAN + ul;
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(vsrc) + tile_first,
+                                                                         (short)0, (int)(valid * 4u), 0x00020000);
+    const uint32_t voff = (unit * USPAN + ul) * 4u;
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t idx = first + i * UL;
-      val[i] = idx < len ? vsrc[idx] : 0u;
-    }
+    for (int i = 0; i < ITEMS; ++i) val[i] = __builtin_amdgcn_raw_buffer_load_b32(rv, voff + i * UL * 4u, 0, 0);
   }
+  // Stores of 4-byte keys and payloads go through buffer resources over the
+  // output (32-bit offsets; n < 2^30 keeps the byte range below 2^32).
+  const __amdgpu_buffer_rsrc_t rs_out =
+      __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)(sizeof(K) == 4 ? len * 4u : 0u), 0x00020000);
+  auto store_key = [&](uint32_t pos, K v) {
+    if constexpr (sizeof(K) == 4)
+      __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, rs_out, pos * 4u, 0, 0);
+    else
+      out[pos] = v;
+  };
 
   // Stable rank of every key within its unit.
   // Ranks (later staging slots) are below TILE <= 2^16; split tiles keep two
@@ -513,11 +523,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       }
       if (valid == (uint32_t)TILE) {
 #pragma unroll
-        for (int i = 0; i < G; ++i) out[gp[i]] = wk[i];
+        for (int i = 0; i < G; ++i) store_key(gp[i], wk[i]);
       } else {
 #pragma unroll
         for (int i = 0; i < G; ++i)
-          if (r * STAGE + tid + (g0 + i) * BLOCK < valid) out[gp[i]] = wk[i];
+          if (r * STAGE + tid + (g0 + i) * BLOCK < valid) store_key(gp[i], wk[i]);
       }
     }
   };
@@ -571,6 +581,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 // counts to rows[t][256] and accumulate all D digit histograms into counts
 // [D][256].  Also clears the look-back rows of passes 1..D-1 (zero_vec
 // 16-byte words at zero_ptr).
+//
+// Counting uses lane-private sub-histograms (SubHist, packed 16-bit
+// counters): every key costs D conflict-free LDS adds.  A block counts at
+// most kHistTilesMaxKeys keys, so no packed counter can overflow (a copy of a
+// bin sees at most 1/C of the block's keys).
+constexpr uint32_t kHistTilesMaxKeys = 1u << 19;
+
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__ keys, size_t n, uint32_t tile,
                                                            uint32_t* __restrict__ rows, uint32_t* __restrict__ counts,
@@ -578,63 +595,78 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__
                                                            uint32_t xform) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
-  __shared__ uint32_t s_hist[D * 256];  // row 0: digit 0 over the block's finished tiles
-  __shared__ uint32_t s_tile[256];
+  constexpr int C = D == 4 ? 32 : 16;  // copies: 4 x 16 KB (u32) or 8 x 8 KB (u64) of LDS
+  using H = dev::SubHist<C, true>;
+  constexpr uint32_t W = H::words(256);  // words per digit
+  __shared__ __align__(16) uint32_t s_h[D * W];
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < D * 256; i += kSortBlock) s_hist[i] = 0;
-  if (tid < 256) s_tile[tid] = 0;
+  const uint32_t lane = tid & 63;
+  for (uint32_t i = tid; i < D * W / 4; i += kSortBlock) reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0, 0, 0, 0);
   const size_t gstride = (size_t)gridDim.x * kSortBlock;
   for (size_t i = (size_t)blockIdx.x * kSortBlock + tid; i < zero_vec; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
   const bool aligned = ((uintptr_t)keys & 15) == 0 && (tile % PER) == 0;
   const size_t ntiles = (n + tile - 1) / tile;
+  auto count = [&](const K (&k)[PER]) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      uint32_t dg[PER];
+#pragma unroll
+      for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(k[j], 8 * d);
+      H::template add_all<PER>(s_h + d * W, dg, lane);
+    }
+  };
+  uint32_t acc0 = 0;  // thread b < 256: digit-0 count of bin b over the block's tiles
   for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const size_t lo = t * tile, hi = min(lo + tile, n);
     size_t scalar_from = lo;
     if (aligned) {
-      const size_t nvec = (hi - lo) / PER;
+      const uint32_t nvec = (uint32_t)((hi - lo) / PER);
       const uint4* vk = reinterpret_cast<const uint4*>(keys + lo);
-#pragma unroll 4
-      for (size_t v = tid; v < nvec; v += kSortBlock) {
-        const uint4 x = vk[v];
-        K k[PER];
-        __builtin_memcpy(k, &x, 16);
-        if (xform) {
+      // U 16-byte loads per thread in flight before any counting.
+      constexpr int U = 4;
+      for (uint32_t v0 = tid; v0 < nvec; v0 += U * kSortBlock) {
+        uint4 x[U];
 #pragma unroll
-          for (int j = 0; j < PER; ++j) k[j] = dev::key_fwd<K>(k[j], xform);
-        }
-        uint32_t dg[PER];
+        for (int u = 0; u < U; ++u)
+          if (v0 + u * kSortBlock < nvec) x[u] = vk[v0 + u * kSortBlock];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(k[j], 0);
-        dev::lds_count<PER>(s_tile, dg);
+        for (int u = 0; u < U; ++u) {
+          if (v0 + u * kSortBlock >= nvec) continue;
+          K k[PER];
+          __builtin_memcpy(k, &x[u], 16);
+          if (xform) {
 #pragma unroll
-        for (int d = 1; d < D; ++d) {
-#pragma unroll
-          for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(k[j], 8 * d);
-          dev::lds_count<PER>(s_hist + d * 256, dg);
+            for (int j = 0; j < PER; ++j) k[j] = dev::key_fwd<K>(k[j], xform);
+          }
+          count(k);
         }
       }
-      scalar_from = lo + nvec * PER;
+      scalar_from = lo + (size_t)nvec * PER;
     }
     for (size_t i = scalar_from + tid; i < hi; i += kSortBlock) {
       K kk = keys[i];
       if (xform) kk = dev::key_fwd<K>(kk, xform);
-      atomicAdd(&s_tile[dev::digit8<K>(kk, 0)], 1u);
 #pragma unroll
-      for (int d = 1; d < D; ++d) atomicAdd(&s_hist[d * 256 + dev::digit8<K>(kk, 8 * d)], 1u);
+      for (int d = 0; d < D; ++d) H::add(s_h + d * W, dev::digit8<K>(kk, 8 * d), lane, 1u);
     }
     __syncthreads();
     if (tid < 256) {
-      const uint32_t c = s_tile[tid];
+      const uint32_t c = H::sum(s_h, tid);
       rows[t * 256 + tid] = c;
-      s_hist[tid] += c;
-      s_tile[tid] = 0;
+      acc0 += c;
     }
     __syncthreads();
+    for (uint32_t i = tid; i < W / 4; i += kSortBlock) reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
   }
-  for (uint32_t i = tid; i < D * 256; i += kSortBlock) {
-    const uint32_t c = s_hist[i];
-    if (c) atomicAdd(&counts[i], c);
+  if (tid < 256) {
+    if (acc0) atomicAdd(&counts[tid], acc0);
+#pragma unroll
+    for (int d = 1; d < D; ++d) {
+      const uint32_t c = H::sum(s_h + d * W, tid);
+      if (c) atomicAdd(&counts[d * 256 + tid], c);
+    }
   }
 }
 

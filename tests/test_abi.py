@@ -45,14 +45,15 @@ def test_argument_errors_need_no_device():
 def test_size_class_choice_needs_no_device(monkeypatch):
     """Built-in pass shape per size class: the 8K-key tile up to 2^24 keys
     (32-lane ranking units up to 2^20, 16-lane above), the split 24K-key tile
-    with 16-lane ranking units above (unless PA_SORT_VARIANT forces one)."""
+    as 768 x 32 with 16-lane ranking units above (unless PA_SORT_VARIANT
+    forces one)."""
     import os
     if os.environ.get("PA_SORT_VARIANT"):
         pytest.skip("PA_SORT_VARIANT forces one variant")
     lib = P.load()
     assert lib.pa_autotune_reset() == 0
     assert [lib.pa_sort_variant_u32(n) for n in (1, 1 << 16, (1 << 16) + 1, 1 << 20)] == [2, 2, 2, 2]
-    assert [lib.pa_sort_variant_u32(n) for n in ((1 << 20) + 1, 1 << 24, (1 << 24) + 1, 100_000_000)] == [7, 7, 6, 6]
+    assert [lib.pa_sort_variant_u32(n) for n in ((1 << 20) + 1, 1 << 24, (1 << 24) + 1, 100_000_000)] == [7, 7, 9, 9]
     assert lib.pa_autotune_sort_u32(1, None) == 1  # needs max_n >= 2; rejected before any HIP call
     assert lib.pa_segmented_plan_create_u32(None, None, 0) is None
 
