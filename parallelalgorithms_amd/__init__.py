@@ -161,11 +161,37 @@ def numel(x):
     return x.numel() if hasattr(x, "numel") else x.size
 
 
+def _check(x, itemsize, name, need=None):
+    """Raises ValueError unless x (torch tensor or numpy array; None passes)
+    has elements of itemsize bytes, is contiguous and holds >= need elements.
+    The library takes raw pointers and counts: a wrong element size, a strided
+    view or a short buffer would make its kernels read or write the wrong
+    memory, so they are refused here before any call."""
+    if x is None:
+        return
+    if hasattr(x, "element_size"):
+        size, contig = x.element_size(), x.is_contiguous()
+    elif hasattr(x, "itemsize"):
+        size, contig = x.itemsize, bool(x.flags["C_CONTIGUOUS"])
+    else:
+        raise TypeError(f"{name}: expected a torch tensor or a numpy array, got {type(x)}")
+    if size != itemsize:
+        raise ValueError(f"{name}: elements must be {itemsize} bytes, got {size}")
+    if not contig:
+        raise ValueError(f"{name}: the array must be contiguous")
+    if need is not None and numel(x) < need:
+        raise ValueError(f"{name}: needs at least {need} elements, got {numel(x)}")
+
+
 def sort_u32(keys, tmp=None, stream=None):
+    _check(keys, 4, "keys")
+    _check(tmp, 4, "tmp", numel(keys))
     call("pa_sort_u32", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
 
 def sort_two_phase_u32(a, b, stream=None):
+    _check(a, 4, "a")
+    _check(b, 4, "b", numel(a))
     call("pa_sort_two_phase_u32", _ptr(a), _ptr(b), numel(a), _stream(stream))
 
 
@@ -182,14 +208,19 @@ def autotune_reset():
 
 
 def sort_u64(keys, tmp=None, stream=None):
+    _check(keys, 8, "keys")
+    _check(tmp, 8, "tmp", numel(keys))
     call("pa_sort_u64", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
 
 def sort_f32(keys, tmp=None, stream=None):
+    _check(keys, 4, "keys")
+    _check(tmp, 4, "tmp", numel(keys))
     call("pa_sort_f32", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
 
 def counting_sort_u8(data, stream=None):
+    _check(data, 1, "data")
     call("pa_counting_sort_u8", _ptr(data), numel(data), _stream(stream))
 
 
@@ -205,89 +236,137 @@ def fill(a, value, stream=None):
 
 
 def sort_f64(keys, tmp=None, stream=None):
+    _check(keys, 8, "keys")
+    _check(tmp, 8, "tmp", numel(keys))
     call("pa_sort_f64", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
 
 def sort_i32(keys, tmp=None, stream=None):
+    _check(keys, 4, "keys")
+    _check(tmp, 4, "tmp", numel(keys))
     call("pa_sort_i32", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
 
 def sort_i64(keys, tmp=None, stream=None):
+    _check(keys, 8, "keys")
+    _check(tmp, 8, "tmp", numel(keys))
     call("pa_sort_i64", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
 
 def sort_kv_u32_u32(keys, vals, ktmp=None, vtmp=None, stream=None):
+    _check(keys, 4, "keys")
+    _check(vals, 4, "vals", numel(keys))
+    _check(ktmp, 4, "ktmp", numel(keys))
+    _check(vtmp, 4, "vtmp", numel(keys))
     call("pa_sort_kv_u32_u32", _ptr(keys), _ptr(vals), _ptr(ktmp), _ptr(vtmp), numel(keys), _stream(stream))
 
 
 def sort_kv_u64_u32(keys, vals, ktmp=None, vtmp=None, stream=None):
+    _check(keys, 8, "keys")
+    _check(vals, 4, "vals", numel(keys))
+    _check(ktmp, 8, "ktmp", numel(keys))
+    _check(vtmp, 4, "vtmp", numel(keys))
     call("pa_sort_kv_u64_u32", _ptr(keys), _ptr(vals), _ptr(ktmp), _ptr(vtmp), numel(keys), _stream(stream))
 
 
 def hist_digits_u32(keys, counts, stream=None):
+    _check(keys, 4, "keys")
+    _check(counts, 8, "counts", 4 * 256)
     call("pa_hist_digits_u32", _ptr(keys), numel(keys), _ptr(counts), _stream(stream))
 
 
 def hist_u32(keys, shift, bits, counts, stream=None):
+    _check(keys, 4, "keys")
+    _check(counts, 8, "counts", 1 << min(max(int(bits), 0), 12))
     call("pa_hist_u32", _ptr(keys), numel(keys), shift, bits, _ptr(counts), _stream(stream))
 
 
 def hist_u64(keys, shift, bits, counts, stream=None):
+    _check(keys, 8, "keys")
+    _check(counts, 8, "counts", 1 << min(max(int(bits), 0), 12))
     call("pa_hist_u64", _ptr(keys), numel(keys), shift, bits, _ptr(counts), _stream(stream))
 
 
 def hist_quanta_u32(keys, first, last, quantum, nq, which_byte, counts, stream=None):
+    _check(keys, 4, "keys", last)
+    _check(counts, 8, "counts", nq * 256)
     call("pa_hist_quanta_u32", _ptr(keys), first, last, quantum, nq, which_byte, _ptr(counts), _stream(stream))
 
 
 def start_of_bins_u32(keys, quantum, nq, digit, out, stream=None):
+    _check(keys, 4, "keys")
+    _check(out, 8, "out", nq * 256)
     call("pa_start_of_bins_u32", _ptr(keys), numel(keys), quantum, nq, digit, _ptr(out), _stream(stream))
 
 
 def hist_u8(data, counts, stream=None):
+    _check(data, 1, "data")
+    _check(counts, 8, "counts", 256)
     call("pa_hist_u8", _ptr(data), numel(data), _ptr(counts), _stream(stream))
 
 
 def select_u32(keys, k, result, stream=None):
+    _check(keys, 4, "keys")
+    _check(result, 4, "result", 1)
     call("pa_select_u32", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
 
 
 def select_u64(keys, k, result, stream=None):
+    _check(keys, 8, "keys")
+    _check(result, 8, "result", 1)
     call("pa_select_u64", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
 
 
 def partition_u32(keys, k, result, stream=None):
+    _check(keys, 4, "keys")
+    _check(result, 4, "result", 1)
     call("pa_partition_u32", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
 
 
 def partition_multi_u32(keys, ks, stream=None):
     import numpy as np
+    _check(keys, 4, "keys")
     ks = np.ascontiguousarray(ks, dtype=np.uint64)
     call("pa_partition_multi_u32", _ptr(keys), numel(keys), ks.ctypes.data, ks.size, _stream(stream))
 
 
 def select_radix_u32(keys, k, result, stream=None):
+    _check(keys, 4, "keys")
+    _check(result, 4, "result", 1)
     call("pa_select_radix_u32", _ptr(keys), numel(keys), k, _ptr(result), _stream(stream))
 
 
 def merge_u32(a, b, out, stream=None):
+    _check(a, 4, "a")
+    _check(b, 4, "b")
+    _check(out, 4, "out", numel(a) + numel(b))
     call("pa_merge_u32", _ptr(a), numel(a), _ptr(b), numel(b), _ptr(out), _stream(stream))
 
 
 def merge_u64(a, b, out, stream=None):
+    _check(a, 8, "a")
+    _check(b, 8, "b")
+    _check(out, 8, "out", numel(a) + numel(b))
     call("pa_merge_u64", _ptr(a), numel(a), _ptr(b), numel(b), _ptr(out), _stream(stream))
 
 
 def merge_inplace_u32(t, n1, stream=None):
+    _check(t, 4, "t", n1)
     call("pa_merge_inplace_u32", _ptr(t), n1, numel(t) - n1, _stream(stream))
 
 
 def merge_inplace_u64(t, n1, stream=None):
+    _check(t, 8, "t", n1)
     call("pa_merge_inplace_u64", _ptr(t), n1, numel(t) - n1, _stream(stream))
 
 
 def segmented_sort_u32(keys, tmp, offsets, stream=None):
     """offsets: host numpy uint64 array of num_segments + 1 entries."""
+    _check(keys, 4, "keys")
+    _check(tmp, 4, "tmp", numel(keys))
+    _check(offsets, 8, "offsets")
+    if len(offsets) and int(offsets[-1]) > numel(keys):
+        raise ValueError(f"offsets: the last offset {int(offsets[-1])} is past the {numel(keys)} keys")
     call("pa_segmented_sort_u32", _ptr(keys), _ptr(tmp), _ptr(offsets), len(offsets) - 1, _stream(stream))
 
 
@@ -380,6 +459,8 @@ class Server:
 
 
 def copy_u32(dst, src, stream=None):
+    _check(src, 4, "src")
+    _check(dst, 4, "dst", numel(src))
     call("pa_copy_u32", _ptr(dst), _ptr(src), numel(src), _stream(stream))
 
 
@@ -408,10 +489,12 @@ def profile_query(prefix):
 
 
 def fill_random_u32(dst, seed, stream=None):
+    _check(dst, 4, "dst")
     call("pa_fill_random_u32", _ptr(dst), numel(dst), seed, _stream(stream))
 
 
 def fill_random_u64(dst, seed, stream=None):
+    _check(dst, 8, "dst")
     call("pa_fill_random_u64", _ptr(dst), numel(dst), seed, _stream(stream))
 
 

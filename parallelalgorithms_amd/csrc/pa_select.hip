@@ -562,8 +562,8 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
   if (!dres) PA_HIP(hipMemcpyAsync(result, dres_tmp, sizeof(K), hipMemcpyDeviceToHost, s));
   if (!dres || !dk) PA_HIP(hipStreamSynchronize(s));
   if (place && !dk) {  // host keys: store the value into the caller's array
-    if (dres)
-      PA_HIP(hipMemcpy(place, dres_tmp, sizeof(K), hipMemcpyDeviceToHost));
+    if (dres)  // the value went to the caller's device result
+      PA_HIP(hipMemcpy(place, result, sizeof(K), hipMemcpyDeviceToHost));
     else
       *place = *result;
   }

@@ -60,6 +60,10 @@ struct Server {
   std::deque<SJob*> pending;
   std::unordered_map<uint64_t, SJob*> jobs;
   std::vector<SUnit*> inflight;
+  // Units taken off `pending` whose launch is not yet in `inflight` (the
+  // dispatcher launches with the mutex released): drain and the completer's
+  // exit wait for them too.
+  int dispatching = 0;
   std::thread dispatcher, completer;
   bool stop = false;
   int device = 0;
@@ -136,6 +140,7 @@ static void dispatch_loop(Server* S) {
     S->peak = std::max(S->peak, S->in_use);
     u->stream = (int)(std::min_element(S->busy.begin(), S->busy.end()) - S->busy.begin());
     S->busy[u->stream]++;
+    S->dispatching++;
     hipStream_t s = S->streams[u->stream];
     lk.unlock();
 
@@ -165,6 +170,7 @@ static void dispatch_loop(Server* S) {
       S->st.coalesced_jobs += u->jobs.size();
     }
     S->inflight.push_back(u);
+    S->dispatching--;
     S->cv_done.notify_all();  // wakes the completer
   }
 }
@@ -174,7 +180,7 @@ static void complete_loop(Server* S) {
   std::unique_lock<std::mutex> lk(S->mu);
   for (;;) {
     S->cv_done.wait(lk, [&] { return S->stop || !S->inflight.empty(); });
-    if (S->stop && S->inflight.empty() && S->pending.empty()) return;
+    if (S->stop && S->inflight.empty() && S->pending.empty() && S->dispatching == 0) return;
     std::vector<SUnit*> units = S->inflight;
     lk.unlock();
     std::vector<SUnit*> finished;
@@ -284,7 +290,7 @@ int pa_server_drain(pa_server* srv) {
   Server* S = reinterpret_cast<Server*>(srv);
   if (!S) return arg_error("pa_server_drain: null server");
   std::unique_lock<std::mutex> lk(S->mu);
-  S->cv_done.wait(lk, [&] { return S->pending.empty() && S->inflight.empty(); });
+  S->cv_done.wait(lk, [&] { return S->pending.empty() && S->inflight.empty() && S->dispatching == 0; });
   return PA_OK;
 }
 

@@ -65,8 +65,9 @@ class Job:
         if self.op == "select" and not 0 <= self.k < self.n:
             raise ValueError(f"job {self.id}: select needs 0 <= k < n")
         width = 32 if self.dtype == "u32" else 64
-        if self.op == "hist" and not (1 <= self.bits <= 16 and 0 <= self.shift and self.shift + self.bits <= width):
-            raise ValueError(f"job {self.id}: hist needs 1 <= bits <= 16 and shift + bits <= {width}")
+        # pa_hist_u32 / pa_hist_u64 take 1..12 bits (at most 4096 bins).
+        if self.op == "hist" and not (1 <= self.bits <= 12 and 0 <= self.shift and self.shift + self.bits <= width):
+            raise ValueError(f"job {self.id}: hist needs 1 <= bits <= 12 and shift + bits <= {width}")
         return self
 
     def to_json(self):

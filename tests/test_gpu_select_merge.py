@@ -56,6 +56,23 @@ def test_select_device_result_and_host_keys(pa, orc):
     assert int(res[0]) == expect
 
 
+def test_select_radix_host_keys_device_result(pa, orc):
+    """SelectRadix's post-condition a[k] == k-th smallest with the keys in
+    host memory and the result in device memory: the value reaches both."""
+    n = 200_003
+    a = make_u32("uniform", n, seed=22)
+    for k in (0, n // 2, n - 1):
+        expect, _ = orc.select_radix_u32(a, k)
+        h = a.copy()
+        r = dev_empty(1, np.uint32)
+        pa.select_radix_u32(h, k, r)
+        assert int(to_host(r, np.uint32)[0]) == expect, k
+        assert int(h[k]) == expect, k
+        mask = np.ones(n, dtype=bool)
+        mask[k] = False
+        np.testing.assert_array_equal(h[mask], a[mask])  # the other keys untouched
+
+
 def test_select_rejects_bad_k(pa):
     import parallelalgorithms_amd as P
     a = to_dev(np.arange(10, dtype=np.uint32))

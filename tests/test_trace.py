@@ -22,7 +22,7 @@ def test_job_round_trip(tmp_path):
 
 @pytest.mark.parametrize("bad,msg", [
     ({"op": "scan"}, "op"), ({"dtype": "u16"}, "dtype"), ({"n": 0}, "n must"), ({"t_us": -1.0}, "t_us"),
-    ({"op": "select", "k": 10}, "k < n"), ({"op": "hist", "bits": 0}, "bits"),
+    ({"op": "select", "k": 10}, "k < n"), ({"op": "hist", "bits": 0}, "bits"), ({"op": "hist", "bits": 13}, "bits <= 12"),
     ({"op": "hist", "shift": 28, "bits": 8}, r"shift \+ bits"), ({"extra": 1}, "unknown job fields"),
 ])
 def test_job_validation(bad, msg):
