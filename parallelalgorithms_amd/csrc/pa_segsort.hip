@@ -538,7 +538,7 @@ static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStre
                          dim3((unsigned)nt), dim3(kSegBlock), 0, s, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                          (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u, counts + p * 256,
                          lbp + (size_t)p * nt * 256, ctr + p, (uint32_t)nt, (uint32_t)(8 * p), 0u, d_ti, tin, tout, d_ll,
-                         4u, d_lr, (const uint32_t*)nullptr);
+                         4u, d_lr);
     }
   }
   PA_HIP(hipGetLastError());

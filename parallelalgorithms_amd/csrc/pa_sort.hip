@@ -121,205 +121,6 @@ __global__ __launch_bounds__(kPrefixThreads) void k_tile_scan(uint32_t* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Chained passes: per-chain digit counts, bases and the interleaved claim order
-// ---------------------------------------------------------------------------
-// Counts row p (p < D) is [kChains][256]: chain-major counts of digit p, where
-// the chain of a key is its position >> log2s for p = 0 (contiguous position
-// ranges of the input) and digit p-1 >> kChainShift for p >= 1.  Pass p's input
-// is sorted by digit p-1, so the keys of value chain c are the contiguous range
-// of that input holding digits [32c, 32c + 32) of digit p-1.  Every row depends
-// only on the multiset of keys, so one read of the input yields all of them.
-template <typename K>
-__global__ __launch_bounds__(kSortBlock) void k_hist_chains(const K* __restrict__ keys, size_t n, uint32_t log2s,
-                                                            uint32_t* __restrict__ partial,
-                                                            uint4* __restrict__ zero_ptr, size_t zero_vec,
-                                                            uint32_t xform) {
-  constexpr int D = sizeof(K);
-  constexpr int PER = 16 / sizeof(K);
-  constexpr int ROW = kChains * 256;
-  __shared__ uint32_t s_hist[D * ROW];
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < D * ROW; i += kSortBlock) s_hist[i] = 0;
-  const size_t gstride = (size_t)gridDim.x * kSortBlock;
-  for (size_t i = (size_t)blockIdx.x * kSortBlock + tid; i < zero_vec; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
-  __syncthreads();
-
-  auto count = [&](const K (&k)[PER], const size_t (&pos)[PER]) {
-    uint32_t idx[PER];
-#pragma unroll
-    for (int j = 0; j < PER; ++j) idx[j] = (uint32_t)(pos[j] >> log2s) * 256 + dev::digit8<K>(k[j], 0);
-    dev::lds_count<PER>(s_hist, idx);
-#pragma unroll
-    for (int d = 1; d < D; ++d) {
-#pragma unroll
-      for (int j = 0; j < PER; ++j)
-        idx[j] = (dev::digit8<K>(k[j], 8 * (d - 1)) >> kChainShift) * 256 + dev::digit8<K>(k[j], 8 * d);
-      dev::lds_count<PER>(s_hist + d * ROW, idx);
-    }
-  };
-
-  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
-  if (head > n) head = n;
-  const size_t nvec = (n - head) / PER;
-  const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
-  constexpr int U = 4;  // 16-byte loads in flight per thread
-  const size_t v0 = (size_t)blockIdx.x * kSortBlock + tid;
-  for (size_t base = 0; base < nvec; base += gstride * U) {
-    uint4 x[U];
-    bool ok[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const size_t v = base + u * gstride + v0;
-      ok[u] = v < nvec;
-      if (ok[u]) x[u] = vk[v];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (!ok[u]) continue;
-      K k[PER];
-      size_t pos[PER];
-      __builtin_memcpy(k, &x[u], 16);
-      const size_t p0 = head + (base + u * gstride + v0) * PER;
-#pragma unroll
-      for (int j = 0; j < PER; ++j) {
-        pos[j] = p0 + j;
-        if (xform) k[j] = dev::key_fwd<K>(k[j], xform);
-      }
-      count(k, pos);
-    }
-  }
-  if (blockIdx.x == 0) {  // head and tail elements outside whole vectors
-    const size_t tail_start = head + nvec * PER;
-    const size_t extra = head + (n - tail_start);
-    for (size_t e = tid; e < extra; e += kSortBlock) {
-      const size_t idx = e < head ? e : tail_start + (e - head);
-      K kk = keys[idx];
-      if (xform) kk = dev::key_fwd<K>(kk, xform);
-      atomicAdd(&s_hist[(uint32_t)(idx >> log2s) * 256 + dev::digit8<K>(kk, 0)], 1u);
-#pragma unroll
-      for (int d = 1; d < D; ++d)
-        atomicAdd(&s_hist[d * ROW + (dev::digit8<K>(kk, 8 * (d - 1)) >> kChainShift) * 256 + dev::digit8<K>(kk, 8 * d)],
-                  1u);
-    }
-  }
-  __syncthreads();
-  uint32_t* out = partial + (size_t)blockIdx.x * D * ROW;
-  for (uint32_t i = tid; i < D * ROW; i += kSortBlock) out[i] = s_hist[i];
-}
-
-// counts[i] = sum over the histogram blocks of partial[b][i]; blockIdx.y takes
-// a share of the blocks and adds it atomically.
-__global__ __launch_bounds__(256) void k_reduce_counts(const uint32_t* __restrict__ partial, uint32_t nparts,
-                                                       uint32_t ncounts, uint32_t* __restrict__ counts) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= ncounts) return;
-  const uint32_t per = (nparts + gridDim.y - 1) / gridDim.y;
-  const uint32_t b0 = blockIdx.y * per, b1 = min(nparts, b0 + per);
-  uint32_t sum = 0;
-  for (uint32_t b = b0; b < b1; ++b) sum += partial[(size_t)b * ncounts + i];
-  if (sum) atomicAdd(&counts[i], sum);
-}
-
-// One block per pass p: chain bases, chain ranges and the claim order.
-//   bases[p][c][d] = G_p[d] + sum_{c' < c} counts[p][c'][d], where G_p[d] is the
-//   number of keys whose digit p is below d: chain c's keys of digit d follow
-//   those of every earlier chain (pass p's input holds the chains in order and
-//   the pass is stable).
-//   info[p][4c..4c+2] = (first key, keys, first look-back row) of chain c;
-//   info[p][32] = tiles in the pass.
-//   tiles[p][r] = (c << 24) | k: tile k of chain c is claimed r-th, in the order
-//   of (2k + 1) / (2 T_c) (T_c = tiles of chain c), ties by chain, so every
-//   chain's tiles are spread evenly over the pass.
-__global__ __launch_bounds__(1024) void k_chain_plan(const uint32_t* __restrict__ counts, uint32_t n, uint32_t log2s,
-                                                     uint32_t tile, uint32_t* __restrict__ bases,
-                                                     uint32_t* __restrict__ info, uint32_t* __restrict__ tiles,
-                                                     uint32_t max_tiles) {
-  constexpr uint32_t ROW = kChains * 256;
-  __shared__ uint32_t s_prev[257];  // exclusive digit starts of digit p-1, [256] = n
-  __shared__ uint32_t s_wsum[2][4];
-  __shared__ uint32_t s_T[kChains], s_rb[kChains + 1];
-  const uint32_t p = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t* cnt = counts + (size_t)p * ROW;
-  uint32_t tot = 0, tot_prev = 0, inc = 0, inc_prev = 0;
-  if (tid < 256) {
-    for (int c = 0; c < kChains; ++c) tot += cnt[c * 256 + tid];
-    if (p > 0) {
-      const uint32_t* prev = counts + (size_t)(p - 1) * ROW;
-      for (int c = 0; c < kChains; ++c) tot_prev += prev[c * 256 + tid];
-    }
-    inc = dev::wave_incl_scan(tot, lane);
-    inc_prev = dev::wave_incl_scan(tot_prev, lane);
-    if (lane == 63) {
-      s_wsum[0][wave] = inc;
-      s_wsum[1][wave] = inc_prev;
-    }
-  }
-  __syncthreads();
-  if (tid < 256) {
-    uint32_t pre = 0, pre_prev = 0;
-    for (uint32_t w = 0; w < wave; ++w) {
-      pre += s_wsum[0][w];
-      pre_prev += s_wsum[1][w];
-    }
-    uint32_t b = pre + inc - tot;  // G_p[tid]
-    for (int c = 0; c < kChains; ++c) {
-      bases[(size_t)p * ROW + c * 256 + tid] = b;
-      b += cnt[c * 256 + tid];
-    }
-    s_prev[tid] = pre_prev + inc_prev - tot_prev;
-    if (tid == 0) s_prev[256] = (uint32_t)n;
-  }
-  __syncthreads();
-  if (tid < kChains) {
-    uint32_t first, len;
-    if (p == 0) {
-      const uint64_t f = (uint64_t)tid << log2s;
-      const uint64_t cap = (uint64_t)1 << log2s;
-      first = f < n ? (uint32_t)f : n;
-      len = (uint64_t)n - first < cap ? n - first : (uint32_t)cap;
-    } else {
-      first = s_prev[tid << kChainShift];
-      len = s_prev[(tid + 1) << kChainShift] - first;
-    }
-    info[(size_t)p * 36 + 4 * tid] = first;
-    info[(size_t)p * 36 + 4 * tid + 1] = len;
-    s_T[tid] = (len + tile - 1) / tile;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t r = 0;
-    for (int c = 0; c < kChains; ++c) {
-      s_rb[c] = r;
-      info[(size_t)p * 36 + 4 * c + 2] = r;
-      info[(size_t)p * 36 + 4 * c + 3] = 0;
-      r += s_T[c];
-    }
-    s_rb[kChains] = r;
-    info[(size_t)p * 36 + 32] = r;
-  }
-  __syncthreads();
-  const uint32_t total = s_rb[kChains];
-  uint32_t* order = tiles + (size_t)p * max_tiles;
-  for (uint32_t t = tid; t < total; t += blockDim.x) {
-    uint32_t c = 0;
-    while (t >= s_rb[c + 1]) ++c;
-    const uint32_t k = t - s_rb[c];
-    const uint64_t Tc = s_T[c];
-    uint32_t rank = k;
-    for (uint32_t c2 = 0; c2 < (uint32_t)kChains; ++c2) {
-      if (c2 == c || s_T[c2] == 0) continue;
-      // tiles k2 of chain c2 with (2k2 + 1) * Tc < (2k + 1) * T_c2, plus the
-      // tie (2k2 + 1) * Tc == (2k + 1) * T_c2 when c2 < c
-      const uint64_t X = (uint64_t)(2 * k + 1) * s_T[c2];
-      uint64_t before = ((X + Tc - 1) / Tc) / 2;
-      if (c2 < c && X % Tc == 0 && ((X / Tc) & 1)) ++before;
-      rank += before < s_T[c2] ? (uint32_t)before : s_T[c2];
-    }
-    order[rank] = (c << 24) | k;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Pass variants (tile shape, look-back width, persistent blocks), selectable
 // at run time; PA_SORT_VARIANT overrides the default (entry 0).
 // ---------------------------------------------------------------------------
@@ -332,15 +133,13 @@ struct PassArgs {
   const uint32_t* gtab;
   uint32_t* lb;
   uint32_t* ctr;
-  uint32_t ntiles;  // plain: tiles; chained: launch bound (tiles + kChains)
+  uint32_t ntiles;
   uint32_t shift;
   uint32_t fmode;
-  const uint32_t* chain_info;
-  const uint32_t* chain_tiles;
 };
 
 struct PassVariant {
-  int block, items, lbw, late, minw, chained;
+  int block, items, lbw, late, minw;
   int (*launch)(hipStream_t s, const PassArgs& a);
 };
 
@@ -354,21 +153,21 @@ int launch_pass(hipStream_t s, const PassArgs& a) {
     hipExtLaunchKernelGGL(kern, dim3(a.ntiles), dim3(BLOCK), 0, s, ps.start, ps.stop, 0, static_cast<const K*>(a.kin),
                           static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr, a.ntiles, a.shift, a.fmode,
                           (const uint64_t*)nullptr, (const uint64_t*)nullptr, (const uint64_t*)nullptr,
-                          (const uint32_t*)nullptr, 0u, a.chain_info, a.chain_tiles);
+                          (const uint32_t*)nullptr, 0u, (const uint32_t*)nullptr);
     ps.attached = true;
   } else {
     hipLaunchKernelGGL(kern, dim3(a.ntiles), dim3(BLOCK), 0, s, static_cast<const K*>(a.kin), static_cast<K*>(a.kout),
                        a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr, a.ntiles, a.shift, a.fmode, nullptr, nullptr, nullptr,
-                       nullptr, 0u, a.chain_info, a.chain_tiles);
+                       nullptr, 0u, nullptr);
   }
   PA_HIP(hipGetLastError());
   return PA_OK;
 }
 
-// PA_VARIANT(key, payload, block, items, look-back width, late look-back, min waves/SIMD, chained,
+// PA_VARIANT(key, payload, block, items, look-back width, late look-back, min waves/SIMD,
 //            uniform-digit fast path, staging rounds, lanes per ranking unit)
-#define PA_VARIANT(K, KV, B, I, W, P, M, C, U, S, L) \
-  { B, I, W, P, M, C, &launch_pass<K, KV, B, I, W, P, M, (C) ? kModeChain : kModePlain, U, S, L> }
+#define PA_VARIANT(K, KV, B, I, W, P, M, U, S, L) \
+  { B, I, W, P, M, &launch_pass<K, KV, B, I, W, P, M, kModePlain, U, S, L> }
 
 // Size classes of a sort call: n <= 2^16, <= 2^20, <= 2^24, larger.
 constexpr int kClasses = 4;
@@ -382,43 +181,40 @@ struct Variants;
 template <>
 struct Variants<uint32_t, false> {
   static constexpr PassVariant table[] = {
-      PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2, 32),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 32),
-      PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 32),  PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 1, true, 2, 32),
-      PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, 0, true, 2, 32), PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, 0, true, 2, 32),
-      PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, 0, true, 2, 16), PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, 0, true, 1, 16),
-      PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, 0, true, 1, 16), PA_VARIANT(uint32_t, false, 768, 32, 8, true, 6, 0, true, 2, 16)};
+      PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, true, 2, 32),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, true, 1, 32),
+      PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, true, 1, 32),  PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, true, 2, 32),
+      PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, true, 2, 32),  PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, true, 2, 16),
+      PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, true, 1, 16),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, true, 1, 16),
+      PA_VARIANT(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16)};
   // 512 x 16 up to 2^24 keys (4M: 0.209 -> 0.180 ms against 512 x 48; even at 16M); above, the 24,576-key
   // split tile as 768 x 32 (12 waves per workgroup, two workgroups per CU: 100M passes 188-227 us against
   // 194-233 us for 512 x 48).
-  static constexpr int by_class[kClasses] = {2, 2, 7, 9};
+  static constexpr int by_class[kClasses] = {2, 2, 6, 8};
 };
 template <>
 struct Variants<uint32_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 0, true, 1, 32),
-                                          PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, 1, true, 1, 32),
-                                          PA_VARIANT(uint32_t, true, 512, 20, 8, true, 4, 0, true, 1, 32)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint32_t, true, 512, 16, 4, true, 4, true, 1, 32),
+                                          PA_VARIANT(uint32_t, true, 512, 20, 8, true, 4, true, 1, 32)};
   // 512 x 20 above 2^24 keys (100M: 2.157 -> 2.117 ms; at 16M it is 6% slower than 512 x 16).
-  static constexpr int by_class[kClasses] = {0, 0, 0, 2};
+  static constexpr int by_class[kClasses] = {0, 0, 0, 1};
 };
 template <>
 struct Variants<uint64_t, false> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 0, true, 1, 32),
-                                          PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, 0, true, 1, 32),
-                                          PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, 1, true, 1, 32),
-                                          PA_VARIANT(uint64_t, false, 512, 16, 8, true, 4, 0, true, 1, 16),
-                                          PA_VARIANT(uint64_t, false, 512, 24, 8, true, 4, 0, true, 2, 16)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, true, 1, 32),
+                                          PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, true, 1, 32),
+                                          PA_VARIANT(uint64_t, false, 512, 16, 8, true, 4, true, 1, 16),
+                                          PA_VARIANT(uint64_t, false, 512, 24, 8, true, 4, true, 2, 16)};
   // 512 x 16 with 16-lane units up to 2^20 keys (1M: 0.155 -> 0.149 ms), the 512 x 24 split with
   // 16-lane units above (4M: 0.331 -> 0.254 ms, 16M: 0.798 -> 0.632 ms, 100M: 4.77 -> 3.57 ms).
-  static constexpr int by_class[kClasses] = {0, 3, 4, 4};
+  static constexpr int by_class[kClasses] = {0, 2, 3, 3};
 };
 template <>
 struct Variants<uint64_t, true> {
-  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 0, true, 1, 32),
-                                          PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, 1, true, 1, 32),
-                                          PA_VARIANT(uint64_t, true, 512, 16, 8, true, 4, 0, true, 1, 16)};
+  static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, true, 512, 8, 4, true, 6, true, 1, 32),
+                                          PA_VARIANT(uint64_t, true, 512, 16, 8, true, 4, true, 1, 16)};
   // 512 x 16 with 16-lane units above 2^20 keys (4M: 0.358 -> 0.284 ms, 16M: 1.10 -> 0.96 ms,
   // 100M: 7.01 -> 5.72 ms).
-  static constexpr int by_class[kClasses] = {0, 0, 2, 2};
+  static constexpr int by_class[kClasses] = {0, 0, 1, 1};
 };
 
 // Choices recorded by the autotuner (-1: none) for u32 keys-only sorts.
@@ -452,22 +248,11 @@ const PassVariant& pick_variant(size_t n) {
   return Variants<K, KV>::table[variant_index<K, KV>(n)];
 }
 
-// Look-back rows (and launch bound) of one pass: every chain rounds up to
-// whole tiles.
+// Look-back rows (tiles) of one pass.
 inline size_t sort_rows(const PassVariant& v, size_t n) {
   const size_t tile = (size_t)v.block * v.items;
-  return (n + tile - 1) / tile + (v.chained ? kChains : 0);
+  return (n + tile - 1) / tile;
 }
-
-// Pass-0 chains are position ranges of 2^log2s keys (at least 2^16, at most
-// kChains of them).
-inline uint32_t chain_log2s(size_t n) {
-  uint32_t l = 16;
-  while ((size_t(1) << l) * kChains < n) ++l;
-  return l;
-}
-
-constexpr unsigned kHistChainBlocks = 512;
 constexpr unsigned kPrefixGroups = 128;  // blocks of the pass-0 prefix scan
 
 // Scratch of one sort call with pass variant var (see enqueue_sort for the
@@ -477,16 +262,11 @@ size_t sort_scratch_bytes(const PassVariant& var, size_t n) {
   constexpr int D = sizeof(K);
   const size_t rows = sort_rows(var, n);
   Layout L;
-  L.take((size_t)D * kChains * 256 * 4);  // counts
-  L.take((size_t)D * kChains * 256 * 4);  // bases
+  L.take((size_t)D * 256 * 4);            // counts
+  L.take((size_t)D * 256 * 4);            // bases
   L.take(D * 4);                          // tile counters
   L.take((size_t)D * rows * 256 * 4);     // look-back table
   L.take((size_t)kPrefixGroups * 256 * 4);  // pass-0 prefix partial sums
-  if (var.chained) {
-    L.take((size_t)D * 36 * 4);                                  // chain tables
-    L.take((size_t)D * rows * 4);                                // claim orders
-    L.take((size_t)kHistChainBlocks * D * kChains * 256 * 4);    // histogram partials
-  }
   return L.off;
 }
 
@@ -495,7 +275,7 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
                  hipStream_t s, uint32_t xform) {
   constexpr int D = sizeof(K);
   const size_t rows = sort_rows(var, n);
-  const size_t ncount = (size_t)D * kChains * 256;
+  const size_t ncount = (size_t)D * 256;
   Layout L;
   uint32_t* counts = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
   uint32_t* bases = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
@@ -503,54 +283,35 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
   const size_t lb_bytes = (size_t)D * rows * 256 * 4;
   uint32_t* lb = reinterpret_cast<uint32_t*>(scratch + L.take(lb_bytes));
   uint32_t* prefix_partial = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)kPrefixGroups * 256 * 4));
-  uint32_t* chain_info = nullptr;
-  uint32_t* chain_tiles = nullptr;
-  uint32_t* partial = nullptr;
-  if (var.chained) {
-    chain_info = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)D * 36 * 4));
-    chain_tiles = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)D * rows * 4));
-    partial = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)kHistChainBlocks * ncount * 4));
-  }
 
   {
     ProfScope ps("k_sort_prepare", s);
     hipLaunchKernelGGL(k_sort_prepare, dim3(1), dim3(256), 0, s, counts, (uint32_t)ncount, ctr, (uint32_t)D);
   }
   const size_t tile = (size_t)var.block * var.items;
-  if (var.chained) {
-    const uint32_t log2s = chain_log2s(n);
-    {
-      ProfScope ps("k_hist_digits", s);
-      hipLaunchKernelGGL(k_hist_chains<K>, dim3(kHistChainBlocks), dim3(kSortBlock), 0, s, keys, n, log2s, partial,
-                         reinterpret_cast<uint4*>(lb), lb_bytes / 16, xform);
-    }
-    hipLaunchKernelGGL(k_reduce_counts, dim3((unsigned)((ncount + 255) / 256), 16), dim3(256), 0, s, partial,
-                       kHistChainBlocks, (uint32_t)ncount, counts);
-    hipLaunchKernelGGL(k_chain_plan, dim3(D), dim3(1024), 0, s, counts, (uint32_t)n, log2s, (uint32_t)tile, bases,
-                       chain_info, chain_tiles, (uint32_t)rows);
-  } else {
-    // One read: all digit histograms, pass 0's per-tile digit counts (into
-    // pass 0's look-back rows) and the clearing of the other passes' rows.
-    // Two small kernels then turn pass 0's rows into inclusive prefixes, so
-    // pass 0 runs without look-back waits; the first also writes the digit
-    // bases.
-    const size_t pass_rows_bytes = rows * 256 * 4;
-    const size_t zero_bytes = lb_bytes - pass_rows_bytes;
-    // Tiles per block: about n / 512 keys per block (two blocks per CU), at
-    // least one tile and at most 49,152 keys, so a small block's fixed costs
-    // (clearing its bins, one global flush) are spread over several tiles.
-    // Measured per histogram read: 16M keys in 8K-key tiles 48 -> 31 us,
-    // 100M keys in 24K-key tiles 126 -> 122 us, 1M keys unchanged.
-    const size_t target = std::min<size_t>(std::max<size_t>(n / 512, tile), 49152);
-    const size_t tpb = std::max<size_t>(1, std::min<size_t>((target + tile / 2) / tile, kHistTilesMaxKeys / tile));
-    const size_t grid = std::max<size_t>(1, (rows + tpb - 1) / tpb);
-    {
-      ProfScope ps("k_hist_digits", s);
-      hipLaunchKernelGGL(k_hist_tiles<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, (uint32_t)tile, lb,
-                         counts, reinterpret_cast<uint4*>(lb + rows * 256), zero_bytes / 16, xform);
-    }
-    const uint32_t groups = (uint32_t)std::min<size_t>(rows, kPrefixGroups);
-    const uint32_t per = (uint32_t)((rows + groups - 1) / groups);
+  // One read: all digit histograms, pass 0's per-tile digit counts (into
+  // pass 0's look-back rows) and the clearing of the other passes' rows.
+  // Two small kernels then turn pass 0's rows into inclusive prefixes, so
+  // pass 0 runs without look-back waits; the first also writes the digit
+  // bases.
+  const size_t pass_rows_bytes = rows * 256 * 4;
+  const size_t zero_bytes = lb_bytes - pass_rows_bytes;
+  // Tiles per block: about n / 512 keys per block (two blocks per CU), at
+  // least one tile and at most 49,152 keys, so a small block's fixed costs
+  // (clearing its bins, one global flush) are spread over several tiles.
+  // Measured per histogram read: 16M keys in 8K-key tiles 48 -> 31 us,
+  // 100M keys in 24K-key tiles 126 -> 122 us, 1M keys unchanged.
+  const size_t target = std::min<size_t>(std::max<size_t>(n / 512, tile), 49152);
+  const size_t tpb = std::max<size_t>(1, std::min<size_t>((target + tile / 2) / tile, kHistTilesMaxKeys / tile));
+  const size_t grid = std::max<size_t>(1, (rows + tpb - 1) / tpb);
+  {
+    ProfScope ps("k_hist_digits", s);
+    hipLaunchKernelGGL(k_hist_tiles<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, (uint32_t)tile, lb,
+                       counts, reinterpret_cast<uint4*>(lb + rows * 256), zero_bytes / 16, xform);
+  }
+  const uint32_t groups = (uint32_t)std::min<size_t>(rows, kPrefixGroups);
+  const uint32_t per = (uint32_t)((rows + groups - 1) / groups);
+  {
     ProfScope ps("k_tile_prefix", s);
     hipLaunchKernelGGL(k_tile_sums, dim3(groups + D), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per, prefix_partial,
                        groups, counts, bases);
@@ -562,14 +323,12 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
   uint32_t* vout = vtmp;
   for (int p = 0; p < D; ++p) {
     uint32_t fmode = 0;
-    if (!var.chained && p == 0) fmode |= kPrefixReady;
+    if (p == 0) fmode |= kPrefixReady;
     if (xform && p == 0) fmode |= kFloatIn;
     if (xform && p == D - 1) fmode |= kFloatOut;
     if (xform == dev::kXformSigned) fmode |= kSignedKeys;
-    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * (var.chained ? kChains * 256 : 256),
-               lb + (size_t)p * rows * 256, ctr + p, (uint32_t)rows, (uint32_t)(8 * p), fmode,
-               var.chained ? chain_info + (size_t)p * 36 : nullptr,
-               var.chained ? chain_tiles + (size_t)p * rows : nullptr};
+    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * 256, lb + (size_t)p * rows * 256, ctr + p,
+               (uint32_t)rows, (uint32_t)(8 * p), fmode};
     PA_TRY(var.launch(s, a));
     std::swap(kin, kout);
     std::swap(vin, vout);

@@ -24,10 +24,7 @@ enum : uint32_t { kFloatIn = 1u, kFloatOut = 2u, kPrefixReady = 4u, kSignedKeys 
 enum : int {
   kModePlain = 0,  // one array, tiles in order, one look-back chain
   kModeSeg = 1,    // independent segments (segmented batch sort)
-  kModeChain = 2,  // one array split into interleaved look-back chains
 };
-constexpr int kChains = 8;            // look-back chains of a chained pass
-constexpr uint32_t kChainShift = 5;  // chain of a key in pass p >= 1: digit p-1 >> 5
 
 
 // One read of the keys -> D digit histograms (D = sizeof(K)), counts laid out
@@ -150,17 +147,9 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 // Segmented mode: tile_info[t] = (segment << 32) | tile index within the
 // segment; the segment's input/output addresses and length come from seg_in /
 // seg_out / seg_len, its digit counts from gtab + segment * gstride * 256 and
-// its first look-back row from chain_info[segment] (a segment's rows are
+// its first look-back row from seg_row[segment] (a segment's rows are
 // contiguous; its tiles may be claimed interleaved with other segments').
 // Plain mode: gtab holds the pass's exclusive digit bases.
-//
-// Chain mode: the array is split into kChains contiguous chains, each with its
-// own look-back rows, and claims interleave the chains (chain_tiles[i] =
-// (chain << 24) | tile within the chain), so consecutive tiles of one chain
-// are claimed several tiles apart and a tile mostly finds its predecessor's
-// inclusive prefix already published.  chain_info[4c + 0..2] = (first key,
-// keys, first look-back row) of chain c, chain_info[32] = tiles in the pass;
-// gtab + 256c holds chain c's output base per digit.
 //
 // LATE: the look-back's first round trip is issued while the block scatters
 // its keys into LDS and read afterwards, so the status loads are in flight
@@ -184,9 +173,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     uint32_t ntiles, uint32_t shift, uint32_t fmode, const uint64_t* __restrict__ tile_info = nullptr,
     const uint64_t* __restrict__ seg_in = nullptr, const uint64_t* __restrict__ seg_out = nullptr,
     const uint32_t* __restrict__ seg_len = nullptr, uint32_t gstride = 0,
-    const uint32_t* __restrict__ chain_info = nullptr, const uint32_t* __restrict__ chain_tiles = nullptr) {
+    const uint32_t* __restrict__ seg_row = nullptr) {
   constexpr bool SEG = MODE == kModeSeg;
-  constexpr bool CHAIN = MODE == kModeChain;
   static_assert(BLOCK >= 256 && BLOCK <= 1024 && BLOCK % 64 == 0, "one thread per digit is needed");
   static_assert(!(SEG && KV), "segmented mode sorts keys only");
   static_assert(UL == 32 || UL == 16, "ranking units are 16 or 32 lanes");
@@ -229,7 +217,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 #pragma unroll
     for (int r = 0; r < UNITS * 256 * SLOTB / 16 / BLOCK; ++r) z[tid + r * BLOCK] = make_uint4(0, 0, 0, 0);
   }
-  if (CHAIN) ntiles = chain_info[32];
   if (tid == 0) s_claim = atomicAdd(tile_counter, 1u);
   __syncthreads();
   // Uniform across the block: the tile geometry below lives in scalar registers.
@@ -241,29 +228,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   K* out;
   const uint32_t* vsrc = nullptr;
   uint32_t* vdst = nullptr;
-  const uint32_t* g;  // digit bases (plain, chain) or segment digit counts (SEG)
-  uint32_t len;       // keys in the tile's array (segment, chain)
+  const uint32_t* g;  // digit bases (plain) or segment digit counts (SEG)
+  uint32_t len;       // keys in the tile's array (segment)
   uint32_t row = claimed;  // look-back row of the tile
-  uint32_t local;          // tile index within its array (segment, chain)
-  if constexpr (CHAIN) {
-    const uint32_t info = chain_tiles[claimed];
-    const uint32_t c = info >> 24;
-    local = info & 0xFFFFFFu;
-    const uint32_t first = chain_info[4 * c];
-    in = kin + first;
-    out = kout;
-    if constexpr (KV) {
-      vsrc = vin + first;
-      vdst = vout;
-    }
-    len = chain_info[4 * c + 1];
-    row = chain_info[4 * c + 2] + local;
-    g = gtab + 256 * c;
-  } else if constexpr (SEG) {
+  uint32_t local;          // tile index within its array (segment)
+  if constexpr (SEG) {
     const uint64_t info = tile_info[claimed];
     const uint32_t seg = (uint32_t)(info >> 32);
     local = (uint32_t)info;
-    row = chain_info[seg] + local;
+    row = seg_row[seg] + local;
     in = reinterpret_cast<const K*>(seg_in[seg]);
     out = reinterpret_cast<K*>(seg_out[seg]);
     len = seg_len[seg];
