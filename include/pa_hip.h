@@ -69,6 +69,18 @@ int pa_copy(void* dst, const void* src, size_t bytes, pa_stream_t stream);
  * library uses its own device scratch. */
 int pa_sort_u32(uint32_t* keys, uint32_t* tmp, size_t n, pa_stream_t stream);
 
+/* Allocation-free form of pa_sort_u32.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  keys and tmp
+ * must be device memory of n elements each (no host staging), n <= 2^27.  The
+ * sort's scratch is a static device array of 22 MiB that every GPU the library
+ * runs on holds until the process ends; no call allocates or frees device
+ * memory.  All calls on one GPU share that array: calls that overlap on one GPU
+ * (different streams or host threads) race, so callers serialise them.
+ * Returns PA_ERR_TOO_LARGE for n > 2^27, or when the pass shape chosen for n
+ * (PA_SORT_VARIANT, the autotuner) needs more scratch than the array holds. */
+int pa_sort_u32_noalloc(uint32_t* keys, uint32_t* tmp, size_t n, pa_stream_t stream);
+
 /* Replaces RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase(unsigned* a,
  * unsigned* b, unsigned long n) (RadixSortLsdParallel.h:86-106): sorted result
  * in BOTH a and b (the reference copies it to b, :80-82).  Also serves

@@ -32,6 +32,7 @@ _SIGS = {
     "pa_set_device": (_i, [_i]),
     "pa_synchronize": (_i, [_vp]),
     "pa_sort_u32": (_i, [_vp, _vp, _sz, _vp]),
+    "pa_sort_u32_noalloc": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_two_phase_u32": (_i, [_vp, _vp, _sz, _vp]),
     "pa_autotune_sort_u32": (_i, [_sz, _vp]),
     "pa_sort_variant_u32": (_i, [_sz]),
@@ -187,6 +188,13 @@ def sort_u32(keys, tmp=None, stream=None):
     _check(keys, 4, "keys")
     _check(tmp, 4, "tmp", numel(keys))
     call("pa_sort_u32", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
+
+
+def sort_u32_noalloc(keys, tmp, stream=None):
+    """pa_sort_u32_noalloc: keys and tmp on the device, no device allocation."""
+    _check(keys, 4, "keys")
+    _check(tmp, 4, "tmp", numel(keys))
+    call("pa_sort_u32_noalloc", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
 
 
 def sort_two_phase_u32(a, b, stream=None):

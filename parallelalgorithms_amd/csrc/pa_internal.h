@@ -92,6 +92,14 @@ struct Layout {
 // ---------------------------------------------------------------------------
 // Kernel timing hooks (pa_profile_*): records a pair of events around a launch.
 // ---------------------------------------------------------------------------
+// While a ProfSuppress lives, launches on its thread are not timed (no event
+// is created or recorded), for entry points that must not allocate.
+struct ProfSuppress {
+  ProfSuppress();
+  ~ProfSuppress();
+  bool prev;
+};
+
 // attach = true: the events are not recorded around the launch; the caller
 // passes start/stop to hipExtLaunchKernelGGL, which stamps them in the
 // dispatch itself (no extra queue packets), and sets `attached` once it did.

@@ -273,8 +273,12 @@ static hipEvent_t prof_event() {
   return e;
 }
 
+static thread_local bool g_prof_suppressed = false;
+ProfSuppress::ProfSuppress() : prev(g_prof_suppressed) { g_prof_suppressed = true; }
+ProfSuppress::~ProfSuppress() { g_prof_suppressed = prev; }
+
 ProfScope::ProfScope(const char* n, hipStream_t s, bool att) : name(n), stream(s), attach(att) {
-  if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  if (g_prof_suppressed || !g_prof_on.load(std::memory_order_relaxed)) return;
   {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     if (!g_prof_only.empty() && std::strncmp(n, g_prof_only.c_str(), g_prof_only.size()) != 0) return;

@@ -26,6 +26,7 @@
 // call: counts D x 256 u32, tile counters D u32, look-back D x tiles x 256 u32.
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <type_traits>
 #include <cstdlib>
 #include <utility>
@@ -414,6 +415,41 @@ int enqueue_large_sort(K* keys, K* tmp, size_t n, char* scratch, hipStream_t s, 
   return PA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free u32 sort (pa_sort_u32_noalloc).  The sort's scratch (digit
+// counts and bases, tile counters, look-back table, prefix partials) lives in
+// a file-scope device array instead of a per-stream allocation; every word a
+// sort reads is written first by its own kernels (k_sort_prepare clears the
+// counters, k_hist_tiles writes pass 0's rows and clears the others), so the
+// contents left by an earlier call do not matter and nothing is cleared with
+// a memset.  The array's device address is looked up once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocMaxKeys = size_t(1) << 27;
+// Covers the built-in pass shape of every size class up to kNoAllocMaxKeys
+// (2^27 keys in 24,576-key tiles: 5,462 look-back rows per pass, 21.4 MB).
+constexpr size_t kNoAllocScratchBytes = size_t(22) << 20;
+static __device__ __align__(256) unsigned char noalloc_scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of the current device's copy of noalloc_scratch.
+static int noalloc_scratch_ptr(char** out) {
+  int dev = 0;
+  PA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return arg_error("pa_sort_u32_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    PA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    if (!p) return arg_error("pa_sort_u32_noalloc: no device address for the scratch array");
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = static_cast<char*>(g_noalloc_ptr[dev]);
+  return PA_OK;
+}
+
 // Host entry: classifies pointers, stages host arrays, runs, copies back.
 template <typename K, bool KV>
 int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipStream_t s, uint32_t xform,
@@ -472,6 +508,28 @@ extern "C" {
 
 int pa_sort_u32(uint32_t* keys, uint32_t* tmp, size_t n, pa_stream_t stream) {
   return sort_entry<uint32_t, false>(keys, nullptr, tmp, nullptr, n, (hipStream_t)stream, dev::kXformNone, "pa_sort_u32");
+}
+
+int pa_sort_u32_noalloc(uint32_t* keys, uint32_t* tmp, size_t n, pa_stream_t stream) {
+  if (n == 0) return PA_OK;
+  if (!keys || !tmp) return arg_error("pa_sort_u32_noalloc: keys and tmp are required");
+  if (n > kNoAllocMaxKeys) {
+    set_last_error("pa_sort_u32_noalloc: n must be <= 2^27 (134,217,728) keys");
+    return PA_ERR_TOO_LARGE;
+  }
+  if (!is_device_ptr(keys) || !is_device_ptr(tmp))
+    return arg_error("pa_sort_u32_noalloc: keys and tmp must be device memory");
+  hipStream_t s = (hipStream_t)stream;
+  ProfSuppress no_timing;  // timing events would be allocations
+  if (n <= small_sort_max_keys()) return small_sort_enqueue_u32(keys, n, s);
+  const PassVariant& var = pick_variant<uint32_t, false>(n);
+  if (sort_scratch_bytes<uint32_t, false>(var, n) > kNoAllocScratchBytes) {
+    set_last_error("pa_sort_u32_noalloc: the pass shape chosen for n needs more than the 22 MiB static scratch");
+    return PA_ERR_TOO_LARGE;
+  }
+  char* scratch = nullptr;
+  PA_TRY(noalloc_scratch_ptr(&scratch));
+  return enqueue_sort<uint32_t, false>(var, keys, nullptr, tmp, nullptr, n, scratch, s, dev::kXformNone);
 }
 
 int pa_sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, pa_stream_t stream) {
