@@ -68,26 +68,14 @@ def cpu_baseline(keys_np, iters=3):
     }
 
 
-def log_uniform(u64, lo, hi):
-    import numpy as np
-    u = (u64 >> np.uint64(11)).astype(np.float64) / float(1 << 53)
-    return np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.uint64)
-
-
 def bench_segmented(pa, torch, count=10_000, seed=1239):
     """10,000 u32 arrays, lengths log-uniform in [64, 4,000,000] (mt19937_64 seed
     1239), keys uniform, 5% presorted: one segmented batch vs per-array calls."""
-    import numpy as np
-    r = pa.gen_u64(seed, 2 * count)
-    lens = log_uniform(r[:count], 64, 4_000_000)
-    presorted = ((r[count:] >> np.uint64(11)).astype(np.float64) / float(1 << 53)) < 0.05
-    offs = np.zeros(count + 1, dtype=np.uint64)
-    offs[1:] = np.cumsum(lens)
+    from parallelalgorithms_amd import workloads
+    lens, offs, presorted = workloads.segmented_batch(count, seed)
     total = int(offs[-1])
     pristine = torch.empty(total, dtype=torch.int32, device="cuda")
-    pa.fill_random_u32(pristine, seed)
-    pre = [pristine[int(offs[i]):int(offs[i + 1])] for i in np.nonzero(presorted)[0]]
-    pa.segmented_sort_ptrs_u32(pre)  # the presorted 5%
+    workloads.fill_segmented_batch(pristine, offs, presorted, seed)  # keys, then the presorted 5%
     tmp = torch.empty_like(pristine)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
