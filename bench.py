@@ -38,8 +38,38 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(keys_np, iters=3):
-    """Times the oracle's SortRadixPar restatement (3-arg form) on host cores."""
+def host_cpu_info():
+    """CPU model, sockets and physical cores of this host from /proc/cpuinfo."""
+    model, sockets, cores = None, set(), set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model is None:
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                    sockets.add(val)
+                elif key == "core id":
+                    core = val
+                elif not key and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "sockets": len(sockets) or None, "physical_cores": len(cores) or None,
+            "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(keys_np, iters=5):
+    """Times the oracle's SortRadixPar restatement (3-arg form) on host cores,
+    as the reference's harness does (RadixSortLsdBenchmark.cpp:22,86-103):
+    input re-copied and scratch paged in before each run, the call alone
+    timed, `iters` runs after one warm-up, median reported."""
     import numpy as np
     import oracle
     threads = cpu_threads()
@@ -64,7 +94,8 @@ def cpu_baseline(keys_np, iters=3):
         "cores": threads,
         "kind": "port",
         "sample": f"full workload: {keys_np.size} u32 keys, median of {iters} runs of the oracle's SortRadixPar "
-                  f"restatement (std::thread pool), sorted={ok}",
+                  f"restatement (std::thread pool, {threads} threads), sorted={ok}",
+        **host_cpu_info(),
     }
 
 
@@ -154,17 +185,23 @@ def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
             "saturation_jobs_per_s": round(sat, 1), "levels": levels, "outputs_equal_across_levels": same}
 
 
-def bench_primitives(pa, torch, n=100_000_000, seed=1241, reps=5):
-    """Device-resident throughput of the path's other entry points at the
-    sort's size: each is timed with HIP events on the launch stream, best of
-    `reps` calls after one warm-up, inputs regenerated where a call sorts in
-    place.  Keys/s counts input keys (merge: output keys; counting sort: bytes)."""
+def bench_configs(pa, torch, n=100_000_000, reps=5):
+    """Device-resident timings of the other configured workloads and entry
+    points: each is timed with HIP events on the launch stream, best of
+    `reps` calls after one warm-up, inputs restored before every call that
+    works in place, and its last output checked against an independent
+    torch computation.  Keys/s counts input keys (merge: output keys;
+    counting sort: bytes).  hbm_frac = algorithmic bytes / time / 8 TB/s,
+    with the bytes of SURVEY-style accounting: LSD sort k(1 + 2D) per key,
+    key-value k(1 + 2D) + v 2D, histogram k, select k (one read: its lower
+    bound), merge 2k per output key."""
     import numpy as np
+    from parallelalgorithms_amd import workloads
     dev = torch.device("cuda:0")
     st = torch.cuda.current_stream()
     out = {}
 
-    def timed(name, fn, fresh=None, units=n):
+    def timed(name, fn, fresh=None, units=n, bytes_per_unit=None, check=None):
         fn()
         best = None
         for _ in range(reps):
@@ -178,46 +215,96 @@ def bench_primitives(pa, torch, n=100_000_000, seed=1241, reps=5):
             torch.cuda.synchronize()
             ms = a.elapsed_time(b)
             best = ms if best is None else min(best, ms)
-        out[name] = {"ms": round(best, 4), "gkeys_per_s": round(units / (best / 1e3) / 1e9, 2)}
+        rec = {"ms": round(best, 4), "gkeys_per_s": round(units / (best / 1e3) / 1e9, 2)}
+        if bytes_per_unit:
+            rec["hbm_frac"] = round(bytes_per_unit * units / (best / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+        if check is not None:
+            rec["bit_exact"] = bool(check())
+        out[name] = rec
 
-    u32 = torch.from_numpy(pa.gen_u32(seed, n).view(np.int32)).to(dev)
-    for m in (10_000, 1_000_000):  # small sorts: launch latency, not bandwidth
-        sm = u32[:m].clone()
-        wm = torch.empty_like(sm)
-        timed(f"sort_u32_{m}", lambda: pa.sort_u32(wm), fresh=lambda: wm.copy_(sm), units=m)
-    w32 = torch.empty_like(u32)
-    t32 = torch.empty_like(u32)
+    def u(t):  # int32 view of u32 keys -> exact int64 values
+        return t.to(torch.int64) & 0xFFFFFFFF
+
+    # Config 1: 10M uniform u32 (mt19937_64 seed 1234), 8-bit digits.
+    m = 10_000_000
+    s10 = torch.from_numpy(pa.gen_u32(1234, m).view(np.int32)).to(dev)
+    w10, t10 = torch.empty_like(s10), torch.empty_like(s10)
+    r10 = u(s10).sort().values
+    timed("config1_sort_u32_10M", lambda: pa.sort_u32(w10, t10), fresh=lambda: w10.copy_(s10), units=m,
+          bytes_per_unit=36, check=lambda: torch.equal(u(w10), r10))
+    del s10, w10, t10, r10
+    # Config 2: 100M u32 in three distributions: sort, top-byte histogram,
+    # select at k = n/2 and n/100.
+    w32 = torch.empty(n, dtype=torch.int32, device=dev)
+    t32 = torch.empty_like(w32)
     r32 = torch.zeros(1, dtype=torch.int32, device=dev)
     cnt = torch.zeros(256, dtype=torch.int64, device=dev)
-    timed("hist_u32_top_byte", lambda: pa.hist_u32(u32, 24, 8, cnt))
-    timed("select_u32_median", lambda: pa.select_u32(u32, n // 2, r32))
-    timed("partition_u32_median", lambda: pa.partition_u32(w32, n // 2, r32), fresh=lambda: w32.copy_(u32))
-    half = torch.sort(u32[: n // 2].to(torch.int64) & 0xFFFFFFFF)[0].to(torch.int32)
-    half2 = torch.sort(u32[n // 2:].to(torch.int64) & 0xFFFFFFFF)[0].to(torch.int32)
-    timed("merge_u32_2x50M", lambda: pa.merge_u32(half, half2, w32))
-    del half, half2
+    for dist in ("uniform", "presorted", "constant"):
+        if dist == "uniform":
+            src = torch.from_numpy(pa.gen_u32(1234, n).view(np.int32)).to(dev)
+        elif dist == "presorted":
+            src = torch.arange(n, dtype=torch.int32, device=dev)
+        else:
+            src = torch.full((n,), 0x5A5A5A5A, dtype=torch.int32, device=dev)
+        ref = u(src).sort().values
+        timed(f"config2_sort_{dist}", lambda: pa.sort_u32(w32, t32), fresh=lambda: w32.copy_(src),
+              bytes_per_unit=36, check=lambda: torch.equal(u(w32), ref))
+        hist_ref = torch.bincount(u(src) >> 24, minlength=256)
+        timed(f"config2_hist_top_byte_{dist}", lambda: pa.hist_u32(src, 24, 8, cnt), bytes_per_unit=4,
+              check=lambda: torch.equal(cnt, hist_ref))
+        for label, k in (("n_2", n // 2), ("n_100", n // 100)):
+            timed(f"config2_select_{label}_{dist}", lambda: pa.select_u32(src, k, r32), bytes_per_unit=4,
+                  check=lambda: int(u(r32)[0]) == int(ref[k]))
+        del src, ref, hist_ref
+    timed("partition_u32_median", lambda: pa.partition_u32(w32, n // 2, r32),
+          fresh=lambda: w32.copy_(torch.from_numpy(pa.gen_u32(1234, n).view(np.int32)).to(dev)))
+    # Config 3: float keys with specials, merge of two sorted 100M runs, and
+    # the 200M u64 key-value sort (below).
+    bits = workloads.f32_with_specials(n, 1236)
+    fref = workloads.sorted_f32_bits(bits)
+    wf = torch.empty_like(bits)
+    timed("config3_sort_f32_specials", lambda: pa.sort_f32(wf.view(torch.float32), t32.view(torch.float32)),
+          fresh=lambda: wf.copy_(bits), bytes_per_unit=36, check=lambda: torch.equal(u(wf), fref))
+    del bits, fref, wf
+    a = u(torch.from_numpy(pa.gen_u32(1237, n).view(np.int32)).to(dev)).sort().values.to(torch.int32)
+    b = u(torch.from_numpy(pa.gen_u32(1238, n).view(np.int32)).to(dev)).sort().values.to(torch.int32)
+    mo = torch.empty(2 * n, dtype=torch.int32, device=dev)
+    mref = torch.cat([u(a), u(b)]).sort().values
+    timed("config3_merge_u32_2x100M", lambda: pa.merge_u32(a, b, mo), units=2 * n, bytes_per_unit=8,
+          check=lambda: torch.equal(u(mo), mref))
+    del a, b, mo, mref
+    # Other entry points at 100M.
+    for mm in (10_000, 1_000_000):  # small sorts: launch latency, not bandwidth
+        sm = torch.from_numpy(pa.gen_u32(1241, mm).view(np.int32)).to(dev)
+        wm = torch.empty_like(sm)
+        timed(f"sort_u32_{mm}", lambda: pa.sort_u32(wm), fresh=lambda: wm.copy_(sm), units=mm)
+    u32 = torch.from_numpy(pa.gen_u32(1241, n).view(np.int32)).to(dev)
     vals = torch.arange(n, dtype=torch.int32, device=dev)
     v2, vt = torch.empty_like(vals), torch.empty_like(vals)
 
     def kv_fresh():
         w32.copy_(u32)
         v2.copy_(vals)
-    timed("sort_kv_u32_u32", lambda: pa.sort_kv_u32_u32(w32, v2, t32, vt), fresh=kv_fresh)
+    timed("sort_kv_u32_u32", lambda: pa.sort_kv_u32_u32(w32, v2, t32, vt), fresh=kv_fresh, bytes_per_unit=68)
     del vals, v2, vt
     b8 = (u32 & 0xFF).to(torch.uint8)
     w8 = torch.empty_like(b8)
-    timed("counting_sort_u8", lambda: pa.counting_sort_u8(w8), fresh=lambda: w8.copy_(b8))
-    del b8, w8, w32, t32
-    u64 = torch.from_numpy(pa.gen_u64(seed + 1, n).view(np.int64)).to(dev)
+    c8 = torch.bincount(b8.to(torch.int64), minlength=256)
+    timed("counting_sort_u8", lambda: pa.counting_sort_u8(w8), fresh=lambda: w8.copy_(b8), bytes_per_unit=2,
+          check=lambda: torch.equal(w8, torch.repeat_interleave(torch.arange(256, device=dev, dtype=torch.uint8), c8)))
+    h8 = torch.zeros(256, dtype=torch.int64, device=dev)
+    timed("hist_u8", lambda: pa.hist_u8(b8, h8), bytes_per_unit=1, check=lambda: torch.equal(h8, c8))
+    del b8, w8, c8, h8, w32, t32
+    u64 = torch.from_numpy(pa.gen_u64(1242, n).view(np.int64)).to(dev)
     w64, t64 = torch.empty_like(u64), torch.empty_like(u64)
-    timed("sort_u64", lambda: pa.sort_u64(w64, t64), fresh=lambda: w64.copy_(u64))
+    timed("sort_u64", lambda: pa.sort_u64(w64, t64), fresh=lambda: w64.copy_(u64), bytes_per_unit=136)
     f64 = (u64 >> 11).to(torch.float64) * (2.0 ** -53) * 2e6 - 1e6
     timed("sort_f64", lambda: pa.sort_f64(w64.view(torch.float64), t64.view(torch.float64)),
-          fresh=lambda: w64.view(torch.float64).copy_(f64))
+          fresh=lambda: w64.view(torch.float64).copy_(f64), bytes_per_unit=136)
     del u64, w64, t64, f64
     # Key-value at the configuration's size: 200M u64 keys, u32 payload = index.
     m = 2 * n
-    k64 = torch.from_numpy(pa.gen_u64(seed + 2, m).view(np.int64)).to(dev)
+    k64 = torch.from_numpy(pa.gen_u64(1235, m).view(np.int64)).to(dev)
     kw = torch.empty_like(k64)
     v0 = torch.arange(m, dtype=torch.int32, device=dev)
     vw = torch.empty_like(v0)
@@ -225,7 +312,8 @@ def bench_primitives(pa, torch, n=100_000_000, seed=1241, reps=5):
     def kv64_fresh():
         kw.copy_(k64)
         vw.copy_(v0)
-    timed("sort_kv_u64_u32_200M", lambda: pa.sort_kv_u64_u32(kw, vw), fresh=kv64_fresh, units=m)
+    timed("config3_sort_kv_u64_u32_200M", lambda: pa.sort_kv_u64_u32(kw, vw), fresh=kv64_fresh, units=m,
+          bytes_per_unit=200)
     del k64, kw, v0, vw
     torch.cuda.empty_cache()
     # The drop-in headers' usual case: keys in ordinary (pageable) host memory,
@@ -241,7 +329,8 @@ def bench_primitives(pa, torch, n=100_000_000, seed=1241, reps=5):
         pa.sort_u32(h)
         dt = time.perf_counter() - t0
         best = dt if best is None else min(best, dt)
-    out["sort_u32_host_memory_pcie_included"] = {"ms": round(best * 1e3, 3), "gkeys_per_s": round(n / best / 1e9, 2)}
+    out["sort_u32_host_memory_pcie_included"] = {"ms": round(best * 1e3, 3), "gkeys_per_s": round(n / best / 1e9, 2),
+                                                 "sorted": bool(np.all(h[1:] >= h[:-1]))}
     return out
 
 
@@ -334,6 +423,11 @@ def main():
     torch.cuda.synchronize()
     pa.profile_enable(False)
     pa.profile_only(None)
+    # Every timed sort's output against the reference order.
+    ref = torch.sort(src.to(torch.int64) & 0xFFFFFFFF)[0].to(torch.int32)
+    timed_ok = all(bool(torch.equal(b, ref)) for b in bufs)
+    del ref
+    print(f"[bench] {len(bufs)} timed outputs equal torch.sort: {timed_ok}", file=sys.stderr, flush=True)
     ms_total = start.elapsed_time(stop)
     ms_step = ms_total / args.steps
     value = n / (ms_step / 1e3) / 1e9
@@ -360,7 +454,7 @@ def main():
     seg = serve = prims = None
     if not args.no_prims:
         print("[bench] other entry points", file=sys.stderr, flush=True)
-        prims = bench_primitives(pa, torch)
+        prims = bench_configs(pa, torch)
     if not args.no_seg:
         print("[bench] segmented batch vs per-array calls", file=sys.stderr, flush=True)
         seg = bench_segmented(pa, torch)
@@ -394,10 +488,11 @@ def main():
         "whole_sort_hbm_frac": round(whole_sort_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "copy_kernel_GBps": round(copy_gbs, 1),
         "kernels": kernels,
-        "primitives_100M": prims,
+        "other_configs": prims,
         "segmented": seg,
         "serve": serve,
         "sorted_ok": ok,
+        "timed_outputs_bit_exact": timed_ok,
         "baseline_ref": "README.md:9 SortRadixPar 100M u32 random, 48-core AWS C7a: 2865 Mkeys/s",
     }
     print(json.dumps(out), flush=True)

@@ -88,18 +88,8 @@ def test_kv_u32_100M_stable(pa):
 
 def test_config3_f32_specials_100M(pa):
     import torch
-    import parallelalgorithms_amd as P
-    n = N
-    r = torch.from_numpy(P.gen_u64(1236, 2 * n).view(np.int64)).cuda()
-    u1 = ((r[:n] >> 11) & ((1 << 53) - 1)).to(torch.float64) / float(1 << 53)
-    u2 = ((r[n:] >> 11) & ((1 << 53) - 1)).to(torch.float64) / float(1 << 53)
-    z = torch.sqrt(-2.0 * torch.log1p(-u1)) * torch.cos(2.0 * np.pi * u2)
-    bits = z.to(torch.float32).view(torch.int32).clone()
-    specials = torch.tensor([0x7F800000, 0xFF800000 - (1 << 32), 0, SIGN32, 0x7FC00000, 0xFFC00000 - (1 << 32),
-                             0x7F800001], dtype=torch.int32, device="cuda")
-    pick = (r[:n] & 127) == 0  # about 0.8% of the keys
-    bits[pick] = specials[(r[n:][pick] % 7).to(torch.int64)]
-    del r, u1, u2, z
+    from parallelalgorithms_amd import workloads
+    bits = workloads.f32_with_specials(N, 1236)
     # The order: transformed bits f(u) = u ^ (sign ? ~0 : 0x80000000), sorted
     # as unsigned, then transformed back.
     b = _u(bits)
@@ -107,6 +97,7 @@ def test_config3_f32_specials_100M(pa):
     fs = f.sort().values
     expect = torch.where(fs >> 31 == 1, fs ^ 0x80000000, fs ^ 0xFFFFFFFF)
     del b, f, fs
+    assert torch.equal(workloads.sorted_f32_bits(bits), expect)  # the bench's check agrees
     d = bits.view(torch.float32)
     pa.sort_f32(d)
     assert torch.equal(_u(d.view(torch.int32)), expect)
