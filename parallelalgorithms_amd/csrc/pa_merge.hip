@@ -19,8 +19,12 @@
 
 namespace pa {
 
-constexpr int kMergeBlock = 256;
-constexpr int kMergeItems = 16;
+// 512 threads x 8 outputs: at 2 x 100M u32 the tile kernel and its partition
+// take 440 us against 529 us for 256 x 16 (each thread's merge is a chain of
+// dependent LDS reads, so shorter chains and more of them in flight pay) and
+// 448-562 us for the other shapes measured (256 x 4/8, 512 x 4, 1024 x 4/8).
+constexpr int kMergeBlock = 512;
+constexpr int kMergeItems = 8;
 constexpr int kMergeTile = kMergeBlock * kMergeItems;
 
 // Number of elements taken from A among the first d outputs of the stable

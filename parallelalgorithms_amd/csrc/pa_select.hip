@@ -9,8 +9,13 @@
 // same scheme over digits 56 .. 0 (an extension; the reference is 32-bit).
 //
 // GPU form (the caller's array is never modified):
+//   k_select_init   state, cleared control words, and a band of top-digit
+//                   bins predicted from an evenly strided sample of the keys
+//                   to hold the answer.
 //   k_select_first  one read of all keys builds the top digit's histogram
-//                   and the keys' minimum and maximum.
+//                   and the keys' minimum and maximum, and compacts the keys
+//                   of the band; when the answer's top digit is in the band,
+//                   the later passes read only those keys.
 //   k_select_pass   (launch i of D - 1; returns at once when the answer is
 //                   known) first derives its state from the counts the
 //                   previous launch left: launch 0 fixes the high bytes that
@@ -47,21 +52,114 @@ struct SelState {
   uint64_t result;
   int32_t shift;       // digit of the next pass (-8 when done)
   uint32_t done;
-  uint32_t src;          // 0: read the caller's keys, 1: read the buffer
+  uint32_t src;          // 0: read the caller's keys, 1: read the buffer, 2: read block regions of it
   uint32_t compact_now;  // the next pass reading the keys compacts candidates
+  // Top-digit bins [band_lo, band_hi] predicted from a sample to hold the
+  // answer; when band_on the first read also compacts their keys.
+  uint32_t band_lo, band_hi, band_on;
 };
 
-__global__ void k_select_init(SelState* st, uint64_t k, uint64_t n, uint64_t cap, int32_t top_shift, uint32_t* zero,
-                              uint32_t nzero) {
-  if (threadIdx.x == 0) {
-    SelState s{};
-    s.k = k;
-    s.cand = n;
-    s.cap = cap;
-    s.shift = top_shift;
-    *st = s;
+// Keys sampled to predict the answer's top-digit bin: runs of kSelRun
+// consecutive keys at an even stride, kSelSampleRuns runs per block of the
+// sampling launch.
+constexpr uint32_t kSelSample = 32768;
+constexpr uint32_t kSelSampleBlocks = 32;
+constexpr uint32_t kSelSampleRuns = 8;
+constexpr uint32_t kSelRun = kSelSample / (kSelSampleBlocks * kSelSampleRuns);
+
+// Initial state, cleared control words and the sample.  kSelSampleBlocks
+// blocks each read kSelSampleRuns runs of kSelRun consecutive keys at an even
+// stride over the array (single keys at that stride would each cost a page
+// walk), or together every key when n < kSelSample, and write their top-digit
+// counts to sample[block][256] (no atomics, so nothing needs clearing first).
+// All of a thread's loads are issued before any is counted.  Block 0 writes
+// the initial state; every block clears a share of the control words.
+template <typename K>
+__global__ __launch_bounds__(256) void k_select_init(SelState* st, const K* __restrict__ keys, uint64_t k, uint64_t n,
+                                                     uint64_t cap, int32_t top_shift, uint32_t* zero, uint32_t nzero,
+                                                     uint32_t* __restrict__ sample) {
+  __shared__ uint32_t s_h[256];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = blockIdx.x * 256 + tid; i < nzero; i += gridDim.x * 256) zero[i] = 0;
+  s_h[tid] = 0;
+  __syncthreads();
+  constexpr uint32_t SPT = kSelSample / kSelSampleBlocks / 256;  // keys per thread
+  constexpr uint32_t TPR = 256 / kSelSampleRuns;                 // threads per run
+  constexpr uint32_t RUNS = kSelSampleBlocks * kSelSampleRuns;
+  const uint64_t run = (uint64_t)blockIdx.x * kSelSampleRuns + tid / TPR, part = tid % TPR;
+  const bool all = n < kSelSample;
+  const uint64_t base = all ? 0 : run * (n - kSelRun) / (RUNS - 1);
+  K v[SPT];
+#pragma unroll
+  for (uint32_t j = 0; j < SPT; ++j) {
+    // all: thread g of the grid takes keys g, g + grid threads, ...; else
+    // key j * TPR + part of its run.
+    const uint64_t i = all ? (uint64_t)blockIdx.x * 256 + tid + (uint64_t)j * kSelSampleBlocks * 256
+                           : base + (uint64_t)j * TPR + part;
+    v[j] = i < n ? keys[i] : K(0);
   }
-  for (uint32_t i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < SPT; ++j) {
+    const uint64_t i = all ? (uint64_t)blockIdx.x * 256 + tid + (uint64_t)j * kSelSampleBlocks * 256 : 0;
+    if (i < n) atomicAdd(&s_h[dev::digit8<K>(v[j], (uint32_t)top_shift)], 1u);
+  }
+  __syncthreads();
+  sample[blockIdx.x * 256 + tid] = s_h[tid];
+  if (blockIdx.x == 0 && tid == 0) {
+    SelState t{};
+    t.k = k;
+    t.cand = n;
+    t.cap = cap;
+    t.shift = top_shift;
+    *st = t;
+  }
+}
+
+// The band from the sample (every block of the first read computes the same
+// one): the answer's rank k maps to sample rank k * S / n, whose sampling
+// error has a standard deviation of at most sqrt(S) / 2, so the bins holding
+// sample ranks within 4 of those deviations of it (a contiguous range) hold
+// the answer with near certainty.  The band is used only when the sample puts
+// at most a quarter of the keys in it, so its keys fit the compaction buffer
+// (cap = n / 2) with room to spare; a miss or an overflow only costs the
+// second read the select would do anyway.  Threads 0..255 take part; s_band
+// receives (lo, hi, on).
+__device__ void sample_band(const uint32_t* __restrict__ sample, uint64_t k, uint64_t n, uint32_t* s_h,
+                            uint32_t* s_wsum, uint32_t* s_band) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t S = n < kSelSample ? n : kSelSample;
+  uint32_t c = 0, inc = 0;
+  if (tid < 256) {
+    uint32_t p[kSelSampleBlocks];
+#pragma unroll
+    for (uint32_t b = 0; b < kSelSampleBlocks; ++b) p[b] = sample[b * 256 + tid];
+#pragma unroll
+    for (uint32_t b = 0; b < kSelSampleBlocks; ++b) c += p[b];
+    s_h[tid] = c;
+    inc = dev::wave_incl_scan(c, lane);
+    if (lane == 63) s_wsum[wave] = inc;
+    if (tid < 2) s_band[tid] = tid ? 255u : 0u;
+  }
+  __syncthreads();
+  const uint64_t ks = k * S / n;
+  uint32_t delta = 1;
+  while ((uint64_t)delta * delta < S) ++delta;  // ceil(sqrt(S))
+  delta = 2 * delta + 1;                        // 4 deviations of at most sqrt(S) / 2
+  const uint64_t lo_r = ks > delta ? ks - delta : 0, hi_r = ks + delta < S ? ks + delta : S - 1;
+  if (tid < 256) {
+    uint32_t pre = 0;
+    for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
+    const uint64_t below = pre + inc - c;
+    if (c > 0 && lo_r >= below && lo_r < below + c) s_band[0] = tid;
+    if (c > 0 && hi_r >= below && hi_r < below + c) s_band[1] = tid;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint64_t in_band = 0;
+    for (uint32_t b = s_band[0]; b <= s_band[1]; ++b) in_band += s_h[b];
+    s_band[2] = S == kSelSample && in_band * 4 <= S;
+  }
+  __syncthreads();
 }
 
 // Finds the bin b of `row` (256 counts, in s_row) that holds rank k; returns it
@@ -96,19 +194,117 @@ __device__ __forceinline__ void flush_counts(const uint32_t* s_hist, uint32_t* g
 template <typename K>
 __device__ __forceinline__ K umax(K a, K b) { return a > b ? a : b; }
 
+// Per-wave compaction of matching keys into buf.  Each wave stages its
+// matches in its own LDS slice and flushes the slice with one global atomic
+// when a step's matches would not fit; finish() sends every wave's last slice
+// out under one counter reservation per block (every wave reaches it at once,
+// and thousands of same-address atomics at the end of a launch would
+// serialise).  Keys whose slot is past cap are dropped; the counter still
+// counts them, so a reader sees the overflow.  All lanes of a wave call take()
+// together (the loops bound their iterations by the wave's first index).
+template <typename K, int U>
+struct WaveCompactor {
+  static constexpr int PER = 16 / sizeof(K);
+  static constexpr uint32_t STAGE = 64 * U * PER;  // one step's worst case per wave
+  K* stage;
+  K* buf;
+  uint64_t cap;
+  unsigned long long* ctr;
+  uint32_t lane;
+  uint32_t staged = 0;  // wave-uniform
+
+  __device__ void put(unsigned long long base) {
+    for (uint32_t j = lane; j < staged; j += 64)
+      if (base + j < cap) buf[base + j] = stage[j];
+    staged = 0;
+  }
+  __device__ void flush() {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(ctr, (unsigned long long)staged);
+    put(__shfl(base, 0, 64));
+  }
+  // Appends this lane's matches (bit u * PER + j of hits: key j of vector u).
+  __device__ void take(const uint4 (&x)[U], uint32_t hits) {
+    if (!__any(hits != 0)) return;
+    const uint32_t nm = (uint32_t)__popc(hits);
+    const uint32_t inc = dev::wave_incl_scan(nm, lane);
+    const uint32_t wtot = __shfl(inc, 63, 64);
+    if (wtot == 0) return;
+    if (staged + wtot > STAGE) flush();
+    __builtin_amdgcn_wave_barrier();
+    uint32_t pos = staged + inc - nm;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      K xs[PER];
+      __builtin_memcpy(xs, &x[u], 16);
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        const bool hit = (hits >> (u * PER + j)) & 1u;
+        if (__any(hit) && hit) stage[pos++] = xs[j];  // no LDS instruction when no lane of the wave hits
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    staged += wtot;
+  }
+  // Block-wide (every thread calls it): the waves' last slices.
+  template <int WAVES>
+  __device__ void finish(uint32_t* s_wcnt, unsigned long long* s_bbase, uint32_t tid, uint32_t wave) {
+    if (lane == 0) s_wcnt[wave] = staged;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t total = 0;
+      for (int w = 0; w < WAVES; ++w) total += s_wcnt[w];
+      *s_bbase = total ? atomicAdd(ctr, (unsigned long long)total) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long base = *s_bbase;
+    for (uint32_t w = 0; w < wave; ++w) base += s_wcnt[w];
+    put(base);
+  }
+};
+
 // One read of all keys: the top digit's histogram, and each block's minimum
 // and maximum key (kept as max(~key) and max(key)) in mm[2 * block] and
-// mm[2 * block + 1].  Four 16-byte loads per thread are in flight per step.
+// mm[2 * block + 1].  When the band is on, block b also compacts the keys of
+// its share whose top digit lies in the band into its own region of buf
+// (cap / grid keys at buf + b * (cap / grid); band_counts[b] keys, more than
+// the region when it overflowed), so no counter is shared between blocks.
+// Four 16-byte loads per thread are in flight per step.
 template <typename K>
 __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict__ keys, size_t n,
-                                                            uint32_t* __restrict__ ghist, K* __restrict__ mm) {
+                                                            uint32_t* __restrict__ ghist, K* __restrict__ mm,
+                                                            SelState* __restrict__ S, K* __restrict__ buf,
+                                                            uint32_t* __restrict__ band_counts,
+                                                            const uint32_t* __restrict__ sample) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
   constexpr int U = 4;
+  constexpr int WAVES = kSelBlock / 64;
+  using C = WaveCompactor<K, U>;
+  using H = dev::SubHist<32, false>;
   __shared__ uint32_t s_hist[256];
-  __shared__ K s_mm[kSelBlock / 64][2];
+  __shared__ __align__(16) uint32_t s_sub[H::words(256)];
+  __shared__ K s_mm[WAVES][2];
+  __shared__ K s_stage[WAVES][C::STAGE];
+  __shared__ uint32_t s_wcnt[WAVES];
+  __shared__ unsigned long long s_bbase;
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_band[3];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid < 256) s_hist[tid] = 0;
+  const uint64_t k = S->k, cap = S->cap;
+  sample_band(sample, k, n, s_hist, s_wsum, s_band);
+  const uint32_t band = s_band[2];
+  const uint32_t band_lo = s_band[0], band_w = s_band[1] - s_band[0];
+  if (blockIdx.x == 0 && tid == 0) {
+    S->band_lo = s_band[0];
+    S->band_hi = s_band[1];
+    S->band_on = band;
+  }
+  __shared__ unsigned long long s_ctr;
+  if (tid == 0) s_ctr = 0;
+  const uint64_t region = cap / gridDim.x;
+  C comp{s_stage[wave], buf + (size_t)blockIdx.x * region, region, &s_ctr, lane};
+  for (uint32_t i = tid; i < H::words(256) / 4; i += kSelBlock) reinterpret_cast<uint4*>(s_sub)[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
   K lo_c = 0, hi = 0;  // max(~key), max(key) over this thread's keys
   size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
@@ -116,8 +312,13 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   const size_t nvec = (n - head) / PER;
   const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
   const size_t stride = (size_t)gridDim.x * kSelBlock;
-  for (size_t v0 = (size_t)blockIdx.x * kSelBlock + tid; v0 < nvec; v0 += stride * U) {
+  // Every lane of a wave runs the same iterations (the bound uses the wave's
+  // first index), so the wave-level compaction stays convergent.
+  const size_t wfirst = (size_t)blockIdx.x * kSelBlock + wave * 64;
+  for (size_t w0 = wfirst; w0 < nvec; w0 += stride * U) {
+    const size_t v0 = w0 + lane;
     uint4 x[U];
+    uint32_t hits = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (v0 + u * stride < nvec) x[u] = vk[v0 + u * stride];
@@ -132,19 +333,36 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
         dg[j] = dev::digit8<K>(kk[j], 8 * (D - 1));
         lo_c = umax<K>(lo_c, (K)~kk[j]);
         hi = umax<K>(hi, kk[j]);
+        hits |= (uint32_t)(dg[j] - band_lo <= band_w) << (u * PER + j);
       }
-      dev::lds_count<PER>(s_hist, dg);
+      H::template add_all<PER>(s_sub, dg, lane);
     }
+    if (band) comp.take(x, hits);
   }
-  if (blockIdx.x == 0) {
+  // Head and tail elements outside whole vectors: wave 0 of block 0.
+  if (blockIdx.x == 0 && wave == 0) {
     const size_t tail = head + nvec * PER;
     const size_t extra = head + (n - tail);
-    for (size_t e = tid; e < extra; e += kSelBlock) {
-      const K key = keys[e < head ? e : tail + (e - head)];
-      atomicAdd(&s_hist[dev::digit8<K>(key, 8 * (D - 1))], 1u);
-      lo_c = umax<K>(lo_c, (K)~key);
-      hi = umax<K>(hi, key);
+    for (size_t e0 = 0; e0 < extra; e0 += 64) {
+      const size_t e = e0 + lane;
+      uint4 x[U] = {};
+      uint32_t hits = 0;
+      if (e < extra) {
+        const K key = keys[e < head ? e : tail + (e - head)];
+        __builtin_memcpy(&x[0], &key, sizeof(K));
+        const uint32_t dg = dev::digit8<K>(key, 8 * (D - 1));
+        H::add(s_sub, dg, lane, 1u);
+        lo_c = umax<K>(lo_c, (K)~key);
+        hi = umax<K>(hi, key);
+        hits = dg - band_lo <= band_w;
+      }
+      if (band) comp.take(x, hits);
     }
+  }
+  if (band) {
+    comp.template finish<WAVES>(s_wcnt, &s_bbase, tid, wave);
+    __syncthreads();
+    if (tid == 0) band_counts[blockIdx.x] = (uint32_t)(s_ctr < 0xFFFFFFFFull ? s_ctr : 0xFFFFFFFFull);
   }
   // Wave maxima, then the block's, stored without atomics (the next launch
   // reduces the blocks' values).
@@ -159,14 +377,17 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   }
   __syncthreads();
   if (tid == 0) {
-    for (int w = 1; w < kSelBlock / 64; ++w) {
+    for (int w = 1; w < WAVES; ++w) {
       lo_c = umax<K>(lo_c, s_mm[w][0]);
       hi = umax<K>(hi, s_mm[w][1]);
     }
     mm[2 * blockIdx.x] = lo_c;
     mm[2 * blockIdx.x + 1] = hi;
   }
-  flush_counts(s_hist, ghist, 256);
+  if (tid < 256) {
+    const uint32_t c = H::sum(s_sub, tid);
+    if (c) atomicAdd(&ghist[tid], c);
+  }
 }
 
 // The state a pass works with, computed by every block of that pass from the
@@ -255,7 +476,8 @@ __device__ void pick_state(const uint32_t* s_row, SelState& t, bool compacted, u
 template <typename K>
 __device__ SelState step_state(int step, const SelState* __restrict__ S, const uint32_t* __restrict__ ghist_first,
                                const uint32_t* __restrict__ ghist_prev,
-                               const unsigned long long* __restrict__ ctr_prev, SelState* __restrict__ S_next,
+                               const unsigned long long* __restrict__ ctr_prev,
+                               const uint32_t* __restrict__ band_counts, SelState* __restrict__ S_next,
                                uint32_t* s_hist, SelShared& sh) {
   SelState t = S[step];
   if (step == 0) {
@@ -285,6 +507,27 @@ __device__ SelState step_state(int step, const SelState* __restrict__ S, const u
       hi = umax<K>(hi, (K)sh.mm[w][1]);
     }
     walk_state<K>(s_hist, (K)~lo_c, hi, t, sh);
+    // The first read compacted the keys of the band's top-digit bins into
+    // block regions: when the answer's top digit is among them and no region
+    // overflowed, every candidate is in the buffer and no later pass reads
+    // the whole input (block b of each pass reads region b).
+    constexpr uint32_t top = 8 * (sizeof(K) - 1);
+    const uint32_t region = (uint32_t)(t.cap / gridDim.x);
+    __shared__ uint32_t s_over;
+    if (threadIdx.x == 0) s_over = 0;
+    __syncthreads();
+    if (t.band_on)
+      for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+        if (band_counts[i] > region) s_over = 1;
+    __syncthreads();
+    if (!t.done && t.band_on && !s_over && ((t.pmask >> top) & 0xFF) == 0xFF) {
+      const uint32_t b = (uint32_t)(t.prefix >> top) & 0xFF;
+      if (b >= t.band_lo && b <= t.band_hi) {
+        t.src = 2;
+        t.buf_count = region;
+        t.compact_now = 0;
+      }
+    }
   } else if (!t.done) {
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_hist[i] = ghist_prev[i];
     __syncthreads();
@@ -305,67 +548,41 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
                                                            K* __restrict__ buf, int step, SelState* __restrict__ S,
                                                            const uint32_t* __restrict__ ghist_first,
                                                            uint32_t* __restrict__ ghist,
-                                                           unsigned long long* __restrict__ ctr) {
+                                                           unsigned long long* __restrict__ ctr,
+                                                           const uint32_t* __restrict__ band_counts) {
   constexpr int PER = 16 / sizeof(K);
   constexpr int U = 4;
   constexpr int WAVES = kSelBlock / 64;
-  constexpr uint32_t STAGE = 64 * U * PER;  // one step's worst case per wave
+  using C = WaveCompactor<K, U>;
   __shared__ uint32_t s_hist[256];
-  __shared__ K s_stage[WAVES][STAGE];
+  __shared__ K s_stage[WAVES][C::STAGE];
+  __shared__ uint32_t s_wcnt[WAVES];
+  __shared__ unsigned long long s_bbase;
   __shared__ SelShared sh;
   const SelState t = step_state<K>(step, S, ghist_first, step ? ghist + (step - 1) * 256 : nullptr,
-                                   step ? ctr + (step - 1) : nullptr, S + step + 1, s_hist, sh);
+                                   step ? ctr + (step - 1) : nullptr, band_counts, S + step + 1, s_hist, sh);
   if (t.done) return;  // uniform for the whole grid
   uint32_t* const gh = ghist + step * 256;
-  unsigned long long* const buf_ctr = ctr + step;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const K prefix = (K)t.prefix, pmask = (K)t.pmask;
   const uint32_t shift = (uint32_t)t.shift;
   const bool from_buf = t.src != 0;
+  const bool regions = t.src == 2;  // block b reads region b of the buffer
   const bool compact = !from_buf && t.compact_now;
-  const K* src = from_buf ? buf : keys;
-  const size_t m = from_buf ? (size_t)t.buf_count : n;
+  const K* src = regions ? buf + (size_t)blockIdx.x * t.buf_count : (from_buf ? buf : keys);
+  const size_t m = regions ? band_counts[blockIdx.x] : (from_buf ? (size_t)t.buf_count : n);
+  C comp{s_stage[wave], buf, t.cap, ctr + step, lane};
   __syncthreads();  // s_hist held the previous histogram until here
   if (tid < 256) s_hist[tid] = 0;
   __syncthreads();
-  K* stage = s_stage[wave];
-  uint32_t staged = 0;  // wave-uniform
-  auto flush = [&]() {
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(buf_ctr, (unsigned long long)staged);
-    base = __shfl(base, 0, 64);
-    for (uint32_t j = lane; j < staged; j += 64) buf[base + j] = stage[j];
-    staged = 0;
-  };
-  // Appends this lane's matches (bit u*PER+j of hits: key j of vector u) to
-  // the wave's slice, flushing first if the slice cannot hold the step.
-  auto take = [&](const uint4 (&x)[U], uint32_t hits) {
-    const uint32_t nm = (uint32_t)__popc(hits);
-    const uint32_t inc = dev::wave_incl_scan(nm, lane);
-    const uint32_t wtot = __shfl(inc, 63, 64);
-    if (wtot == 0) return;
-    if (staged + wtot > STAGE) flush();
-    __builtin_amdgcn_wave_barrier();
-    uint32_t pos = staged + inc - nm;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      K xs[PER];
-      __builtin_memcpy(xs, &x[u], 16);
-#pragma unroll
-      for (int j = 0; j < PER; ++j)
-        if ((hits >> (u * PER + j)) & 1u) stage[pos++] = xs[j];
-    }
-    __builtin_amdgcn_wave_barrier();
-    staged += wtot;
-  };
   size_t head = ((16 - ((uintptr_t)src & 15)) & 15) / sizeof(K);
   if (head > m) head = m;
   const size_t nvec = (m - head) / PER;
   const uint4* vk = reinterpret_cast<const uint4*>(src + head);
-  const size_t stride = (size_t)gridDim.x * kSelBlock;
+  const size_t stride = regions ? (size_t)kSelBlock : (size_t)gridDim.x * kSelBlock;
   // Every lane of a wave runs the same iterations (the bound uses the wave's
   // first index), so the wave-level staging stays convergent.
-  const size_t wfirst = (size_t)blockIdx.x * kSelBlock + wave * 64;
+  const size_t wfirst = (regions ? 0 : (size_t)blockIdx.x * kSelBlock) + wave * 64;
   for (size_t w0 = wfirst; w0 < nvec; w0 += stride * U) {
     const size_t v0 = w0 + lane;
     uint4 x[U];
@@ -393,10 +610,11 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
       }
       hits |= h << (u * PER);
     }
-    if (compact) take(x, hits);
+    if (compact) comp.take(x, hits);
   }
-  // Head and tail elements outside whole vectors (at most 2 * PER - 2): wave 0 of block 0.
-  if (blockIdx.x == 0 && wave == 0) {
+  // Head and tail elements outside whole vectors (at most 2 * PER - 2): wave 0
+  // of block 0, or of every block when each reads its own region.
+  if ((regions || blockIdx.x == 0) && wave == 0) {
     const size_t tail = head + nvec * PER;
     const size_t extra = head + (m - tail);
     for (size_t e0 = 0; e0 < extra; e0 += 64) {
@@ -411,27 +629,10 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
           hits = 1;
         }
       }
-      if (compact) take(x, hits);
+      if (compact) comp.take(x, hits);
     }
   }
-  if (compact) {
-    // The waves' last slices go out under one counter reservation per block:
-    // every wave reaches this point at once, and thousands of same-address
-    // atomics at the end of the launch would serialise.
-    __shared__ uint32_t s_wcnt[WAVES];
-    __shared__ unsigned long long s_bbase;
-    if (lane == 0) s_wcnt[wave] = staged;
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t total = 0;
-      for (int w = 0; w < WAVES; ++w) total += s_wcnt[w];
-      s_bbase = total ? atomicAdd(buf_ctr, (unsigned long long)total) : 0ull;
-    }
-    __syncthreads();
-    unsigned long long base = s_bbase;
-    for (uint32_t w = 0; w < wave; ++w) base += s_wcnt[w];
-    for (uint32_t j = lane; j < staged; j += 64) buf[base + j] = stage[j];
-  }
+  if (compact) comp.template finish<WAVES>(s_wcnt, &s_bbase, tid, wave);
   __syncthreads();
   flush_counts(s_hist, gh, 256);
 }
@@ -462,20 +663,26 @@ __global__ __launch_bounds__(256) void k_select_final(const SelState* __restrict
 
 // Scratch: the D per-step states, then the zeroed control words (the top
 // digit's histogram from the first read, the per-block minima and maxima,
-// one histogram per pass, one 64-bit compaction counter per pass), then the
-// compaction buffer.
+// one histogram per pass, one 64-bit compaction counter per pass and the
+// band's), then the compaction buffer.
 constexpr unsigned kSelMaxGrid = 512;  // blocks of the first read and the passes
 constexpr size_t kSelMinMaxWords = 2 * kSelMaxGrid * 2;  // two 64-bit words per block
 template <typename K>
 static constexpr size_t select_zero_words() {
   return 256 + kSelMinMaxWords + (sizeof(K) - 1) * 256 + 2 * (sizeof(K) - 1);
 }
+// The sample's per-block counts and the band's per-block key counts follow
+// the cleared words (written in full before they are read, never cleared).
+template <typename K>
+static constexpr size_t select_control_words() {
+  return select_zero_words<K>() + kSelSampleBlocks * 256 + kSelMaxGrid;
+}
 
 template <typename K>
 static size_t select_bytes(size_t n) {
   Layout L;
   L.take(sizeof(K) * sizeof(SelState));
-  L.take(select_zero_words<K>() * 4);
+  L.take(select_control_words<K>() * 4);
   L.take(std::max<size_t>(n / 2, 1) * sizeof(K));
   return L.off;
 }
@@ -492,19 +699,21 @@ static void launch_select(const K* din, size_t n, size_t k, SelState* S, uint32_
   uint32_t* ghist_first = z;  // followed by the minimum and maximum words
   uint32_t* ghist = z + 256 + kSelMinMaxWords;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ghist + (D - 1) * 256);
-  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, S, (uint64_t)k, (uint64_t)n, (uint64_t)cap,
-                     (int32_t)(8 * (D - 1)), z, nz);
+  uint32_t* sample = reinterpret_cast<uint32_t*>(ctr + (D - 1));
+  uint32_t* band_counts = sample + kSelSampleBlocks * 256;
+  hipLaunchKernelGGL(k_select_init<K>, dim3(kSelSampleBlocks), dim3(256), 0, s, S, din, (uint64_t)k, (uint64_t)n,
+                     (uint64_t)cap, (int32_t)(8 * (D - 1)), z, nz, sample);
   const size_t nvec = n / (16 / sizeof(K));
   const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + 4 * kSelBlock - 1) / (4 * kSelBlock), kSelMaxGrid));
   {
     ProfScope ps("k_select_first", s);
     hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghist_first,
-                       reinterpret_cast<K*>(ghist_first + 256));
+                       reinterpret_cast<K*>(ghist_first + 256), S, buf, band_counts, sample);
   }
   for (int step = 0; step < D - 1; ++step) {
     ProfScope ps("k_select_pass", s);
     hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, step, S, ghist_first, ghist,
-                       ctr);
+                       ctr, band_counts);
   }
   hipLaunchKernelGGL(k_select_final<K>, dim3(1), dim3(256), 0, s, S + (D - 1), ghist + (D - 2) * 256, ctr + (D - 2),
                      dst, place);
@@ -516,7 +725,7 @@ static int select_enqueue(const K* din, size_t n, size_t k, char* b, K* value_ou
   const size_t nz = select_zero_words<K>();
   Layout L;
   SelState* S = (SelState*)(b + L.take(sizeof(K) * sizeof(SelState)));
-  uint32_t* z = (uint32_t*)(b + L.take(nz * 4));
+  uint32_t* z = (uint32_t*)(b + L.take(select_control_words<K>() * 4));
   K* buf = (K*)(b + L.take(cap * sizeof(K)));
   launch_select<K>(din, n, k, S, z, (uint32_t)nz, buf, cap, value_out, nullptr, s);
   PA_HIP(hipGetLastError());
@@ -541,7 +750,7 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
   Layout L;
   const size_t o_st = L.take(sizeof(K) * sizeof(SelState));
   const size_t nz = select_zero_words<K>();
-  const size_t o_z = L.take(nz * 4);
+  const size_t o_z = L.take(select_control_words<K>() * 4);
   const size_t o_res = L.take(sizeof(K));
   const size_t o_buf = L.take(cap * sizeof(K));
   const size_t o_in = dk ? 0 : L.take(n * sizeof(K));

@@ -56,6 +56,28 @@ def test_select_device_result_and_host_keys(pa, orc):
     assert int(res[0]) == expect
 
 
+def test_select_sample_band_miss_falls_back(pa, orc):
+    """The select predicts the answer's top-byte bins from runs of keys at an
+    even stride.  Keys at exactly those positions are drawn from the low
+    quarter of the key range and every other key from the high quarter, so
+    the predicted bins miss the answer (and, for other ranks, hold it): the
+    result must not depend on the prediction."""
+    n = 1_000_003
+    runs, run = 256, 128  # the sampling layout: 256 runs of 128 keys
+    import parallelalgorithms_amd as P
+    rnd = P.gen_u32(77, n)
+    a = (rnd & np.uint32(0x3FFFFFFF)) | np.uint32(0xC0000000)
+    for i in range(runs):
+        base = i * (n - run) // (runs - 1)
+        a[base:base + run] = rnd[base:base + run] & np.uint32(0x3FFFFFFF)
+    d = to_dev(a)
+    for k in (0, 100, n // 100, n // 2, n - 1):
+        expect, _ = orc.select_radix_u32(a, k)
+        res = np.zeros(1, dtype=np.uint32)
+        pa.select_u32(d, k, res)
+        assert int(res[0]) == expect, k
+
+
 def test_select_radix_host_keys_device_result(pa, orc):
     """SelectRadix's post-condition a[k] == k-th smallest with the keys in
     host memory and the result in device memory: the value reaches both."""
