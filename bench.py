@@ -167,12 +167,19 @@ def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
     of the top byte, sizes log-uniform 4K..64M, keys u32 or u64 (50/50),
     Poisson arrivals at fractions of the measured saturation rate, 64 GB
     admission budget.  Latency = completion - submission.  Every level replays
-    the same jobs; their output digests must agree across levels."""
+    the same jobs; their output digests must agree across levels.  The same
+    jobs also run as plain per-job calls on one stream (the baseline the
+    server's saturation throughput is compared with, digests equal), and
+    once more through a server whose budget (1 GiB) is below the stream's
+    largest jobs side by side, so admission waits and releases."""
     from parallelalgorithms_amd import trace
     base = trace.generate(jobs, seed)
     trace.replay(base, paced=False, digests=False)  # warm-up (allocations, code objects)
     ref, st = trace.replay(base, paced=False)
     sat = st["jobs_per_s"]
+    trace.replay_plain(base, digests=False)  # warm-up
+    plain_res, plain = trace.replay_plain(base)
+    tight_res, tight = trace.replay(base, budget_bytes=1 << 30, paced=False)
     levels = []
     same = True
     for load in loads:
@@ -182,7 +189,15 @@ def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):
                        "achieved_jobs_per_s": st["jobs_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
                        "gkeys_per_s": round(st["gkeys_per_s"], 2), "coalesced_jobs": st["coalesced_jobs"]})
     return {"workload": f"{jobs} jobs: 70% sort, 20% select, 10% hist; u32/u64 keys 50/50; sizes log-uniform 4K..64M",
-            "saturation_jobs_per_s": round(sat, 1), "levels": levels, "outputs_equal_across_levels": same}
+            "saturation_jobs_per_s": round(sat, 1), "levels": levels, "outputs_equal_across_levels": same,
+            "plain_per_job": {"jobs_per_s": plain["jobs_per_s"], "span_s": plain["span_s"],
+                              "gkeys_per_s": plain["gkeys_per_s"],
+                              "server_saturation_over_plain": round(sat / plain["jobs_per_s"], 3),
+                              "outputs_equal_plain": not trace.compare(ref, plain_res)["outputs_differ"]},
+            "budget_1gib": {"jobs_per_s": tight["jobs_per_s"], "peak_bytes_in_flight": tight["peak_bytes_in_flight"],
+                            "admission_waits": tight["admission_waits"],
+                            "scratch_releases": tight["scratch_releases"], "oversize_units": tight["oversize_units"],
+                            "outputs_equal": not trace.compare(ref, tight_res)["outputs_differ"]}}
 
 
 def bench_configs(pa, torch, n=100_000_000, reps=5):

@@ -284,13 +284,19 @@ typedef struct pa_server_stats {
   uint64_t launches;           /* units launched (a coalesced batch counts once) */
   uint64_t coalesced_batches;  /* batches of two or more small sorts */
   uint64_t coalesced_jobs;     /* jobs that ran inside such batches */
-  uint64_t peak_bytes_in_flight;
+  uint64_t peak_bytes_in_flight; /* peak device scratch held by the server's streams */
   uint64_t pending;            /* jobs waiting for admission */
+  uint64_t admission_waits;    /* units that waited for units in flight to fit the budget */
+  uint64_t scratch_releases;   /* idle streams' scratch blocks given back to fit a unit */
+  uint64_t oversize_units;     /* units larger than the budget alone, run by themselves */
+  uint64_t bytes_held;         /* device scratch the server's streams hold now */
 } pa_server_stats;
 
 typedef struct pa_server pa_server;
 
-/* hbm_budget_bytes: device scratch admitted at once (0 = 64 GiB);
+/* hbm_budget_bytes: bound on the device scratch the library holds for the
+ * server's streams at once (0 = 64 GiB); the job arrays are the caller's and
+ * are not counted;
  * num_streams: concurrent HIP streams; coalesce_max_keys: u32 sorts up to
  * this size are batched together (0 = 1 Mi keys).  NULL on failure. */
 pa_server* pa_server_create(size_t hbm_budget_bytes, int num_streams, size_t coalesce_max_keys);
@@ -301,6 +307,11 @@ int pa_server_wait(pa_server* server, uint64_t ticket, double* latency_ms);
 int pa_server_drain(pa_server* server);
 int pa_server_get_stats(pa_server* server, pa_server_stats* out);
 void pa_server_destroy(pa_server* server);
+/* Device scratch (bytes) the library allocates for this job run alone on a
+ * stream holding none: the figure the server's admission charges.  Computed
+ * from the same layout functions the sort, select, histogram entry points
+ * allocate with; the job's arrays are not included. */
+int pa_job_scratch_bytes(const pa_job* job, size_t* bytes);
 
 /* ---- Measurement helpers (no reference counterpart) ------------------------ */
 

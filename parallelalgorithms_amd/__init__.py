@@ -74,6 +74,7 @@ _SIGS = {
     "pa_server_drain": (_i, [_vp]),
     "pa_server_get_stats": (_i, [_vp, _vp]),
     "pa_server_destroy": (None, [_vp]),
+    "pa_job_scratch_bytes": (_i, [_vp, _vp]),
     "pa_profile_enable": (_i, [_i]),
     "pa_profile_only": (_i, [ctypes.c_char_p]),
     "pa_profile_query": (_i, [ctypes.c_char_p, _vp, _vp]),
@@ -427,7 +428,8 @@ class Job(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in ("submitted", "completed", "failed", "launches",
                                                       "coalesced_batches", "coalesced_jobs",
-                                                      "peak_bytes_in_flight", "pending")]
+                                                      "peak_bytes_in_flight", "pending", "admission_waits",
+                                                      "scratch_releases", "oversize_units", "bytes_held")]
 
 
 class Server:
@@ -438,6 +440,15 @@ class Server:
         self.h = self.lib.pa_server_create(budget_bytes, streams, coalesce_max_keys)
         if not self.h:
             raise PAError("pa_server_create", 3, self.lib.pa_last_error_string().decode())
+
+    @staticmethod
+    def job_scratch_bytes(op, n, key_type=KEY_U32, bits=8):
+        """Device scratch the library allocates for one job run alone
+        (pa_job_scratch_bytes): what the server's admission charges."""
+        job = Job(op, key_type, None, n, 0, 0, bits, None)
+        b = ctypes.c_size_t(0)
+        call("pa_job_scratch_bytes", ctypes.addressof(job), ctypes.addressof(b))
+        return b.value
 
     def submit(self, op, keys, key_type=KEY_U32, k=0, shift=0, bits=8, out=None):
         job = Job(op, key_type, _ptr(keys), numel(keys), k, shift, bits, _ptr(out))

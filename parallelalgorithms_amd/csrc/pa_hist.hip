@@ -306,6 +306,25 @@ int pa_hist_digits_u32(const uint32_t* keys, size_t n, uint64_t* counts, pa_stre
 
 namespace pa {
 
+// Scratch layout of one pa_hist_u32 / pa_hist_u64 call: u32 and u64 counts,
+// then (host keys) the staged input.
+struct HistEntryLayout {
+  size_t o_c32, o_c64, o_in, total;
+};
+static HistEntryLayout hist_entry_layout(uint32_t nb, size_t in_bytes) {
+  HistEntryLayout Y;
+  Layout L;
+  Y.o_c32 = L.take(nb * 4);
+  Y.o_c64 = L.take(nb * 8);
+  Y.o_in = in_bytes ? L.take(in_bytes) : 0;
+  Y.total = L.off;
+  return Y;
+}
+
+size_t hist_call_scratch_bytes(unsigned bits) {
+  return hist_entry_layout(1u << (bits > 12 ? 12 : bits), 0).total;
+}
+
 template <typename K>
 static int hist_key_entry(const K* keys, size_t n, unsigned shift, unsigned bits, uint64_t* counts, hipStream_t s,
                           const char* who) {
@@ -318,15 +337,14 @@ static int hist_key_entry(const K* keys, size_t n, unsigned shift, unsigned bits
   const uint32_t nb = 1u << bits;
   const bool dk = n == 0 || is_device_ptr(keys);
   const bool dout = is_device_ptr(counts);
-  Layout L;
-  size_t o_c32 = L.take(nb * 4), o_c64 = L.take(nb * 8), o_in = dk ? 0 : L.take(n * sizeof(K));
+  const HistEntryLayout Y = hist_entry_layout(nb, dk ? 0 : n * sizeof(K));
   void* base;
-  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  PA_TRY(ensure_scratch(ctx, Y.total, &base));
   char* b = (char*)base;
-  uint32_t* c32 = (uint32_t*)(b + o_c32);
-  uint64_t* c64 = dout ? counts : (uint64_t*)(b + o_c64);
+  uint32_t* c32 = (uint32_t*)(b + Y.o_c32);
+  uint64_t* c64 = dout ? counts : (uint64_t*)(b + Y.o_c64);
   const void* din;
-  PA_TRY(stage_in(keys, n * sizeof(K), b + o_in, dk, s, &din));
+  PA_TRY(stage_in(keys, n * sizeof(K), b + Y.o_in, dk, s, &din));
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, c32, (size_t)nb);
   if (n) {
     ProfScope ps(sizeof(K) == 4 ? "k_hist_u32" : "k_hist_u64", s);

@@ -56,8 +56,22 @@ struct StreamCtx {
 };
 
 StreamCtx& stream_ctx(hipStream_t s);
-// Ensures ctx.scratch holds >= bytes; on growth waits for the stream first.
+// Ensures ctx.scratch holds >= bytes; on growth waits for the stream first,
+// frees the old block and allocates scratch_alloc_bytes(bytes).
 int ensure_scratch(StreamCtx& ctx, size_t bytes, void** out);
+size_t scratch_alloc_bytes(size_t bytes);
+// Device scratch a stream's context holds now; releasing it waits for the
+// stream and frees the block (the next call allocates again).
+size_t stream_scratch_held(hipStream_t s);
+int release_stream_scratch(hipStream_t s);
+
+// Scratch bytes (the ensure_scratch request) of one call of the sort,
+// select, histogram and pointer-list segmented sort entry points with device
+// keys, device outputs and no caller tmp: what the server admits jobs by.
+size_t sort_call_scratch_bytes(bool u64, size_t n);
+size_t select_call_scratch_bytes(bool u64, size_t n);
+size_t hist_call_scratch_bytes(unsigned bits);
+int segmented_ptrs_scratch_bytes(uint32_t* const* keys, const uint64_t* lens, size_t num, size_t* bytes);
 int ensure_pinned(StreamCtx& ctx, size_t bytes, void** out);
 // A pinned upload buffer of >= bytes whose previous copy has finished (waits
 // only if all ring slots are still in flight).  After enqueuing the copy that

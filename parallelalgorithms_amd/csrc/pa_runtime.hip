@@ -50,6 +50,11 @@ StreamCtx& stream_ctx(hipStream_t s) {
   return *c;
 }
 
+size_t scratch_alloc_bytes(size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  return align_up(bytes + bytes / 8, size_t(1) << 20);
+}
+
 int ensure_scratch(StreamCtx& ctx, size_t bytes, void** out) {
   if (bytes == 0) bytes = 256;
   if (ctx.scratch_bytes < bytes) {
@@ -59,11 +64,29 @@ int ensure_scratch(StreamCtx& ctx, size_t bytes, void** out) {
       ctx.scratch = nullptr;
       ctx.scratch_bytes = 0;
     }
-    size_t want = align_up(bytes + bytes / 8, size_t(1) << 20);
+    const size_t want = scratch_alloc_bytes(bytes);
     PA_HIP(hipMalloc(&ctx.scratch, want));
     ctx.scratch_bytes = want;
   }
   *out = ctx.scratch;
+  return PA_OK;
+}
+
+size_t stream_scratch_held(hipStream_t s) {
+  StreamCtx& ctx = stream_ctx(s);
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  return ctx.scratch_bytes;
+}
+
+int release_stream_scratch(hipStream_t s) {
+  StreamCtx& ctx = stream_ctx(s);
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  if (ctx.scratch) {
+    PA_HIP(hipStreamSynchronize(ctx.stream));
+    PA_HIP(hipFree(ctx.scratch));
+    ctx.scratch = nullptr;
+    ctx.scratch_bytes = 0;
+  }
   return PA_OK;
 }
 

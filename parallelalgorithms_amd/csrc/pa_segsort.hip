@@ -635,6 +635,23 @@ int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets
   return run_plan_alloc(P, tmp ? 0 : large_total * 4, tmp == nullptr, s, ctx);
 }
 
+}  // extern "C"
+
+namespace pa {
+int segmented_ptrs_scratch_bytes(uint32_t* const* keys, const uint64_t* lens, size_t num, size_t* bytes) {
+  *bytes = 0;
+  if (num == 0) return PA_OK;
+  if (!keys || !lens) return arg_error("segmented sort: null table");
+  SegPlan P;
+  uint64_t large_total = 0;
+  PA_TRY(build_ptr_plan(keys, lens, num, "segmented sort", &P, &large_total));
+  *bytes = plan_layout(P, large_total * 4).total;
+  return PA_OK;
+}
+}  // namespace pa
+
+extern "C" {
+
 int pa_segmented_sort_ptrs_u32(uint32_t* const* keys, const uint64_t* lens, size_t num_segments, pa_stream_t stream) {
   if (num_segments == 0) return PA_OK;
   if (!keys || !lens) return arg_error("pa_segmented_sort_ptrs_u32: null table");

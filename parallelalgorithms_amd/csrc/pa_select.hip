@@ -737,6 +737,30 @@ int select_enqueue_u32(const uint32_t* din, size_t n, size_t k, char* scratch, u
   return select_enqueue<uint32_t>(din, n, k, scratch, value_out, s);
 }
 
+// Scratch layout of one select call: states, control words, the device
+// result slot, the compaction buffer and (host keys) the staged input.
+struct SelectEntryLayout {
+  size_t o_st, o_z, o_res, o_buf, o_in, total;
+};
+template <typename K>
+static SelectEntryLayout select_entry_layout(size_t n, bool dk) {
+  const size_t cap = std::max<size_t>(n / 2, 1);
+  SelectEntryLayout Y;
+  Layout L;
+  Y.o_st = L.take(sizeof(K) * sizeof(SelState));
+  Y.o_z = L.take(select_control_words<K>() * 4);
+  Y.o_res = L.take(sizeof(K));
+  Y.o_buf = L.take(cap * sizeof(K));
+  Y.o_in = dk ? 0 : L.take(n * sizeof(K));
+  Y.total = L.off;
+  return Y;
+}
+
+size_t select_call_scratch_bytes(bool u64, size_t n) {
+  if (n == 0) return 0;
+  return u64 ? select_entry_layout<uint64_t>(n, true).total : select_entry_layout<uint32_t>(n, true).total;
+}
+
 template <typename K>
 static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, hipStream_t s, const char* who) {
   if (!keys || !result) return arg_error(who);
@@ -747,15 +771,11 @@ static int select_impl(const K* keys, size_t n, size_t k, K* result, K* place, h
   const bool dk = is_device_ptr(keys);
   const bool dres = is_device_ptr(result);
   const size_t cap = std::max<size_t>(n / 2, 1);
-  Layout L;
-  const size_t o_st = L.take(sizeof(K) * sizeof(SelState));
+  const SelectEntryLayout Y = select_entry_layout<K>(n, dk);
   const size_t nz = select_zero_words<K>();
-  const size_t o_z = L.take(select_control_words<K>() * 4);
-  const size_t o_res = L.take(sizeof(K));
-  const size_t o_buf = L.take(cap * sizeof(K));
-  const size_t o_in = dk ? 0 : L.take(n * sizeof(K));
+  const size_t o_st = Y.o_st, o_z = Y.o_z, o_res = Y.o_res, o_buf = Y.o_buf, o_in = Y.o_in;
   void* base;
-  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  PA_TRY(ensure_scratch(ctx, Y.total, &base));
   char* b = (char*)base;
   SelState* S = (SelState*)(b + o_st);
   uint32_t* z = (uint32_t*)(b + o_z);

@@ -12,9 +12,19 @@
 //   - small u32 sorts waiting together are coalesced into one segmented batch
 //     (pa_segmented_sort_ptrs_u32: 7 launches for the whole batch);
 //   - every other job is launched on its own;
-//   - a unit is admitted only while the estimated device scratch of the units
-//     in flight stays within the budget, and goes to the stream with the least
-//     work in flight, so new work overlaps work already on the GPU.
+//   - admission is priced in the library's own scratch: every stream holds
+//     the scratch block the library allocated for it (it grows to the largest
+//     call run there and is kept for reuse), and a unit is placed only where
+//     the sum over the server's streams after its call stays within the
+//     budget.  The call's size comes from the same layout functions the
+//     entry points allocate with (pa_job_scratch_bytes), so the prediction is
+//     the allocation.  When no stream fits, idle streams give their blocks
+//     back, and if that is not enough the dispatcher waits for units in
+//     flight to finish.  A unit larger than the whole budget runs alone,
+//     after every other block has been released (counted as oversize).
+//   - among the streams that fit, the one whose held block grows least wins,
+//     then the one with the least work in flight, so new work overlaps work
+//     already on the GPU.
 // A completer thread polls the units' events and stamps each job's
 // completion time; latency = completion - submission (host steady clock).
 // Each job's result is what the same call returns alone: units never share
@@ -23,6 +33,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -38,6 +49,7 @@ struct SJob {
   uint64_t ticket;
   Clock::time_point submit, done;
   int status = PA_OK;
+  std::string error;  // the call's message when status is not PA_OK
   bool finished = false;
 };
 
@@ -45,16 +57,20 @@ struct SUnit {
   std::vector<SJob*> jobs;
   int stream = 0;
   hipEvent_t ev = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;  // scratch the unit's call asks the library for
   bool batch = false;
 };
 
+// Units in flight per stream before the dispatcher looks for another stream.
+constexpr int kUnitsPerStream = 2;
+
 struct Server {
-  size_t budget = 0, in_use = 0, peak = 0;
+  size_t budget = 0, peak = 0;
   size_t coalesce_max = 1 << 20;     // largest sort that joins a batch
   size_t batch_max_keys = 64 << 20;  // keys per batch
   std::vector<hipStream_t> streams;
-  std::vector<int> busy;  // units in flight per stream
+  std::vector<int> busy;       // units in flight per stream
+  std::vector<size_t> held;    // scratch the library holds for each stream
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<SJob*> pending;
@@ -71,13 +87,45 @@ struct Server {
   pa_server_stats st{};
 };
 
-static size_t job_bytes(const pa_job& j) {
-  const size_t kb = j.key_type == PA_KEY_U64 ? 8 : 4;
+// Scratch the library's call for one job asks for (device-resident keys and
+// outputs, as the server requires).
+static size_t call_bytes(const pa_job& j) {
+  const bool u64 = j.key_type == PA_KEY_U64;
   switch (j.op) {
-    case PA_OP_SORT: return j.n * kb + j.n / 8192 * 8 * 1024 + (1 << 20);
-    case PA_OP_SELECT: return j.n / 2 * kb + (1 << 20);
-    default: return 1 << 20;
+    case PA_OP_SORT: return sort_call_scratch_bytes(u64, j.n);
+    case PA_OP_SELECT: return select_call_scratch_bytes(u64, j.n);
+    case PA_OP_HIST: return hist_call_scratch_bytes(j.bits);
+    default: return 0;
   }
+}
+
+// Scratch of a coalesced batch: the segmented sort's plan for these arrays.
+static size_t batch_bytes(const SUnit& u) {
+  std::vector<uint32_t*> ptrs;
+  std::vector<uint64_t> lens;
+  for (SJob* j : u.jobs) {
+    ptrs.push_back((uint32_t*)j->d.keys);
+    lens.push_back(j->d.n);
+  }
+  size_t b = 0;
+  if (segmented_ptrs_scratch_bytes(ptrs.data(), lens.data(), ptrs.size(), &b) != PA_OK) {
+    // The launch reports the error; price the batch as its keys plus a block.
+    b = 0;
+    for (SJob* j : u.jobs) b += j->d.n * 8;
+  }
+  return b;
+}
+
+// Block a stream holds after a call asking for `need` bytes: the library
+// keeps a block that is large enough and otherwise replaces it.
+static size_t held_after(size_t held, size_t need) {
+  return held >= need ? held : scratch_alloc_bytes(need);
+}
+
+static size_t total_held(const Server& S) {
+  size_t t = 0;
+  for (size_t h : S.held) t += h;
+  return t;
 }
 
 static bool coalescable(const Server& S, const SJob* j) {
@@ -100,6 +148,88 @@ static int launch_one(const pa_job& j, hipStream_t s) {
   }
 }
 
+// Picks the stream for a unit asking for `need` bytes, under S->mu.  Among
+// the streams with room for another unit whose held block after the call
+// keeps the total within the budget, it prefers one that does not grow a
+// block under a unit in flight (growing syncs the stream), then the least
+// busy one, then the least total.  Returns -1 when none fits.
+static int place(const Server& S, size_t need) {
+  const size_t total = total_held(S);
+  int best = -1;
+  bool best_grow = false;
+  size_t best_after = 0;
+  for (int i = 0; i < (int)S.streams.size(); ++i) {
+    if (S.busy[i] >= kUnitsPerStream) continue;
+    const size_t a = total - S.held[i] + held_after(S.held[i], need);
+    if (a > S.budget) continue;
+    const bool grow = S.held[i] < need && S.busy[i] > 0;
+    const bool better = best < 0 || grow < best_grow ||
+                        (grow == best_grow && (S.busy[i] < S.busy[best] ||
+                                               (S.busy[i] == S.busy[best] && a < best_after)));
+    if (better) {
+      best = i;
+      best_grow = grow;
+      best_after = a;
+    }
+  }
+  return best;
+}
+
+// Admission of one unit, entered and left with `lk` held; returns the
+// stream.  Every wait here ends: it waits only while units are in flight,
+// and the completer retires them.
+static int admit(Server* S, std::unique_lock<std::mutex>& lk, size_t need) {
+  bool waited = false;
+  std::vector<char> stuck(S->streams.size(), 0);  // streams whose release failed
+  for (;;) {
+    const int s = place(*S, need);
+    if (s >= 0) return s;
+    int in_flight = 0, any_room = -1, largest_idle = -1;
+    size_t busy_held = 0;
+    for (int i = 0; i < (int)S->streams.size(); ++i) {
+      in_flight += S->busy[i];
+      if (S->busy[i] > 0) busy_held += S->held[i];
+      if (S->busy[i] < kUnitsPerStream && any_room < 0) any_room = i;
+      if (S->busy[i] == 0 && S->held[i] > 0 && !stuck[i] &&
+          (largest_idle < 0 || S->held[i] > S->held[largest_idle]))
+        largest_idle = i;
+    }
+    // Would the unit fit once every idle stream's block is given back?
+    bool fits_after_release = false;
+    for (int i = 0; i < (int)S->streams.size(); ++i) {
+      if (S->busy[i] >= kUnitsPerStream) continue;
+      const size_t a = S->busy[i] ? busy_held - S->held[i] + held_after(S->held[i], need)
+                                  : busy_held + scratch_alloc_bytes(need);
+      if (a <= S->budget) fits_after_release = true;
+    }
+    if (largest_idle >= 0 && (fits_after_release || in_flight == 0)) {
+      // Give back the largest idle block and look again.  Only the
+      // dispatcher launches on the server's streams, so an idle stream
+      // stays idle while the lock is released.
+      hipStream_t st = S->streams[largest_idle];
+      lk.unlock();
+      if (release_stream_scratch(st) != PA_OK) (void)hipGetLastError();
+      const size_t h = stream_scratch_held(st);
+      lk.lock();
+      if (h == S->held[largest_idle]) stuck[largest_idle] = 1;
+      S->held[largest_idle] = h;
+      S->st.scratch_releases++;
+      continue;
+    }
+    if (in_flight > 0) {
+      // Wait for units in flight to finish (or for a stream's quota to free).
+      if (!waited && any_room >= 0) S->st.admission_waits++;
+      waited = waited || any_room >= 0;
+      S->cv_done.wait(lk);
+      continue;
+    }
+    // Nothing in flight and no idle block left to give back: the unit is
+    // larger than the budget on its own.  Run it alone.
+    S->st.oversize_units++;
+    return any_room >= 0 ? any_room : 0;
+  }
+}
+
 static void dispatch_loop(Server* S) {
   (void)hipSetDevice(S->device);
   std::unique_lock<std::mutex> lk(S->mu);
@@ -107,7 +237,7 @@ static void dispatch_loop(Server* S) {
     S->cv_work.wait(lk, [&] {
       if (S->stop) return true;
       if (S->pending.empty()) return false;
-      return *std::min_element(S->busy.begin(), S->busy.end()) < 2;
+      return *std::min_element(S->busy.begin(), S->busy.end()) < kUnitsPerStream;
     });
     if (S->stop && S->pending.empty()) return;
     if (S->pending.empty()) continue;
@@ -116,9 +246,12 @@ static void dispatch_loop(Server* S) {
     SUnit* u = new SUnit();
     SJob* head = S->pending.front();
     if (coalescable(*S, head)) {
+      // A batch's scratch is about 4 bytes a key: keep it within a quarter
+      // of the budget so batches never crowd out the other jobs.
+      const size_t max_keys = std::max(S->coalesce_max, std::min(S->batch_max_keys, S->budget / 16));
       size_t keys = 0;
       for (auto it = S->pending.begin(); it != S->pending.end();) {
-        if (coalescable(*S, *it) && keys + (*it)->d.n <= S->batch_max_keys) {
+        if (coalescable(*S, *it) && keys + (*it)->d.n <= max_keys) {
           keys += (*it)->d.n;
           u->jobs.push_back(*it);
           it = S->pending.erase(it);
@@ -131,16 +264,13 @@ static void dispatch_loop(Server* S) {
       u->jobs.push_back(head);
       S->pending.pop_front();
     }
-    for (SJob* j : u->jobs) u->bytes += job_bytes(j->d);
-
-    // Admission against the device budget (a unit larger than the whole
-    // budget runs alone once nothing else is in flight).
-    S->cv_done.wait(lk, [&] { return S->in_use == 0 || S->in_use + u->bytes <= S->budget || S->stop; });
-    S->in_use += u->bytes;
-    S->peak = std::max(S->peak, S->in_use);
-    u->stream = (int)(std::min_element(S->busy.begin(), S->busy.end()) - S->busy.begin());
-    S->busy[u->stream]++;
     S->dispatching++;
+    lk.unlock();
+    u->bytes = u->batch ? batch_bytes(*u) : call_bytes(u->jobs[0]->d);
+    lk.lock();
+
+    u->stream = admit(S, lk, u->bytes);
+    S->busy[u->stream]++;
     hipStream_t s = S->streams[u->stream];
     lk.unlock();
 
@@ -162,8 +292,15 @@ static void dispatch_loop(Server* S) {
       if (st == PA_OK) st = PA_ERR_HIP;
     }
 
+    const std::string msg = st == PA_OK ? std::string() : std::string(pa_last_error_string());
+    const size_t now_held = stream_scratch_held(s);
     lk.lock();
-    for (SJob* j : u->jobs) j->status = st;
+    S->held[u->stream] = now_held;
+    S->peak = std::max(S->peak, total_held(*S));
+    for (SJob* j : u->jobs) {
+      j->status = st;
+      j->error = msg;
+    }
     S->st.launches++;
     if (u->batch) {
       S->st.coalesced_batches++;
@@ -190,7 +327,10 @@ static void complete_loop(Server* S) {
         finished.push_back(u);
       } else if (e != hipErrorNotReady) {
         (void)hipGetLastError();
-        for (SJob* j : u->jobs) j->status = PA_ERR_HIP;
+        for (SJob* j : u->jobs) {
+          j->status = PA_ERR_HIP;
+          j->error = std::string("server: ") + hipGetErrorString(e);
+        }
         finished.push_back(u);
       }
     }
@@ -207,7 +347,6 @@ static void complete_loop(Server* S) {
         else
           S->st.failed++;
       }
-      S->in_use -= u->bytes;
       S->busy[u->stream]--;
       S->inflight.erase(std::find(S->inflight.begin(), S->inflight.end(), u));
       if (u->ev) (void)hipEventDestroy(u->ev);
@@ -243,6 +382,7 @@ pa_server* pa_server_create(size_t hbm_budget_bytes, int num_streams, size_t coa
     }
     S->streams.push_back(s);
     S->busy.push_back(0);
+    S->held.push_back(0);
   }
   S->dispatcher = std::thread(dispatch_loop, S);
   S->completer = std::thread(complete_loop, S);
@@ -281,6 +421,8 @@ int pa_server_wait(pa_server* srv, uint64_t ticket, double* latency_ms) {
   S->cv_done.wait(lk, [&] { return j->finished; });
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(j->done - j->submit).count();
   const int st = j->status;
+  // The message was recorded on the dispatcher's thread; hand it to this one.
+  if (st != PA_OK) set_last_error(j->error.empty() ? std::string("pa_server_wait: job failed") : j->error);
   S->jobs.erase(it);
   delete j;
   return st;
@@ -301,6 +443,15 @@ int pa_server_get_stats(pa_server* srv, pa_server_stats* out) {
   *out = S->st;
   out->peak_bytes_in_flight = S->peak;
   out->pending = S->pending.size();
+  out->bytes_held = total_held(*S);
+  return PA_OK;
+}
+
+int pa_job_scratch_bytes(const pa_job* job, size_t* bytes) {
+  if (!job || !bytes) return arg_error("pa_job_scratch_bytes: null argument");
+  if (job->op != PA_OP_SORT && job->op != PA_OP_SELECT && job->op != PA_OP_HIST)
+    return arg_error("pa_job_scratch_bytes: unknown op");
+  *bytes = scratch_alloc_bytes(call_bytes(*job));
   return PA_OK;
 }
 
@@ -316,7 +467,11 @@ void pa_server_destroy(pa_server* srv) {
   S->cv_done.notify_all();
   S->dispatcher.join();
   S->completer.join();
-  for (hipStream_t s : S->streams) (void)hipStreamDestroy(s);
+  for (hipStream_t s : S->streams) {
+    // The library's block for this stream would otherwise outlive it.
+    if (release_stream_scratch(s) != PA_OK) (void)hipGetLastError();
+    (void)hipStreamDestroy(s);
+  }
   for (auto& kv : S->jobs) delete kv.second;
   delete S;
 }

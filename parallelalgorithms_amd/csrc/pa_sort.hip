@@ -450,6 +450,32 @@ static int noalloc_scratch_ptr(char** out) {
   return PA_OK;
 }
 
+// Scratch layout of one sort call: the sort's own scratch, then staging for
+// whichever of keys / tmp / values / value tmp are not device memory.
+struct SortEntryLayout {
+  size_t o_sort, o_k, o_kt, o_v, o_vt, total;
+};
+template <typename K, bool KV>
+SortEntryLayout sort_entry_layout(size_t n, bool dk, bool dkt, bool dv, bool dvt) {
+  const bool large = n > kMaxSortN;
+  const PassVariant& var = pick_variant<K, KV>(large ? large_chunk_keys(n) : n);
+  SortEntryLayout Y;
+  Layout L;
+  Y.o_sort = L.take(large ? large_sort_scratch_bytes<K>(n) : sort_scratch_bytes<K, KV>(var, n));
+  Y.o_k = dk ? 0 : L.take(n * sizeof(K));
+  Y.o_kt = dkt ? 0 : L.take(n * sizeof(K));
+  Y.o_v = dv ? 0 : L.take(n * 4);
+  Y.o_vt = dvt ? 0 : L.take(n * 4);
+  Y.total = L.off;
+  return Y;
+}
+
+size_t sort_call_scratch_bytes(bool u64, size_t n) {
+  if (n == 0) return 0;
+  return u64 ? sort_entry_layout<uint64_t, false>(n, true, false, true, true).total
+             : sort_entry_layout<uint32_t, false>(n, true, false, true, true).total;
+}
+
 // Host entry: classifies pointers, stages host arrays, runs, copies back.
 template <typename K, bool KV>
 int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipStream_t s, uint32_t xform,
@@ -468,14 +494,10 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   const bool dkt = ktmp && is_device_ptr(ktmp);
   const bool dvt = !KV || (vtmp && is_device_ptr(vtmp));
   const PassVariant& var = pick_variant<K, KV>(large ? large_chunk_keys(n) : n);
-  Layout L;
-  const size_t o_sort = L.take(large ? large_sort_scratch_bytes<K>(n) : sort_scratch_bytes<K, KV>(var, n));
-  const size_t o_k = dk ? 0 : L.take(n * sizeof(K));
-  const size_t o_kt = dkt ? 0 : L.take(n * sizeof(K));
-  const size_t o_v = dv ? 0 : L.take(n * 4);
-  const size_t o_vt = dvt ? 0 : L.take(n * 4);
+  const SortEntryLayout Y = sort_entry_layout<K, KV>(n, dk, dkt, dv, dvt);
+  const size_t o_sort = Y.o_sort, o_k = Y.o_k, o_kt = Y.o_kt, o_v = Y.o_v, o_vt = Y.o_vt;
   void* base = nullptr;
-  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  PA_TRY(ensure_scratch(ctx, Y.total, &base));
   char* b = static_cast<char*>(base);
   K* dkeys = dk ? keys : reinterpret_cast<K*>(b + o_k);
   K* dktmp = dkt ? ktmp : reinterpret_cast<K*>(b + o_kt);

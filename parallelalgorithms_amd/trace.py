@@ -166,14 +166,10 @@ def latency_stats(latencies_ms):
             "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3)}
 
 
-def replay(jobs, budget_bytes=64 << 30, streams=4, coalesce_max_keys=1 << 20, paced=True, digests=True):
-    """Runs the jobs through one server, each submitted at its t_us (or all at
-    once when paced is False).  Inputs are generated on the device before the
-    clock starts.  Returns (result dicts in job order, stats dict)."""
-    import time
+def _prepare(jobs):
+    """Device inputs (generated from each job's seed) and output buffers."""
     import torch
-    from . import KEY_U32, KEY_U64, OP_HIST, OP_SELECT, OP_SORT, Server, fill_random_u32, fill_random_u64
-    opc = {"sort": OP_SORT, "select": OP_SELECT, "hist": OP_HIST}
+    from . import fill_random_u32, fill_random_u64
     arrs, outs = [], []
     for j in jobs:
         j.validate()
@@ -188,6 +184,68 @@ def replay(jobs, budget_bytes=64 << 30, streams=4, coalesce_max_keys=1 << 20, pa
         else:
             outs.append(None)
     torch.cuda.synchronize()
+    return arrs, outs
+
+
+def _add_digests(jobs, results, arrs, outs):
+    for i, j in enumerate(jobs):
+        r = results[i]
+        if r["status"] != "ok":
+            continue
+        if j.op == "sort":
+            r["digest"] = digest_keys(arrs[i])
+        elif j.op == "select":
+            v = int(outs[i].cpu()[0]) & (_MASK64 if j.dtype == "u64" else 0xFFFFFFFF)
+            r["digest"] = f"{v:x}"
+        else:
+            r["digest"] = ",".join(str(int(c)) for c in outs[i].cpu().tolist())
+
+
+def replay_plain(jobs, digests=True):
+    """The same jobs as plain library calls, one after another in arrival
+    order on the default stream, with no server: the per-job baseline the
+    server's throughput is compared with.  Returns (results, stats) like
+    replay; latencies are not defined here."""
+    import time
+    import torch
+    from . import hist_u32, hist_u64, select_u32, select_u64, sort_u32, sort_u64
+    arrs, outs = _prepare(jobs)
+    results = []
+    t0 = time.perf_counter()
+    for i, j in enumerate(jobs):
+        wide = j.dtype == "u64"
+        try:
+            if j.op == "sort":
+                (sort_u64 if wide else sort_u32)(arrs[i])
+            elif j.op == "select":
+                (select_u64 if wide else select_u32)(arrs[i], j.k, outs[i])
+            else:
+                (hist_u64 if wide else hist_u32)(arrs[i], j.shift, j.bits, outs[i])
+            results.append({"id": j.id, "status": "ok", "latency_ms": None})
+        except Exception as e:
+            results.append({"id": j.id, "status": f"error: {e}", "latency_ms": None})
+    torch.cuda.synchronize()
+    span = time.perf_counter() - t0
+    if digests:
+        _add_digests(jobs, results, arrs, outs)
+    keys = float(sum(j.n for j in jobs))
+    stats = {"jobs": len(jobs), "span_s": round(span, 6), "jobs_per_s": round(len(jobs) / span, 1),
+             "keys": int(keys), "gkeys_per_s": round(keys / span / 1e9, 3),
+             "failed": sum(r["status"] != "ok" for r in results)}
+    del arrs, outs
+    torch.cuda.empty_cache()
+    return results, stats
+
+
+def replay(jobs, budget_bytes=64 << 30, streams=4, coalesce_max_keys=1 << 20, paced=True, digests=True):
+    """Runs the jobs through one server, each submitted at its t_us (or all at
+    once when paced is False).  Inputs are generated on the device before the
+    clock starts.  Returns (result dicts in job order, stats dict)."""
+    import time
+    import torch
+    from . import KEY_U32, KEY_U64, OP_HIST, OP_SELECT, OP_SORT, Server
+    opc = {"sort": OP_SORT, "select": OP_SELECT, "hist": OP_HIST}
+    arrs, outs = _prepare(jobs)
     srv = Server(budget_bytes=budget_bytes, streams=streams, coalesce_max_keys=coalesce_max_keys)
     results = []
     try:
@@ -216,22 +274,14 @@ def replay(jobs, budget_bytes=64 << 30, streams=4, coalesce_max_keys=1 << 20, pa
         srv.close()
     torch.cuda.synchronize()
     if digests:
-        for i, j in enumerate(jobs):
-            r = results[i]
-            if r["status"] != "ok":
-                continue
-            if j.op == "sort":
-                r["digest"] = digest_keys(arrs[i])
-            elif j.op == "select":
-                v = int(outs[i].cpu()[0]) & (_MASK64 if j.dtype == "u64" else 0xFFFFFFFF)
-                r["digest"] = f"{v:x}"
-            else:
-                r["digest"] = ",".join(str(int(c)) for c in outs[i].cpu().tolist())
+        _add_digests(jobs, results, arrs, outs)
     keys = float(sum(j.n for j in jobs))
     stats = {"jobs": len(jobs), "span_s": round(span, 6), "jobs_per_s": round(len(jobs) / span, 1),
              "keys": int(keys), "gkeys_per_s": round(keys / span / 1e9, 3), **latency_stats(lats),
              "failed": sum(r["status"] != "ok" for r in results), "launches": st["launches"],
-             "coalesced_jobs": st["coalesced_jobs"], "peak_bytes_in_flight": st["peak_bytes_in_flight"]}
+             "coalesced_jobs": st["coalesced_jobs"], "peak_bytes_in_flight": st["peak_bytes_in_flight"],
+             "admission_waits": st["admission_waits"], "scratch_releases": st["scratch_releases"],
+             "oversize_units": st["oversize_units"]}
     del arrs, outs
     torch.cuda.empty_cache()
     return results, stats

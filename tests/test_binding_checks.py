@@ -48,3 +48,21 @@ def test_short_buffers_are_refused(call, what):
 def test_valid_arguments_pass_the_checks():
     P._check(np.zeros(8, np.uint32), 4, "keys", 8)
     P._check(None, 4, "tmp", 8)
+
+
+def test_job_scratch_charge():
+    """The server's per-job charge (pa_job_scratch_bytes) needs no device: it
+    is the library's own layout, rounded as the allocator rounds (+1/8, whole
+    MiB).  A device sort of n u32 keys needs at least its n-key ping-pong
+    buffer; select at least its n/2-key compaction buffer."""
+    M = 1 << 20
+    for n in (1, 1000, 10 * M, 100 * M, (1 << 30) + 5):
+        b = P.Server.job_scratch_bytes(P.OP_SORT, n, P.KEY_U32)
+        assert b % M == 0 and b >= 4 * n
+        assert P.Server.job_scratch_bytes(P.OP_SORT, n, P.KEY_U64) >= 8 * n
+        assert P.Server.job_scratch_bytes(P.OP_SELECT, n, P.KEY_U32) >= 4 * (n // 2)
+    sizes = [P.Server.job_scratch_bytes(P.OP_SORT, n, P.KEY_U32) for n in (M, 2 * M, 4 * M, 8 * M, 16 * M)]
+    assert sizes == sorted(sizes)
+    assert P.Server.job_scratch_bytes(P.OP_HIST, 100 * M, P.KEY_U32, 12) == M  # counts only
+    with pytest.raises(P.PAError):
+        P.call("pa_job_scratch_bytes", None, None)
