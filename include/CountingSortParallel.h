@@ -34,8 +34,8 @@ inline void counting_sort_parallel_inner(unsigned char* array_to_sort, size_t l,
   (void)threshold_count;
   (void)threshold_fill;
   if (r <= l) return;
-  int st = pa_counting_sort_u8(array_to_sort + l, r - l, nullptr);
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+  int st = pa_counting_sort_u8(array_to_sort + l, r - l, pa_thread_stream());
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   if (st != PA_OK)
     throw std::runtime_error(std::string("counting_sort_parallel: ") + pa_status_string(st) + ": " +
                              pa_last_error_string());

@@ -26,8 +26,8 @@ inline void fill(T* src, const T& value, size_t l, size_t r, const char* who) {
   static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16,
                 "parallel_fill takes elements of 1, 2, 4, 8 or 16 bytes");
   if (r <= l) return;
-  int st = pa_fill(src + l, sizeof(T), &value, r - l, nullptr);
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+  int st = pa_fill(src + l, sizeof(T), &value, r - l, pa_thread_stream());
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   if (st != PA_OK)
     throw std::runtime_error(std::string(who) + ": " + pa_status_string(st) + ": " + pa_last_error_string());
 }

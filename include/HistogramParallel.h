@@ -33,8 +33,8 @@ inline size_t* byte_histogram(const unsigned char* in, size_t l, size_t r, const
   size_t* out = new size_t[NumberOfBins]{};
   if (l >= r) return out;
   uint64_t tmp[256];
-  int st = pa_hist_u8(in + l, r - l, tmp, nullptr);
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+  int st = pa_hist_u8(in + l, r - l, tmp, pa_thread_stream());
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   if (st != PA_OK) {
     delete[] out;
     check(st, who);
@@ -50,8 +50,8 @@ inline size_t* digit_histogram(unsigned* in, size_t l, size_t r, unsigned long s
   size_t* out = new size_t[R]{};
   if (l > r) return out;
   uint64_t* tmp = new uint64_t[R];
-  int st = pa_hist_u32(in + l, r - l + 1, (unsigned)shift, log2_exact(R), tmp, nullptr);
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+  int st = pa_hist_u32(in + l, r - l + 1, (unsigned)shift, log2_exact(R), tmp, pa_thread_stream());
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   if (st != PA_OK) {
     delete[] tmp;
     delete[] out;
@@ -68,8 +68,8 @@ inline size_t** quanta_histogram(unsigned* in, size_t l, size_t r, size_t workQu
   uint64_t* flat = new uint64_t[numberOfQuantas * 256]();
   int st = PA_OK;
   if (l <= r) {
-    st = pa_hist_quanta_u32(in, l, r + 1, workQuanta, numberOfQuantas, whichByte, flat, nullptr);
-    if (st == PA_OK) st = pa_synchronize(nullptr);
+    st = pa_hist_quanta_u32(in, l, r + 1, workQuanta, numberOfQuantas, whichByte, flat, pa_thread_stream());
+    if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   }
   if (st != PA_OK) {
     delete[] flat;
@@ -121,8 +121,8 @@ inline size_t** HistogramByteComponentsParallel(unsigned inArray[], size_t l, si
   (void)parallelThreshold;
   uint64_t tmp[4 * 256] = {};
   if (l <= r) {
-    int st = pa_hist_digits_u32(inArray + l, r - l + 1, tmp, nullptr);
-    if (st == PA_OK) st = pa_synchronize(nullptr);
+    int st = pa_hist_digits_u32(inArray + l, r - l + 1, tmp, pa_thread_stream());
+    if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
     pa_hist_detail::check(st, "HistogramByteComponentsParallel");
   }
   size_t** count = new size_t*[Log2ofPowerOfTwoRadix];

@@ -8,7 +8,10 @@
 // than silently running somewhere else.  Every merge here produces the merged
 // run the reference produces; its parallel thresholds and in-place strategies
 // (block rotations, memory-pressure checks) only chose how the CPU got there.
-// A failure throws std::runtime_error; there is no CPU fallback.
+// As in the reference, merges of at most 32,768 keys (merge_parallel_L5: its
+// parallel_threshold) whose arrays are in host memory run on the calling
+// thread; larger ones and any in device memory run on the GPU, on the calling
+// thread's stream.  A failure throws std::runtime_error.
 #ifndef PA_DROPIN_PARALLELMERGE_H
 #define PA_DROPIN_PARALLELMERGE_H
 
@@ -19,6 +22,7 @@
 #include <type_traits>
 
 #include "pa_hip.h"
+#include "pa_host_small.h"
 
 namespace ParallelAlgorithms {
 
@@ -33,18 +37,26 @@ inline void check(int status, const char* what) {
 }
 
 // dst[0 .. na+nb) = merge of a[0..na) and b[0..nb); ties take a first.
+// Up to host_max keys with every array in host memory are merged on the
+// calling thread (merge_parallel_L5's sequential case, ParallelMerge.h:296-299).
 template <class T>
-inline void merge(const T* a, size_t na, const T* b, size_t nb, T* dst, const char* who) {
+inline void merge(const T* a, size_t na, const T* b, size_t nb, T* dst, const char* who,
+                  size_t host_max = pa_host_small::kMergeMax) {
   static_assert(supported<T>, "the GPU merge handles 32/64-bit unsigned keys");
   if (na + nb == 0) return;
+  if (na + nb <= host_max && pa_host_small::on_host(dst) && (na == 0 || pa_host_small::on_host(a)) &&
+      (nb == 0 || pa_host_small::on_host(b))) {
+    pa_host_small::merge(a, na, b, nb, dst);
+    return;
+  }
   int st;
   if (sizeof(T) == 4)
     st = pa_merge_u32(reinterpret_cast<const uint32_t*>(a), na, reinterpret_cast<const uint32_t*>(b), nb,
-                      reinterpret_cast<uint32_t*>(dst), nullptr);
+                      reinterpret_cast<uint32_t*>(dst), pa_thread_stream());
   else
     st = pa_merge_u64(reinterpret_cast<const uint64_t*>(a), na, reinterpret_cast<const uint64_t*>(b), nb,
-                      reinterpret_cast<uint64_t*>(dst), nullptr);
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+                      reinterpret_cast<uint64_t*>(dst), pa_thread_stream());
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   check(st, who);
 }
 
@@ -54,12 +66,16 @@ inline void merge_in_place(T* t, size_t l, size_t m, size_t r, const char* who) 
   static_assert(supported<T>, "the GPU merge handles 32/64-bit unsigned keys");
   if (r + 1 <= l) return;  // empty range (size_t wrap allowed, as in the reference)
   const size_t n1 = m + 1 - l, n2 = r - m;
+  if (n1 + n2 <= pa_host_small::kMergeMax && pa_host_small::on_host(t + l)) {
+    pa_host_small::merge_in_place(t + l, n1, n2);
+    return;
+  }
   int st;
   if (sizeof(T) == 4)
-    st = pa_merge_inplace_u32(reinterpret_cast<uint32_t*>(t) + l, n1, n2, nullptr);
+    st = pa_merge_inplace_u32(reinterpret_cast<uint32_t*>(t) + l, n1, n2, pa_thread_stream());
   else
-    st = pa_merge_inplace_u64(reinterpret_cast<uint64_t*>(t) + l, n1, n2, nullptr);
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+    st = pa_merge_inplace_u64(reinterpret_cast<uint64_t*>(t) + l, n1, n2, pa_thread_stream());
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   check(st, who);
 }
 
@@ -124,9 +140,8 @@ inline void merge_dac_hybrid(const _Type* t, size_t p1, size_t r1, size_t p2, si
 template <class _Type>
 inline void merge_parallel_L5(_Type* t, size_t p1, size_t r1, size_t p2, size_t r2, _Type* a, size_t p3,
                               size_t parallel_threshold = 32768) {
-  (void)parallel_threshold;
   pa_merge_detail::merge(t + p1, pa_merge_detail::incl_len(p1, r1), t + p2, pa_merge_detail::incl_len(p2, r2), a + p3,
-                         "merge_parallel_L5");
+                         "merge_parallel_L5", parallel_threshold);
 }
 template <class _Type>
 inline void merge_parallel_quad(_Type* t, size_t p1, size_t r1, size_t p2, size_t r2, _Type* a, size_t p3) {

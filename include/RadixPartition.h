@@ -23,13 +23,13 @@
 
 namespace pa_partition_detail {
 inline void check(int st, const char* who) {
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   if (st != PA_OK) throw std::runtime_error(std::string(who) + ": " + pa_status_string(st) + ": " + pa_last_error_string());
 }
 inline unsigned one(unsigned a[], size_t start, size_t length, size_t k) {
   if (k < start || k >= start + length) throw std::out_of_range("PartitionRadix: k must lie in [aStart, aStart + aLength)");
   uint32_t value = 0;
-  check(pa_partition_u32(a + start, length, k - start, &value, nullptr), "PartitionRadix");
+  check(pa_partition_u32(a + start, length, k - start, &value, pa_thread_stream()), "PartitionRadix");
   return value;
 }
 inline void many(unsigned a[], size_t start, size_t length, const size_t k[], size_t k_start, size_t k_length) {
@@ -39,7 +39,7 @@ inline void many(unsigned a[], size_t start, size_t length, const size_t k[], si
     if (kk < start || kk >= start + length) throw std::out_of_range("PartitionRadix: k must lie in [aStart, aStart + aLength)");
     ks[i] = kk - start;
   }
-  check(pa_partition_multi_u32(a + start, length, ks.data(), ks.size(), nullptr), "PartitionRadix");
+  check(pa_partition_multi_u32(a + start, length, ks.data(), ks.size(), pa_thread_stream()), "PartitionRadix");
 }
 }  // namespace pa_partition_detail
 

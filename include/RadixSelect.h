@@ -23,8 +23,8 @@ namespace pa_select_detail {
 inline unsigned select(unsigned a[], size_t start, size_t length, size_t k) {
   if (k < start || k >= start + length) throw std::out_of_range("SelectRadix: k must lie in [start, start + length)");
   uint32_t value = 0;
-  int st = pa_select_radix_u32(a + start, length, k - start, &value, nullptr);
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+  int st = pa_select_radix_u32(a + start, length, k - start, &value, pa_thread_stream());
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   if (st != PA_OK)
     throw std::runtime_error(std::string("SelectRadix: ") + pa_status_string(st) + ": " + pa_last_error_string());
   return value;

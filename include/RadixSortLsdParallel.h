@@ -5,8 +5,10 @@
 // Same namespace, names, template parameters and default arguments as the
 // reference.  Arrays may live in host memory (staged to the GPU and back, as
 // the reference's callers pass them) or in device memory.  Calls block until
-// the result is in place.  A failure throws std::runtime_error; there is no
-// CPU fallback.
+// the result is in place; each host thread uses its own stream
+// (pa_thread_stream).  Host arrays of fewer than 100 keys are insertion-sorted
+// on the calling thread, as the reference does.  A failure throws
+// std::runtime_error; nothing larger falls back to the CPU.
 #ifndef PA_DROPIN_RADIXSORTLSDPARALLEL_H
 #define PA_DROPIN_RADIXSORTLSDPARALLEL_H
 
@@ -16,6 +18,7 @@
 #include <string>
 
 #include "pa_hip.h"
+#include "pa_host_small.h"
 
 namespace ParallelAlgorithms {
 
@@ -25,8 +28,18 @@ inline void check(int status, const char* what) {
     throw std::runtime_error(std::string(what) + ": " + pa_status_string(status) + ": " + pa_last_error_string());
 }
 // Host-staged calls already return complete; device-pointer calls are
-// enqueued on the null stream, so wait for it to keep the blocking contract.
-inline void finish(const char* what) { check(pa_synchronize(nullptr), what); }
+// enqueued on the calling thread's stream, so wait for it to keep the
+// blocking contract.
+inline void finish(const char* what) { check(pa_synchronize(pa_thread_stream()), what); }
+// Fewer than 100 keys in host memory: insertion sort on this thread, as the
+// reference does (RadixSortLsdParallel.h:383-386).
+template <class T>
+inline bool small_on_host(T* a, size_t n) {
+  if (n == 0) return true;
+  if (n >= pa_host_small::kSortBelow || !pa_host_small::on_host(a)) return false;
+  pa_host_small::insertion_sort(a, n);
+  return true;
+}
 }  // namespace pa_detail
 
 // RadixSortLsdParallel.h:340-364.  The reference allocates its own scratch and
@@ -34,7 +47,8 @@ inline void finish(const char* what) { check(pa_synchronize(nullptr), what); }
 // own tiles, so parallelThreshold has no effect on the (identical) result.
 inline void SortRadixPar(unsigned* a, size_t a_size, size_t parallelThreshold = 64 * 1024) {
   (void)parallelThreshold;
-  pa_detail::check(pa_sort_u32(a, nullptr, a_size, nullptr), "SortRadixPar");
+  if (pa_detail::small_on_host(a, a_size)) return;
+  pa_detail::check(pa_sort_u32(a, nullptr, a_size, pa_thread_stream()), "SortRadixPar");
   pa_detail::finish("SortRadixPar");
 }
 
@@ -42,7 +56,8 @@ inline void SortRadixPar(unsigned* a, size_t a_size, size_t parallelThreshold = 
 // ping-pong buffer when it is device memory (contents unspecified afterwards).
 inline void SortRadixPar(unsigned* a, unsigned* tmp_work_buff, size_t a_size, size_t parallelThreshold = 512 * 1024) {
   (void)parallelThreshold;
-  pa_detail::check(pa_sort_u32(a, tmp_work_buff, a_size, nullptr), "SortRadixPar");
+  if (pa_detail::small_on_host(a, a_size)) return;
+  pa_detail::check(pa_sort_u32(a, tmp_work_buff, a_size, pa_thread_stream()), "SortRadixPar");
   pa_detail::finish("SortRadixPar");
 }
 
@@ -51,26 +66,28 @@ inline void SortRadixPar(unsigned* a, unsigned* tmp_work_buff, size_t a_size, si
 // key-value sorts.
 inline void SortRadixPar(uint64_t* a, uint64_t* tmp_work_buff, size_t a_size, size_t parallelThreshold = 512 * 1024) {
   (void)parallelThreshold;
-  pa_detail::check(pa_sort_u64(a, tmp_work_buff, a_size, nullptr), "SortRadixPar(u64)");
+  if (pa_detail::small_on_host(a, a_size)) return;
+  pa_detail::check(pa_sort_u64(a, tmp_work_buff, a_size, pa_thread_stream()), "SortRadixPar(u64)");
   pa_detail::finish("SortRadixPar(u64)");
 }
 inline void SortRadixPar(float* a, float* tmp_work_buff, size_t a_size, size_t parallelThreshold = 512 * 1024) {
   (void)parallelThreshold;
-  pa_detail::check(pa_sort_f32(a, tmp_work_buff, a_size, nullptr), "SortRadixPar(float)");
+  if (pa_detail::small_on_host(a, a_size)) return;
+  pa_detail::check(pa_sort_f32(a, tmp_work_buff, a_size, pa_thread_stream()), "SortRadixPar(float)");
   pa_detail::finish("SortRadixPar(float)");
 }
 inline void SortRadixParKeyValue(unsigned* keys, unsigned* values, size_t n) {
-  pa_detail::check(pa_sort_kv_u32_u32(keys, values, nullptr, nullptr, n, nullptr), "SortRadixParKeyValue");
+  pa_detail::check(pa_sort_kv_u32_u32(keys, values, nullptr, nullptr, n, pa_thread_stream()), "SortRadixParKeyValue");
   pa_detail::finish("SortRadixParKeyValue");
 }
 inline void SortRadixParKeyValue(uint64_t* keys, unsigned* values, size_t n) {
-  pa_detail::check(pa_sort_kv_u64_u32(keys, values, nullptr, nullptr, n, nullptr), "SortRadixParKeyValue");
+  pa_detail::check(pa_sort_kv_u64_u32(keys, values, nullptr, nullptr, n, pa_thread_stream()), "SortRadixParKeyValue");
   pa_detail::finish("SortRadixParKeyValue");
 }
 
 // RadixSortLsdParallel.h:86-106: sorted result in BOTH a and b (:80-82).
 inline void RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase(unsigned* a, unsigned* b, unsigned long a_size) {
-  pa_detail::check(pa_sort_two_phase_u32(a, b, a_size, nullptr), "RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase");
+  pa_detail::check(pa_sort_two_phase_u32(a, b, a_size, pa_thread_stream()), "RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase");
   pa_detail::finish("RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase");
 }
 
@@ -78,7 +95,7 @@ inline void RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase(unsigned* a, uns
 // a_size >= 100 (its body is commented out, :503); here it sorts like the
 // TwoPhase form (result in a and b) -- a documented divergence.
 inline void RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase_DeRandomize(unsigned* a, unsigned* b, size_t a_size) {
-  pa_detail::check(pa_sort_two_phase_u32(a, b, a_size, nullptr),
+  pa_detail::check(pa_sort_two_phase_u32(a, b, a_size, pa_thread_stream()),
                    "RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase_DeRandomize");
   pa_detail::finish("RadixSortLSDPowerOf2RadixParallel_unsigned_TwoPhase_DeRandomize");
 }
@@ -90,7 +107,7 @@ inline void SortRadixInnerPar(unsigned* inputArray, unsigned* workArray, size_t 
                               size_t ParallelWorkQuantum = 64 * 1024) {
   static_assert(PowerOfTwoRadix == 256 && Log2ofPowerOfTwoRadix == 8, "the GPU sort uses 8-bit digits");
   (void)ParallelWorkQuantum;
-  pa_detail::check(pa_sort_u32(inputArray, workArray, inputSize, nullptr), "SortRadixInnerPar");
+  pa_detail::check(pa_sort_u32(inputArray, workArray, inputSize, pa_thread_stream()), "SortRadixInnerPar");
   pa_detail::finish("SortRadixInnerPar");
 }
 
@@ -102,8 +119,8 @@ inline size_t** ComputeStartOfBinsPar(unsigned* inArray, size_t size, size_t wor
   static_assert(PowerOfTwoRadix == 256 && Log2ofPowerOfTwoRadix == 8, "8-bit digits");
   (void)parallelThreshold;
   uint64_t* flat = new uint64_t[numberOfQuantas * 256];
-  int st = pa_start_of_bins_u32(inArray, size, workQuanta, numberOfQuantas, digit, flat, nullptr);
-  if (st == PA_OK) st = pa_synchronize(nullptr);
+  int st = pa_start_of_bins_u32(inArray, size, workQuanta, numberOfQuantas, digit, flat, pa_thread_stream());
+  if (st == PA_OK) st = pa_synchronize(pa_thread_stream());
   if (st != PA_OK) {
     delete[] flat;
     pa_detail::check(st, "ComputeStartOfBinsPar");

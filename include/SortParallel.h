@@ -64,7 +64,7 @@ inline void sort_par(_Type* src, size_t l, size_t r, _Type* dst, size_t dst_size
   if (dst_size < r) throw std::invalid_argument("dst_size must be larger or equal to r, to be able to return dst[l to r-1]");
   if (r <= l) return;
   pa_sort_keys_detail::sort_inclusive(src, l, r - 1, dst, true, "sort_par");
-  if (!srcToDst) pa_sort_keys_detail::check(pa_copy(src + l, dst + l, (r - l) * sizeof(_Type), nullptr), "sort_par");
+  if (!srcToDst) pa_sort_keys_detail::check(pa_copy(src + l, dst + l, (r - l) * sizeof(_Type), pa_thread_stream()), "sort_par");
 }
 
 // SortParallel.h:89-93.

@@ -55,6 +55,15 @@ int pa_version(void);
 int pa_set_device(int device);
 /* Blocks until all work enqueued on `stream` has finished. */
 int pa_synchronize(pa_stream_t stream);
+/* A stream owned by the calling host thread on its current device, created
+ * on first use (ordered after null-stream work, like the null stream) and
+ * reused by later threads once this one ends.  The drop-in headers enqueue on
+ * it, so concurrent callers on different threads run concurrently, each with
+ * its own scratch.  NULL (the null stream) if a stream cannot be created.
+ * No reference counterpart (the reference runs on TBB's task pool). */
+pa_stream_t pa_thread_stream(void);
+/* 1 if p points into device memory (hipMalloc or managed), else 0. */
+int pa_is_device_pointer(const void* p);
 /* Copies `bytes` bytes between any mix of host and device memory, stream-
  * ordered (blocking when host memory is involved).  Used by the drop-in
  * headers where a reference call puts its result in a second buffer. */

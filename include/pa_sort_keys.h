@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "pa_hip.h"
+#include "pa_host_small.h"
 
 namespace pa_sort_keys_detail {
 
@@ -29,31 +30,38 @@ inline void check(int status, const char* what) {
     throw std::runtime_error(std::string(what) + ": " + pa_status_string(status) + ": " + pa_last_error_string());
 }
 
-// Sorts a[0, n) ascending in place (host or device memory) and waits.
+// Sorts a[0, n) ascending in place (host or device memory) and waits; fewer
+// than 100 keys in host memory are sorted on the calling thread.
 template <class T>
 inline void sort_keys(T* a, size_t n, const char* who) {
   if (n == 0) return;
+  if (n < pa_host_small::kSortBelow && pa_host_small::on_host(a)) {
+    // Same order as the GPU sorts; the reference's sorts also finish small
+    // ranges with an insertion sort.
+    pa_host_small::insertion_sort(a, n);
+    return;
+  }
   int st;
   if constexpr (std::is_same_v<T, float>) {
-    st = pa_sort_f32(a, nullptr, n, nullptr);
+    st = pa_sort_f32(a, nullptr, n, pa_thread_stream());
   } else if constexpr (std::is_same_v<T, double>) {
-    st = pa_sort_f64(a, nullptr, n, nullptr);
+    st = pa_sort_f64(a, nullptr, n, pa_thread_stream());
   } else if constexpr (std::is_integral_v<T> && !std::is_same_v<T, bool> && sizeof(T) == 4) {
     if constexpr (std::is_signed_v<T>)
-      st = pa_sort_i32(reinterpret_cast<int32_t*>(a), nullptr, n, nullptr);
+      st = pa_sort_i32(reinterpret_cast<int32_t*>(a), nullptr, n, pa_thread_stream());
     else
-      st = pa_sort_u32(reinterpret_cast<uint32_t*>(a), nullptr, n, nullptr);
+      st = pa_sort_u32(reinterpret_cast<uint32_t*>(a), nullptr, n, pa_thread_stream());
   } else if constexpr (std::is_integral_v<T> && !std::is_same_v<T, bool> && sizeof(T) == 8) {
     if constexpr (std::is_signed_v<T>)
-      st = pa_sort_i64(reinterpret_cast<int64_t*>(a), nullptr, n, nullptr);
+      st = pa_sort_i64(reinterpret_cast<int64_t*>(a), nullptr, n, pa_thread_stream());
     else
-      st = pa_sort_u64(reinterpret_cast<uint64_t*>(a), nullptr, n, nullptr);
+      st = pa_sort_u64(reinterpret_cast<uint64_t*>(a), nullptr, n, pa_thread_stream());
   } else {
     static_assert(unsupported<T>::value,
                   "GPU sorts take 32/64-bit integers, float or double elements");
   }
   check(st, who);
-  check(pa_synchronize(nullptr), who);
+  check(pa_synchronize(pa_thread_stream()), who);
 }
 
 // Sorts src[l, r] (inclusive bounds, the reference merge sorts' convention).
@@ -67,7 +75,7 @@ inline void sort_inclusive(T* src, size_t l, size_t r, T* dst, bool src_to_dst, 
   T* target = src;
   if (src_to_dst) {
     if (!dst) throw std::invalid_argument(std::string(who) + ": dst is null");
-    check(pa_copy(dst + l, src + l, n * sizeof(T), nullptr), who);
+    check(pa_copy(dst + l, src + l, n * sizeof(T), pa_thread_stream()), who);
     target = dst;
   }
   sort_keys(target + l, n, who);

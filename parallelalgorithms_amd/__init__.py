@@ -75,6 +75,8 @@ _SIGS = {
     "pa_server_get_stats": (_i, [_vp, _vp]),
     "pa_server_destroy": (None, [_vp]),
     "pa_job_scratch_bytes": (_i, [_vp, _vp]),
+    "pa_thread_stream": (_vp, []),
+    "pa_is_device_pointer": (_i, [_vp]),
     "pa_profile_enable": (_i, [_i]),
     "pa_profile_only": (_i, [ctypes.c_char_p]),
     "pa_profile_query": (_i, [ctypes.c_char_p, _vp, _vp]),

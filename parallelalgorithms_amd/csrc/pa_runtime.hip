@@ -31,6 +31,60 @@ int arg_error(const char* msg) {
 }
 
 // ---------------------------------------------------------------------------
+// Per-thread streams
+// ---------------------------------------------------------------------------
+// Each host thread gets its own stream per device on first use, so callers
+// on different threads neither share the null stream's serialisation nor its
+// scratch.  A thread's streams go back to a pool when the thread ends (no HIP
+// call there) and are handed to the next thread that asks, so a process
+// holds at most as many as it has threads using the library at once.
+struct StreamPool {
+  std::mutex mu;
+  std::map<int, std::vector<hipStream_t>> free_by_device;
+};
+static StreamPool& stream_pool() {
+  static StreamPool* p = new StreamPool();  // never destroyed: used from thread exits
+  return *p;
+}
+struct ThreadStreams {
+  std::vector<std::pair<int, hipStream_t>> held;
+  ~ThreadStreams() {
+    StreamPool& P = stream_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    for (auto& d : held) P.free_by_device[d.first].push_back(d.second);
+  }
+};
+static thread_local ThreadStreams t_streams;
+
+hipStream_t thread_stream() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  for (auto& d : t_streams.held)
+    if (d.first == dev) return d.second;
+  hipStream_t s = nullptr;
+  {
+    StreamPool& P = stream_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto& f = P.free_by_device[dev];
+    if (!f.empty()) {
+      s = f.back();
+      f.pop_back();
+    }
+  }
+  // Default flags: the stream is ordered after work the caller put on the
+  // null stream, as the null stream itself would be.
+  if (!s && hipStreamCreate(&s) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;  // the null stream
+  }
+  t_streams.held.emplace_back(dev, s);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
 // Stream contexts
 // ---------------------------------------------------------------------------
 static std::mutex g_ctx_mu;
@@ -389,6 +443,10 @@ int pa_set_device(int device) {
   PA_HIP(hipSetDevice(device));
   return PA_OK;
 }
+
+pa_stream_t pa_thread_stream(void) { return (pa_stream_t)thread_stream(); }
+
+int pa_is_device_pointer(const void* p) { return is_device_ptr(p) ? 1 : 0; }
 
 int pa_synchronize(pa_stream_t stream) {
   PA_HIP(hipStreamSynchronize((hipStream_t)stream));
