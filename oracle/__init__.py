@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# PA_ORACLE_LIB selects another build of the same source (the sanitizer build).
+LIB_PATH = os.environ.get("PA_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libpa_ref.so")
 
 _vp, _sz, _u, _u64, _u32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint64, ctypes.c_uint32

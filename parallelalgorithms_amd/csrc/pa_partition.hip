@@ -33,7 +33,17 @@ constexpr int kPartBlock = 256;
 constexpr int kPartItems = 16;
 constexpr uint32_t kPartTile = kPartBlock * kPartItems;  // 4096 keys
 constexpr unsigned kPartGrid = 1024;
-static_assert(kPartGrid <= 1024, "k_part_offsets gives each block one thread of its single 1024-thread workgroup");
+// k_part_offsets gives each block of the count and scatter passes one thread
+// of its single workgroup, so it serves at most this many blocks.
+constexpr unsigned kPartOffsetsThreads = 1024;
+
+// Host check before the launch of k_part_offsets: a grid it cannot serve
+// would leave blocks without offsets and the scatter writing through them.
+int part_offsets_grid_check(size_t nb) {
+  if (nb == 0 || nb > kPartOffsetsThreads)
+    return arg_error("partition: the offsets pass serves 1 to 1024 blocks");
+  return PA_OK;
+}
 
 // Block b owns the contiguous tiles [b * tpb, (b + 1) * tpb), tpb =
 // ceil(tiles / blocks), and item j of lane l in wave w of a tile is key
@@ -106,11 +116,12 @@ __global__ __launch_bounds__(kPartBlock) void k_part_count(const uint32_t* __res
 }
 
 // offsets[b][g] = first output slot of block b's keys of group g
-// (0: < v, 1: == v, 2: > v).  One block of 1024 threads, nb <= 1024.
-__global__ __launch_bounds__(1024) void k_part_offsets(const uint32_t* __restrict__ counts, uint32_t nb, size_t n,
+// (0: < v, 1: == v, 2: > v).  One block of 1024 threads, nb <= 1024
+// (part_offsets_grid_check runs on the host before every launch).
+__global__ __launch_bounds__(kPartOffsetsThreads) void k_part_offsets(const uint32_t* __restrict__ counts, uint32_t nb, size_t n,
                                                        const uint32_t* __restrict__ items_per_block,
                                                        uint32_t* __restrict__ offsets) {
-  __shared__ uint32_t s_sum[3][16];
+  __shared__ uint32_t s_sum[3][kPartOffsetsThreads / 64];
   __shared__ uint32_t s_tot[3];
   const uint32_t b = threadIdx.x, lane = b & 63, wave = b >> 6;
   uint32_t c[3] = {0, 0, 0};
@@ -128,7 +139,7 @@ __global__ __launch_bounds__(1024) void k_part_offsets(const uint32_t* __restric
   __syncthreads();
   if (b < 3) {
     uint32_t t = 0;
-    for (int w = 0; w < 16; ++w) t += s_sum[b][w];
+    for (unsigned w = 0; w < kPartOffsetsThreads / 64; ++w) t += s_sum[b][w];
     s_tot[b] = t;
   }
   __syncthreads();
@@ -216,6 +227,10 @@ using namespace pa;
 
 extern "C" {
 
+// The offsets pass's grid check on its own, for host-side tests (no device
+// needed).  Not part of the public header.
+int pa_partition_offsets_grid_check(size_t nb) { return part_offsets_grid_check(nb); }
+
 int pa_partition_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_stream_t stream) {
   if (!keys || !result) return arg_error("pa_partition_u32: null pointer");
   if (n == 0 || k >= n) return arg_error("pa_partition_u32: need 0 <= k < n");
@@ -226,6 +241,7 @@ int pa_partition_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_st
   const bool dk = is_device_ptr(keys);
   const bool dres = is_device_ptr(result);
   const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((n + kPartTile - 1) / kPartTile, kPartGrid));
+  PA_TRY(part_offsets_grid_check(nb));  // before anything is enqueued
   Layout L;
   const size_t o_sel = L.take(select_scratch_bytes_u32(n));
   const size_t o_piv = L.take(4);
@@ -253,7 +269,8 @@ int pa_partition_u32(uint32_t* keys, size_t n, size_t k, uint32_t* result, pa_st
     hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(kPartBlock), 0, s, din, n, pivot, counts, tmp);
   }
   hipLaunchKernelGGL(k_part_block_items, dim3((nb + 255) / 256), dim3(256), 0, s, nb, n, items);
-  hipLaunchKernelGGL(k_part_offsets, dim3(1), dim3(1024), 0, s, counts, nb, n, items, offsets);
+  PA_TRY(part_offsets_grid_check(nb));
+  hipLaunchKernelGGL(k_part_offsets, dim3(1), dim3(kPartOffsetsThreads), 0, s, counts, nb, n, items, offsets);
   {
     ProfScope ps("k_part_scatter", s);
     hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(kPartBlock), 0, s, tmp, n, pivot, offsets, din);

@@ -1,6 +1,8 @@
 """The ctypes binding refuses arrays the library would misread: wrong element
 size, strided views, short payload / scratch / count buffers.  These checks run
 before any library call, so they need no device."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -66,3 +68,15 @@ def test_job_scratch_charge():
     assert P.Server.job_scratch_bytes(P.OP_HIST, 100 * M, P.KEY_U32, 12) == M  # counts only
     with pytest.raises(P.PAError):
         P.call("pa_job_scratch_bytes", None, None)
+
+
+def test_partition_offsets_grid_is_checked_on_the_host():
+    """The partition's offsets pass is one 1024-thread workgroup serving one
+    block per thread: the host refuses any other grid before launching."""
+    chk = P.load().pa_partition_offsets_grid_check
+    chk.restype = ctypes.c_int
+    chk.argtypes = [ctypes.c_size_t]
+    assert chk(1) == P.PA_OK and chk(1024) == P.PA_OK
+    for nb in (0, 1025, 2048, 8192):
+        assert chk(nb) == 1  # PA_ERR_ARG
+        assert "1024" in P.load().pa_last_error_string().decode()
