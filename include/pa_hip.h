@@ -110,6 +110,49 @@ int pa_autotune_sort_u32(size_t max_n, pa_stream_t stream);
 int pa_sort_variant_u32(size_t n);
 int pa_autotune_reset(void);
 
+/* Tuning targets and candidate shapes.  No reference counterpart: the
+ * reference fixes 8-bit digits (SortRadixInnerPar<256, 8>,
+ * RadixSortLsdParallel.h:215-216,249-253; the digit loop runs
+ * sizeof(key) * 8 / Log2ofPowerOfTwoRadix times) and lists persisting tuned
+ * settings as future work (TODO.txt:43).
+ *   PA_TUNE_U32      keys-only sorts of 32-bit keys (u32, i32, f32), 4 size
+ *                    classes as above;
+ *   PA_TUNE_U64      keys-only sorts of 64-bit keys, same classes;
+ *   PA_TUNE_SEG_U32  the segmented sort (one class): the shape of its
+ *                    onesweep class and the largest segment of its persistent
+ *                    queue class.
+ * A shape is a tile (block x items), look-back width, staging rounds, lanes
+ * per ranking unit and digit width (4 to 8 bits: ceil(key bits / width)
+ * passes; an odd count ends with a copy back, which the tuner's timings
+ * include). */
+enum pa_tune_target { PA_TUNE_U32 = 0, PA_TUNE_U64 = 1, PA_TUNE_SEG_U32 = 2 };
+typedef struct pa_sort_shape {
+  int block, items, lookback, split, lanes_per_unit, digit_bits, passes;
+  uint64_t queue_max_keys; /* PA_TUNE_SEG_U32: largest segment of the queue class */
+} pa_sort_shape;
+/* Number of candidate shapes of a target (0 for an unknown target). */
+int pa_sort_shape_count(int target);
+int pa_sort_shape_info(int target, int index, pa_sort_shape* out);
+/* Index of the shape a sort of n keys will use (PA_TUNE_SEG_U32: n ignored). */
+int pa_sort_shape_choice(int target, size_t n);
+/* Sets the shape of one size class (index -1: the built-in choice). */
+int pa_sort_shape_set(int target, int size_class, int index);
+/* Times every candidate at one size per class (up to max_n keys, on device
+ * memory it allocates and frees) and keeps the fastest for this process. */
+int pa_autotune_sort(int target, size_t max_n, pa_stream_t stream);
+/* Tunes the segmented sort on a batch with these segment lengths (random
+ * keys in device memory it allocates and frees; the caller's arrays are not
+ * touched). */
+int pa_autotune_segmented_u32(const uint64_t* lens, size_t num_segments, pa_stream_t stream);
+/* Tuning caches: a JSON file holding the chosen shapes (by name) per device
+ * key, the GPU architecture and the HIP runtime version (pa_tune_device_key).
+ * Save keeps other devices' entries; load applies only the entry of this
+ * device and leaves the rest of the state alone.  With PA_TUNE_CACHE=path in
+ * the environment the first sort loads it and every tuning run saves it. */
+int pa_tune_cache_save(const char* path);
+int pa_tune_cache_load(const char* path);
+const char* pa_tune_device_key(void);
+
 /* 64-bit keys, 8 passes.  No reference counterpart on the parallel path (the
  * serial u64 LSD at RadixSortLSD.h:356-400 mis-sorts keys >= 2^32). */
 int pa_sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, pa_stream_t stream);

@@ -26,6 +26,8 @@ _SIGS = {
     "orc_sort_radix_par_alloc_u32": (None, [_vp, _sz, _sz]),
     "orc_sort_two_phase_u32": (None, [_vp, _vp, _sz]),
     "orc_sort_radix_par_u64": (None, [_vp, _vp, _sz, _sz]),
+    "orc_sort_radix_bits_u32": (None, [_vp, _vp, _sz, _u, _u, _sz]),
+    "orc_sort_radix_bits_u64": (None, [_vp, _vp, _sz, _u, _u, _sz]),
     "orc_sort_kv_u64_u32": (None, [_vp, _vp, _vp, _vp, _sz, _sz]),
     "orc_sort_kv_u32_u32": (None, [_vp, _vp, _vp, _vp, _sz, _sz]),
     "orc_f32_to_ordered": (_u32, [_u32]),
@@ -119,6 +121,16 @@ def sort_u64(a, thr=512 * 1024):
     a = np.array(a, dtype=np.uint64, copy=True)
     tmp = np.empty_like(a)
     load().orc_sort_radix_par_u64(_p(a), _p(tmp), a.size, thr)
+    return a
+
+
+def sort_radix_bits(a, bits, max_passes=64, thr=512 * 1024):
+    """SortRadixInnerPar with `bits`-bit digits (4..8) on a u32 or u64 array;
+    max_passes stops after that many passes (the state the tests inspect)."""
+    a = np.ascontiguousarray(a).copy()
+    tmp = np.empty_like(a)
+    fn = load().orc_sort_radix_bits_u64 if a.dtype == np.uint64 else load().orc_sort_radix_bits_u32
+    fn(_p(a), _p(tmp), a.size, bits, max_passes, thr)
     return a
 
 

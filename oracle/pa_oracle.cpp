@@ -390,6 +390,70 @@ void sort_inner_par(K* in, K* work, size_t n, size_t quantum) {
   }
 }
 
+// SortRadixInnerPar<PowerOfTwoRadix, Log2ofPowerOfTwoRadix> at any radix of
+// 2^4 .. 2^8 (RadixSortLsdParallel.h:215-336): the digit loop runs while the
+// shifted mask has bits left (:250-253, :309-311), i.e. ceil(key bits / bits)
+// passes, each a per-quantum count, a (bin x quantum) start-of-bin prefix and
+// the buffered stable permutation (PermuteDerandomizedNew, :169-212).  Two
+// departures, both where the reference only works for 8-bit digits: its
+// ComputeStartOfBinsPar always counts bytes (HistogramByteComponentsQCPar's
+// whichByte, HistogramParallel.h:337-338), so the counts here are of the
+// pass's own digit; and its copy of an odd pass count's result back into the
+// input is commented out (:327-329), so the result here always ends in `in`.
+// max_passes < passes stops early (no copy back): the state after that many
+// passes, for tests of the digit order.
+template <class K>
+void sort_inner_par_bits(K* in, K* work, size_t n, size_t quantum, unsigned bits, unsigned max_passes) {
+  if (n == 0) return;
+  const unsigned R = 1u << bits;
+  const K mask = (K)(R - 1);
+  const unsigned passes = (8 * sizeof(K) + bits - 1) / bits;
+  const size_t nq = (n % quantum) == 0 ? n / quantum : n / quantum + 1;
+  std::vector<uint64_t> count(nq * R), sob(nq * R);
+  K* const orig = in;
+  unsigned done = 0;
+  for (unsigned p = 0; p < passes && p < max_passes; ++p, ++done) {
+    const unsigned shift = bits * p;
+    pool().parallel_for(nq, [&](size_t q) {
+      uint64_t* c = &count[q * R];
+      std::fill(c, c + R, 0);
+      const size_t s0 = q * quantum, e0 = std::min(n, s0 + quantum);
+      for (size_t i = s0; i < e0; ++i) c[(in[i] >> shift) & mask]++;
+    });
+    uint64_t run = 0;
+    for (unsigned b = 0; b < R; ++b)
+      for (size_t q = 0; q < nq; ++q) {
+        sob[q * R + b] = run;
+        run += count[q * R + b];
+      }
+    pool().parallel_for(nq, [&](size_t q) {
+      thread_local std::vector<K> buf;
+      thread_local std::vector<size_t> idx;
+      buf.resize((size_t)R * kBufferDepth);
+      idx.resize(R);
+      uint64_t* start = &sob[q * R];
+      for (unsigned b = 0; b < R; ++b) idx[b] = b * kBufferDepth;
+      const size_t s0 = q * quantum, e0 = std::min(n, s0 + quantum);
+      for (size_t i = s0; i < e0; ++i) {
+        const K v = in[i];
+        const unsigned d = (unsigned)((v >> shift) & mask);
+        if (idx[d] == (d + 1) * kBufferDepth) {
+          std::memcpy(work + start[d], buf.data() + d * kBufferDepth, kBufferDepth * sizeof(K));
+          start[d] += kBufferDepth;
+          idx[d] = d * kBufferDepth;
+        }
+        buf[idx[d]++] = v;
+      }
+      for (unsigned b = 0; b < R; ++b) {
+        const size_t items = idx[b] - b * kBufferDepth;
+        std::memcpy(work + start[b], buf.data() + b * kBufferDepth, items * sizeof(K));
+      }
+    });
+    std::swap(in, work);
+  }
+  if (in != orig) std::memcpy(orig, in, n * sizeof(K));
+}
+
 // Key-value generalisation (new).  Stable: ties keep input order.
 template <class K, class V>
 void sort_inner_par_kv(K* kin, V* vin, K* kwork, V* vwork, size_t n, size_t quantum) {
@@ -681,6 +745,17 @@ void orc_sort_two_phase_u32(uint32_t* a, uint32_t* b, size_t n) {
   }
   // 4 passes: the result is in `a`; copy it to `b` as well (:80-82).
   std::memcpy(b, a, n * sizeof(uint32_t));
+}
+
+// SortRadixInnerPar at digit widths of 4 to 8 bits (the GPU sort's tuned
+// widths); result in a whatever the pass count (see sort_inner_par_bits).
+void orc_sort_radix_bits_u32(uint32_t* a, uint32_t* tmp, size_t n, unsigned bits, unsigned max_passes, size_t thr) {
+  if (bits < 4 || bits > 8) return;
+  sort_inner_par_bits<uint32_t>(a, tmp, n, quantum_3arg(n, thr), bits, max_passes);
+}
+void orc_sort_radix_bits_u64(uint64_t* a, uint64_t* tmp, size_t n, unsigned bits, unsigned max_passes, size_t thr) {
+  if (bits < 4 || bits > 8) return;
+  sort_inner_par_bits<uint64_t>(a, tmp, n, quantum_3arg(n, thr), bits, max_passes);
 }
 
 // New: 64-bit keys through the same quantum structure (8 passes).

@@ -37,6 +37,15 @@ _SIGS = {
     "pa_autotune_sort_u32": (_i, [_sz, _vp]),
     "pa_sort_variant_u32": (_i, [_sz]),
     "pa_autotune_reset": (_i, []),
+    "pa_sort_shape_count": (_i, [_i]),
+    "pa_sort_shape_info": (_i, [_i, _i, _vp]),
+    "pa_sort_shape_choice": (_i, [_i, _sz]),
+    "pa_sort_shape_set": (_i, [_i, _i, _i]),
+    "pa_autotune_sort": (_i, [_i, _sz, _vp]),
+    "pa_autotune_segmented_u32": (_i, [_vp, _sz, _vp]),
+    "pa_tune_cache_save": (_i, [ctypes.c_char_p]),
+    "pa_tune_cache_load": (_i, [ctypes.c_char_p]),
+    "pa_tune_device_key": (ctypes.c_char_p, []),
     "pa_sort_u64": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_f32": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_f64": (_i, [_vp, _vp, _sz, _vp]),
@@ -212,6 +221,56 @@ def autotune_sort_u32(max_n, stream=None):
 
 def sort_variant_u32(n):
     return load().pa_sort_variant_u32(n)
+
+
+TUNE_U32, TUNE_U64, TUNE_SEG_U32 = 0, 1, 2
+
+
+class SortShape(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int), ("items", ctypes.c_int), ("lookback", ctypes.c_int), ("split", ctypes.c_int),
+                ("lanes_per_unit", ctypes.c_int), ("digit_bits", ctypes.c_int), ("passes", ctypes.c_int),
+                ("queue_max_keys", ctypes.c_uint64)]
+
+
+def sort_shapes(target):
+    """Candidate shapes of a tuning target (pa_sort_shape_info), as dicts."""
+    lib = load()
+    out = []
+    for i in range(lib.pa_sort_shape_count(target)):
+        sh = SortShape()
+        call("pa_sort_shape_info", target, i, ctypes.addressof(sh))
+        out.append({name: getattr(sh, name) for name, _ in SortShape._fields_})
+    return out
+
+
+def sort_shape_choice(target, n=0):
+    return load().pa_sort_shape_choice(target, n)
+
+
+def sort_shape_set(target, size_class, index):
+    call("pa_sort_shape_set", target, size_class, index)
+
+
+def autotune_sort(target, max_n, stream=None):
+    call("pa_autotune_sort", target, max_n, _stream(stream))
+
+
+def autotune_segmented_u32(lens, stream=None):
+    import numpy as np
+    lens = np.ascontiguousarray(lens, dtype=np.uint64)
+    call("pa_autotune_segmented_u32", lens.ctypes.data, lens.size, _stream(stream))
+
+
+def tune_cache_save(path):
+    call("pa_tune_cache_save", os.fsencode(path))
+
+
+def tune_cache_load(path):
+    call("pa_tune_cache_load", os.fsencode(path))
+
+
+def tune_device_key():
+    return load().pa_tune_device_key().decode()
 
 
 def autotune_reset():

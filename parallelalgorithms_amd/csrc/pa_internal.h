@@ -143,6 +143,29 @@ int launch_hist_digits_u32(const uint32_t* keys, size_t n, uint32_t* counts, hip
 // one workgroup's LDS, one launch (pa_segsort.hip).
 int small_sort_enqueue_u32(uint32_t* keys, size_t n, hipStream_t s);
 size_t small_sort_max_keys();
+
+// Tuning state of one target (PA_TUNE_U32, PA_TUNE_U64, PA_TUNE_SEG_U32):
+// its candidate shapes by name and the chosen index per size class.
+struct TuneTarget {
+  const char* key;  // name in tuning caches
+  int classes;
+  int (*count)();
+  std::string (*name)(int index);
+  int (*get)(int size_class);              // -1: built-in choice
+  void (*set)(int size_class, int index);  // -1: back to the built-in choice
+};
+const TuneTarget* sort_tune_target(int target);  // PA_TUNE_U32 / PA_TUNE_U64
+const TuneTarget* seg_tune_target();
+int sort_shape_info(int target, int index, pa_sort_shape* out);
+int seg_shape_info(int index, pa_sort_shape* out);
+int sort_shape_choice(int target, size_t n);
+int seg_shape_choice();
+int autotune_sort(int target, size_t max_n, hipStream_t s);
+void seg_tune_reset();
+// Loads the cache named by PA_TUNE_CACHE once per process (no-op without it).
+void tune_cache_autoload();
+// Writes the cache named by PA_TUNE_CACHE, if set, after a tuning run.
+void tune_cache_autosave();
 // Merge of two sorted device runs into a device output (pa_merge.hip), with
 // merge_split_bytes(na + nb) bytes of caller scratch for the split table.
 size_t merge_split_bytes(size_t n);
@@ -189,6 +212,11 @@ __device__ __forceinline__ K key_inv(K v, uint32_t xform) {
 template <typename K>
 __device__ __forceinline__ uint32_t digit8(K k, uint32_t shift) {
   return (uint32_t)(k >> shift) & 0xFFu;
+}
+// RB-bit digit at bit `shift` (the last digit of a key may hold fewer bits).
+template <typename K, int RB>
+__device__ __forceinline__ uint32_t digit_bits(K k, uint32_t shift) {
+  return (uint32_t)(k >> shift) & ((1u << RB) - 1u);
 }
 
 // Inclusive prefix sum across the 64 lanes of a wave.

@@ -29,6 +29,10 @@
 // So the whole batch costs at most 10 launches however many segments it
 // holds, where per-array calls cost 6 launches per array.
 #include <algorithm>
+#include <cstdio>
+#include <string>
+#include <atomic>
+#include <cstdlib>
 #include <vector>
 
 #include "pa_sort_kernels.h"
@@ -43,13 +47,32 @@ constexpr uint32_t kLdsLimit[kLdsClasses] = {4096, 8192, 16384};
 constexpr uint32_t kLdsMax = kLdsLimit[kLdsClasses - 1];
 constexpr uint64_t kQueueDefaultMax = 65536;
 
-// Segmented onesweep tile shape (keys only): the plain sort's default pass,
-// 512 x 48 keys staged in two rounds, ranked in 16-lane units.
+// Block of the segmented histogram read (one tile per block).
 constexpr int kSegBlock = 512;
-constexpr int kSegItems = 48;
-constexpr int kSegLbw = 8;
-constexpr int kSegSplit = 2;
-constexpr uint32_t kSegTile = kSegBlock * kSegItems;
+
+// Shapes of the segmented sort (the PA_TUNE_SEG_U32 tuning target): the
+// onesweep class's tile (block x items, staged in `split` rounds, ranked in
+// 16-lane units) and digit width, and the longest segment of the queue
+// class.  Widths of 8, 6 and 4 bits give 4, 6 and 8 passes: an even count
+// leaves every large segment's result in place without a copy back.
+struct SegPassArgs {
+  const uint32_t* counts;
+  uint32_t* lb;
+  uint32_t* ctr;
+  uint32_t ntiles, shift, gstride;
+  const uint64_t *tile_info, *tin, *tout;
+  const uint32_t *len, *row;
+};
+struct SegVariant {
+  int block, items, lbw, split, rb;
+  uint64_t queue_max;
+  int (*pass)(hipStream_t s, const SegPassArgs& a);
+  int (*hist)(hipStream_t s, const uint64_t* tile_info, const uint64_t* seg_in, const uint32_t* seg_len,
+              uint32_t ntiles, uint32_t tile, uint32_t* counts, uint4* zero_ptr, size_t zero_vec);
+  uint32_t tile() const { return (uint32_t)(block * items); }
+  int passes() const { return (32 + rb - 1) / rb; }
+  uint32_t bins() const { return 1u << rb; }
+};
 // Claim order: the large segments are cut into consecutive groups of at least
 // kSegGroupTiles tiles.  Groups are claimed in order; inside a group, tile k
 // of a segment with T tiles goes to bucket floor((2k + 1) * kSegBuckets / (2T)),
@@ -170,13 +193,11 @@ constexpr int kQueueBlock = 512;
 constexpr uint32_t kQueueChunk = kQueueBlock * kSmallItems;
 constexpr size_t kQueueGrid = 512;  // two workgroups per CU on the 256 CUs of an MI355X
 
-// Longest segment served by the queue.  PA_SEG_QUEUE_MAX overrides it (read
-// per plan, so sweeps can change it between calls); at most kLdsMax disables
-// the class.
-static uint64_t seg_queue_max() {
-  const char* e = getenv("PA_SEG_QUEUE_MAX");
-  return e ? strtoull(e, nullptr, 10) : (uint64_t)kQueueDefaultMax;
-}
+// Longest segment served by the queue: the shape's, unless PA_SEG_QUEUE_MAX
+// overrides it (read per plan, so sweeps can change it between calls); at
+// most kLdsMax disables the class.
+struct SegVariant;
+static uint64_t seg_queue_max(const SegVariant& v);
 
 __global__ __launch_bounds__(kQueueBlock) void k_seg_queue(const uint64_t* __restrict__ seg_ptr,
                                                            const uint64_t* __restrict__ seg_tmp,
@@ -281,23 +302,28 @@ __global__ __launch_bounds__(BLOCK) void k_sort_small(uint32_t* __restrict__ key
   lds_sort_segment<BLOCK>(keys, n);
 }
 
+// Clears the segments' digit counts and the 16 tile / queue counters (one
+// per digit pass, up to 8, then the queue's).
 __global__ __launch_bounds__(256) void k_seg_prepare(uint32_t* __restrict__ counts, size_t ncounts,
                                                      uint32_t* __restrict__ counters) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncounts; i += stride) counts[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 16) counters[threadIdx.x] = 0;
 }
 
-// One tile per block: 4 digit histograms of the tile's keys into its
-// segment's counts; also clears the look-back table.
+// One tile of `tile` keys per block: the D digit histograms (RB-bit digits)
+// of the tile's keys into its segment's counts [D][R]; also clears the
+// look-back table.
+template <int RB>
 __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restrict__ tile_info,
                                                         const uint64_t* __restrict__ seg_in,
-                                                        const uint32_t* __restrict__ seg_len,
+                                                        const uint32_t* __restrict__ seg_len, uint32_t tile,
                                                         uint32_t* __restrict__ counts, uint4* __restrict__ zero_ptr,
                                                         size_t zero_vec) {
-  __shared__ uint32_t s_hist[4 * 256];
+  constexpr uint32_t R = 1u << RB, D = (32 + RB - 1) / RB;
+  __shared__ uint32_t s_hist[D * R];
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < 4 * 256; i += kSegBlock) s_hist[i] = 0;
+  for (uint32_t i = tid; i < D * R; i += kSegBlock) s_hist[i] = 0;
   const size_t stride = (size_t)gridDim.x * kSegBlock;
   for (size_t i = (size_t)blockIdx.x * kSegBlock + tid; i < zero_vec; i += stride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
@@ -305,7 +331,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restri
   const uint32_t seg = (uint32_t)(info >> 32), local = (uint32_t)info;
   const uint32_t* k = reinterpret_cast<const uint32_t*>(seg_in[seg]);
   const uint32_t n = seg_len[seg];
-  const uint32_t lo = local * kSegTile, hi = min(lo + kSegTile, n);
+  const uint32_t lo = local * tile, hi = min(lo + tile, n);
   // Eight loads in flight per thread before any counting (segments start at
   // any offset, so the loads are 4-byte, coalesced across the wave).
   constexpr int U = 8;
@@ -320,15 +346,68 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restri
     for (int u = 0; u < U; ++u) {
       if (i0 + u * kSegBlock >= hi) break;  // the indices grow with u
 #pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        const uint32_t dg[1] = {(x[u] >> (8 * d)) & 0xFFu};
-        dev::lds_count<1>(s_hist + d * 256, dg);
+      for (uint32_t d = 0; d < D; ++d) {
+        const uint32_t dg[1] = {dev::digit_bits<uint32_t, RB>(x[u], RB * d)};
+        dev::lds_count<1>(s_hist + d * R, dg);
       }
     }
   }
   __syncthreads();
-  for (uint32_t i = tid; i < 4 * 256; i += kSegBlock)
-    if (s_hist[i]) atomicAdd(&counts[(size_t)seg * 1024 + i], s_hist[i]);
+  for (uint32_t i = tid; i < D * R; i += kSegBlock)
+    if (s_hist[i]) atomicAdd(&counts[(size_t)seg * D * R + i], s_hist[i]);
+}
+
+template <int RB>
+int launch_seg_hist(hipStream_t s, const uint64_t* tile_info, const uint64_t* seg_in, const uint32_t* seg_len,
+                    uint32_t ntiles, uint32_t tile, uint32_t* counts, uint4* zero_ptr, size_t zero_vec) {
+  ProfScope ps("k_seg_hist", s);
+  hipLaunchKernelGGL(k_seg_hist<RB>, dim3(ntiles), dim3(kSegBlock), 0, s, tile_info, seg_in, seg_len, tile, counts,
+                     zero_ptr, zero_vec);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+template <int B, int I, int LBW, int SPLIT, int RB>
+int launch_seg_pass(hipStream_t s, const SegPassArgs& a) {
+  ProfScope ps("k_onesweep_seg", s);
+  hipLaunchKernelGGL((k_onesweep<uint32_t, false, B, I, LBW, kModeSeg, true, 4, true, SPLIT, 16, RB>), dim3(a.ntiles),
+                     dim3(B), 0, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)nullptr,
+                     (uint32_t*)nullptr, 0u, a.counts, a.lb, a.ctr, a.ntiles, a.shift, 0u, a.tile_info, a.tin, a.tout,
+                     a.len, a.gstride, a.row);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+#define PA_SEG_VARIANT(B, I, W, S, RB, Q) \
+  { B, I, W, S, RB, Q, &launch_seg_pass<B, I, W, S, RB>, &launch_seg_hist<RB> }
+// Entry 0 is the built-in shape: the plain sort's large-array pass shape of
+// 512 x 48 keys in two staging rounds, 8-bit digits, queue up to 65,536 keys.
+static const SegVariant kSegVariants[] = {
+    PA_SEG_VARIANT(512, 48, 8, 2, 8, 65536), PA_SEG_VARIANT(512, 48, 8, 2, 8, 32768),
+    PA_SEG_VARIANT(512, 48, 8, 2, 8, 16384), PA_SEG_VARIANT(512, 16, 4, 1, 8, 65536),
+    PA_SEG_VARIANT(768, 32, 8, 2, 8, 65536), PA_SEG_VARIANT(512, 48, 8, 2, 6, 65536),
+    PA_SEG_VARIANT(512, 48, 8, 2, 4, 65536), PA_SEG_VARIANT(512, 16, 4, 1, 6, 65536)};
+constexpr int kSegVariantCount = sizeof(kSegVariants) / sizeof(SegVariant);
+
+static std::atomic<int> g_seg_tuned{-1};
+
+int seg_shape_choice() {
+  static const int forced = [] {
+    const char* e = getenv("PA_SEG_VARIANT");
+    const int v = e ? atoi(e) : -1;
+    return (v >= 0 && v < kSegVariantCount) ? v : -1;
+  }();
+  if (forced >= 0) return forced;
+  tune_cache_autoload();
+  const int t = g_seg_tuned.load(std::memory_order_relaxed);
+  return t >= 0 ? t : 0;
+}
+
+static const SegVariant& seg_variant() { return kSegVariants[seg_shape_choice()]; }
+
+static uint64_t seg_queue_max(const SegVariant& v) {
+  const char* e = getenv("PA_SEG_QUEUE_MAX");
+  return e ? strtoull(e, nullptr, 10) : v.queue_max;
 }
 
 // Sorts the segments described by (address, length) lists; large segments get
@@ -336,7 +415,10 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_hist(const uint64_t* __restri
 struct SegPlan {
   std::vector<uint64_t> small_ptr[kLdsClasses], queue_ptr, queue_tmp, large_in, large_tmp, tile_info;
   std::vector<uint32_t> small_len[kLdsClasses], queue_len, large_len, large_tiles, large_row;
-  const uint64_t queue_max = seg_queue_max();
+  // The shape in force when the plan is made (a captured plan keeps it).
+  const SegVariant* var = &seg_variant();
+  const uint32_t tile = var->tile();
+  const uint64_t queue_max = seg_queue_max(*var);
   void add(uint64_t ptr, uint64_t len, uint64_t tmp_ptr) {
     if (len <= 1) return;
     for (int c = 0; c < kLdsClasses; ++c) {
@@ -355,7 +437,7 @@ struct SegPlan {
     large_in.push_back(ptr);
     large_tmp.push_back(tmp_ptr);
     large_len.push_back((uint32_t)len);
-    large_tiles.push_back((uint32_t)((len + kSegTile - 1) / kSegTile));
+    large_tiles.push_back((uint32_t)((len + tile - 1) / tile));
   }
   // Look-back rows (contiguous per segment) and the interleaved claim order:
   // a counting sort of the tiles by bucket, stable in (segment, tile).
@@ -384,7 +466,7 @@ struct SegPlan {
       large_in.push_back(qp[i]);
       large_tmp.push_back(qt[i]);
       large_len.push_back(ql[i]);
-      large_tiles.push_back((uint32_t)((ql[i] + kSegTile - 1) / kSegTile));
+      large_tiles.push_back((uint32_t)((ql[i] + tile - 1) / tile));
     }
     queue_ptr.assign(qp.begin() + first, qp.end());
     queue_tmp.assign(qt.begin() + first, qt.end());
@@ -451,9 +533,10 @@ static PlanLayout plan_layout(const SegPlan& P, size_t extra) {
   Y.table_bytes = T.off;
   Layout L;
   Y.o_tab = L.take(Y.table_bytes);
-  Y.o_cnt = L.take(nl * 1024 * 4);
-  Y.o_ctr = L.take(32);
-  Y.o_lb = L.take((size_t)4 * nt * 256 * 4);
+  const size_t DR = (size_t)P.var->passes() * P.var->bins();  // counts per segment
+  Y.o_cnt = L.take(nl * DR * 4);
+  Y.o_ctr = L.take(64);
+  Y.o_lb = L.take(DR * nt * 4);
   Y.o_work = L.take(extra);
   Y.total = L.off;
   return Y;
@@ -501,7 +584,9 @@ static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStre
   uint32_t* counts = (uint32_t*)(dev + Y.o_cnt);
   uint32_t* ctr = (uint32_t*)(dev + Y.o_ctr);
   uint32_t* lbp = (uint32_t*)(dev + Y.o_lb);
-  const size_t lb_bytes = (size_t)4 * nt * 256 * 4;
+  const SegVariant& V = *P.var;
+  const uint32_t D = (uint32_t)V.passes(), R = V.bins();
+  const size_t lb_bytes = (size_t)D * R * nt * 4;
   for (int c = 0; c < kLdsClasses; ++c) {
     const size_t ns = P.small_ptr[c].size();
     if (!ns) continue;
@@ -516,29 +601,31 @@ static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStre
   }
   const size_t nq = P.queue_len.size();
   if (nl || nq)
-    hipLaunchKernelGGL(k_seg_prepare, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((nl * 1024 + 255) / 256, 4096))),
-                       dim3(256), 0, s, counts, nl * 1024, ctr);
+    hipLaunchKernelGGL(k_seg_prepare,
+                       dim3((unsigned)std::max<size_t>(1, std::min<size_t>((nl * D * R + 255) / 256, 4096))), dim3(256),
+                       0, s, counts, nl * D * R, ctr);
   if (nq) {
     ProfScope ps("k_seg_queue", s);
     hipLaunchKernelGGL(k_seg_queue, dim3((unsigned)std::min<size_t>(nq, kQueueGrid)), dim3(kQueueBlock), 0, s,
                        (const uint64_t*)(tab + Y.t_qp), (const uint64_t*)(tab + Y.t_qt), (const uint32_t*)(tab + Y.t_ql),
-                       (uint32_t)nq, ctr + 4);
+                       (uint32_t)nq, ctr + 8);
   }
   if (nl) {
-    {
-      ProfScope ps("k_seg_hist", s);
-      hipLaunchKernelGGL(k_seg_hist, dim3((unsigned)nt), dim3(kSegBlock), 0, s, d_ti, d_li, d_ll, counts, (uint4*)lbp,
-                         lb_bytes / 16);
-    }
-    for (int p = 0; p < 4; ++p) {
-      const uint64_t* tin = (p & 1) ? d_lt : d_li;
-      const uint64_t* tout = (p & 1) ? d_li : d_lt;
-      ProfScope ps("k_onesweep_seg", s);
-      hipLaunchKernelGGL((k_onesweep<uint32_t, false, kSegBlock, kSegItems, kSegLbw, kModeSeg, true, 4, true, kSegSplit, 16>),
-                         dim3((unsigned)nt), dim3(kSegBlock), 0, s, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                         (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u, counts + p * 256,
-                         lbp + (size_t)p * nt * 256, ctr + p, (uint32_t)nt, (uint32_t)(8 * p), 0u, d_ti, tin, tout, d_ll,
-                         4u, d_lr);
+    PA_TRY(V.hist(s, d_ti, d_li, d_ll, (uint32_t)nt, V.tile(), counts, (uint4*)lbp, lb_bytes / 16));
+    for (uint32_t p = 0; p < D; ++p) {
+      SegPassArgs a;
+      a.counts = counts + p * R;
+      a.lb = lbp + (size_t)p * nt * R;
+      a.ctr = ctr + p;
+      a.ntiles = (uint32_t)nt;
+      a.shift = (uint32_t)V.rb * p;
+      a.gstride = D;
+      a.tile_info = d_ti;
+      a.tin = (p & 1) ? d_lt : d_li;
+      a.tout = (p & 1) ? d_li : d_lt;
+      a.len = d_ll;
+      a.row = d_lr;
+      PA_TRY(V.pass(s, a));
     }
   }
   PA_HIP(hipGetLastError());
@@ -601,6 +688,44 @@ int small_sort_enqueue_u32(uint32_t* keys, size_t n, hipStream_t s) {
 }
 size_t small_sort_max_keys() { return kLdsMax; }
 
+static std::string seg_variant_name(int i) {
+  const SegVariant& v = kSegVariants[i];
+  char b[96];
+  std::snprintf(b, sizeof(b), "%dx%d/lb%d/sp%d/ul16/rb%d/q%llu", v.block, v.items, v.lbw, v.split, v.rb,
+                (unsigned long long)v.queue_max);
+  return b;
+}
+
+const TuneTarget* seg_tune_target() {
+  static const TuneTarget t = [] {
+    TuneTarget x;
+    x.key = "seg_u32";
+    x.classes = 1;
+    x.count = [] { return kSegVariantCount; };
+    x.name = seg_variant_name;
+    x.get = [](int) { return g_seg_tuned.load(); };
+    x.set = [](int, int i) { g_seg_tuned.store(i); };
+    return x;
+  }();
+  return &t;
+}
+
+void seg_tune_reset() { g_seg_tuned.store(-1); }
+
+int seg_shape_info(int index, pa_sort_shape* out) {
+  if (index < 0 || index >= kSegVariantCount) return arg_error("pa_sort_shape_info: index");
+  const SegVariant& v = kSegVariants[index];
+  out->block = v.block;
+  out->items = v.items;
+  out->lookback = v.lbw;
+  out->split = v.split;
+  out->lanes_per_unit = 16;
+  out->digit_bits = v.rb;
+  out->passes = v.passes();
+  out->queue_max_keys = v.queue_max;
+  return PA_OK;
+}
+
 }  // namespace pa
 
 using namespace pa;
@@ -662,6 +787,70 @@ int pa_segmented_sort_ptrs_u32(uint32_t* const* keys, const uint64_t* lens, size
   StreamCtx& ctx = stream_ctx(s);
   std::lock_guard<std::mutex> lk(ctx.mu);
   return run_plan_alloc(P, large_total * 4, true, s, ctx);
+}
+
+int pa_autotune_segmented_u32(const uint64_t* lens, size_t num_segments, pa_stream_t stream) {
+  if (!lens || num_segments == 0) return arg_error("pa_autotune_segmented_u32: empty length list");
+  uint64_t total = 0;
+  for (size_t i = 0; i < num_segments; ++i) {
+    if (lens[i] >= (1ull << 30)) {
+      set_last_error("pa_autotune_segmented_u32: a segment holds 2^30 or more keys");
+      return PA_ERR_TOO_LARGE;
+    }
+    total += lens[i];
+  }
+  if (total == 0) return arg_error("pa_autotune_segmented_u32: no keys");
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t* keys = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = PA_OK;
+  std::vector<uint32_t*> ptrs(num_segments);
+  do {
+    hipError_t e;
+    if ((e = hipMalloc(&keys, total * 4)) != hipSuccess) {
+      st = hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+      break;
+    }
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+      st = hip_fail(e, "hipEventCreate", __FILE__, __LINE__);
+      break;
+    }
+    uint64_t off = 0;
+    for (size_t i = 0; i < num_segments; ++i) {
+      ptrs[i] = keys + off;
+      off += lens[i];
+    }
+    float best_ms = 0.f;
+    int best = -1;
+    const int before = g_seg_tuned.load();
+    for (int v = 0; v < kSegVariantCount && st == PA_OK; ++v) {
+      g_seg_tuned.store(v);
+      float sum = 0.f;
+      for (int r = 0; r < 3 && st == PA_OK; ++r) {  // one warm-up, two timed
+        if ((st = pa_fill_random_u32(keys, total, 0x5e9 + r, stream)) != PA_OK) break;
+        (void)hipEventRecord(e0, s);
+        if ((st = pa_segmented_sort_ptrs_u32(ptrs.data(), lens, num_segments, stream)) != PA_OK) break;
+        (void)hipEventRecord(e1, s);
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) {
+          st = hip_fail(e, "hipEventSynchronize", __FILE__, __LINE__);
+          break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) sum += ms;
+      }
+      if (st == PA_OK && (best < 0 || sum < best_ms)) {
+        best = v;
+        best_ms = sum;
+      }
+    }
+    g_seg_tuned.store(st == PA_OK ? best : before);
+  } while (false);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(keys);
+  if (st == PA_OK) tune_cache_autosave();
+  return st;
 }
 
 struct pa_segplan {

@@ -28,7 +28,9 @@
 #include <atomic>
 #include <mutex>
 #include <type_traits>
+#include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -44,31 +46,34 @@ __global__ __launch_bounds__(256) void k_sort_prepare(uint32_t* __restrict__ cou
   for (uint32_t i = threadIdx.x; i < ncounters; i += blockDim.x) counters[i] = 0;
 }
 
-// Pass-0 prefixes.  rows[t][d] holds digit d's count in pass-0 tile t; the
-// two kernels below turn every row into the inclusive status word of its tile
-// (count of digit d in tiles 0..t), as if every tile had already resolved its
-// look-back.  Group g of blocks owns tiles [g * per, (g + 1) * per).  Blocks
-// have 1024 threads: thread (q, d) = (tid >> 8, tid & 255) handles digit d for
-// a quarter of the group's tiles, so each thread keeps several independent
-// loads in flight instead of walking the rows one after another.
+// Pass-0 prefixes.  rows[t][d] holds digit d's count in pass-0 tile t (R =
+// 2^RB digits); the two kernels below turn every row into the inclusive
+// status word of its tile (count of digit d in tiles 0..t), as if every tile
+// had already resolved its look-back.  Group g of blocks owns tiles
+// [g * per, (g + 1) * per).  Blocks have 1024 threads: thread (q, d) =
+// (tid / R, tid % R) handles digit d for one of Q = 1024 / R slices of the
+// group's tiles, so each thread keeps several independent loads in flight
+// instead of walking the rows one after another.
 constexpr unsigned kPrefixThreads = 1024;
 
 // Step 1: partial[g][d] = count of digit d over group g's tiles (blocks
-// g < groups).  The D blocks after them write the digit bases of pass p =
-// blockIdx.x - groups: base[p][d] = sum_{d' < d} counts[p][d'], the first
+// g < groups).  The passes' blocks after them write the digit bases of pass
+// p = blockIdx.x - groups: base[p][d] = sum_{d' < d} counts[p][d'], the first
 // output slot of digit d.
+template <int RB>
 __global__ __launch_bounds__(kPrefixThreads) void k_tile_sums(const uint32_t* __restrict__ rows, uint32_t ntiles,
                                                               uint32_t per, uint32_t* __restrict__ partial,
                                                               uint32_t groups, const uint32_t* __restrict__ counts,
                                                               uint32_t* __restrict__ base) {
-  __shared__ uint32_t s_q[4][256];
-  const uint32_t d = threadIdx.x & 255, q = threadIdx.x >> 8;
+  constexpr uint32_t R = 1u << RB, Q = kPrefixThreads / R;
+  __shared__ uint32_t s_q[Q][R];
+  const uint32_t d = threadIdx.x % R, q = threadIdx.x / R;
   if (blockIdx.x >= groups) {
-    // Threads 0..255 (waves 0..3) scan the pass's 256 counts.
+    // Threads 0..R-1 scan the pass's R counts.
     const uint32_t p = blockIdx.x - groups, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t c = 0, inc = 0;
     if (q == 0) {
-      c = counts[p * 256 + d];
+      c = counts[p * R + d];
       inc = dev::wave_incl_scan(c, lane);
       if (lane == 63) s_q[0][wave] = inc;
     }
@@ -76,48 +81,59 @@ __global__ __launch_bounds__(kPrefixThreads) void k_tile_sums(const uint32_t* __
     if (q == 0) {
       uint32_t pre = 0;
       for (uint32_t w = 0; w < wave; ++w) pre += s_q[0][w];
-      base[p * 256 + d] = pre + inc - c;
+      base[p * R + d] = pre + inc - c;
     }
     return;
   }
   const uint32_t t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
   uint32_t acc = 0;
 #pragma unroll 8
-  for (uint32_t t = t0 + q; t < t1; t += 4) acc += rows[(size_t)t * 256 + d];
+  for (uint32_t t = t0 + q; t < t1; t += Q) acc += rows[(size_t)t * R + d];
   s_q[q][d] = acc;
   __syncthreads();
-  if (q == 0) partial[blockIdx.x * 256 + d] = s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d];
+  if (q == 0) {
+    uint32_t sum = 0;
+    for (uint32_t w = 0; w < Q; ++w) sum += s_q[w][d];
+    partial[blockIdx.x * R + d] = sum;
+  }
 }
 
 // Step 2: group g adds the partials of groups < g, then scans its own tiles
-// in chunks of 4 x 8: thread (q, d) loads 8 consecutive rows at once.
+// in chunks of Q x 8: thread (q, d) loads 8 consecutive rows at once.
+template <int RB>
 __global__ __launch_bounds__(kPrefixThreads) void k_tile_scan(uint32_t* __restrict__ rows, uint32_t ntiles,
                                                               uint32_t per, const uint32_t* __restrict__ partial) {
-  __shared__ uint32_t s_q[4][256];
-  const uint32_t d = threadIdx.x & 255, q = threadIdx.x >> 8;
+  constexpr uint32_t R = 1u << RB, Q = kPrefixThreads / R;
+  __shared__ uint32_t s_q[Q][R];
+  const uint32_t d = threadIdx.x % R, q = threadIdx.x / R;
   uint32_t acc = 0;
 #pragma unroll 8
-  for (uint32_t g = q; g < blockIdx.x; g += 4) acc += partial[g * 256 + d];
+  for (uint32_t g = q; g < blockIdx.x; g += Q) acc += partial[g * R + d];
   s_q[q][d] = acc;
   __syncthreads();
-  uint32_t base = s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d];
+  uint32_t base = 0;
+  for (uint32_t w = 0; w < Q; ++w) base += s_q[w][d];
   const uint32_t t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
-  for (uint32_t c0 = t0; c0 < t1; c0 += 32) {
+  for (uint32_t c0 = t0; c0 < t1; c0 += 8 * Q) {
     const uint32_t tq = c0 + 8 * q;
     uint32_t v[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = tq + j < t1 ? rows[(size_t)(tq + j) * 256 + d] : 0u;
+    for (int j = 0; j < 8; ++j) v[j] = tq + j < t1 ? rows[(size_t)(tq + j) * R + d] : 0u;
 #pragma unroll
     for (int j = 1; j < 8; ++j) v[j] += v[j - 1];
     __syncthreads();  // every thread has read s_q of the previous step
     s_q[q][d] = v[7];
     __syncthreads();
-    uint32_t off = base;
-    for (uint32_t w = 0; w < q; ++w) off += s_q[w][d];
+    uint32_t off = base, all = 0;
+    for (uint32_t w = 0; w < Q; ++w) {
+      const uint32_t x = s_q[w][d];
+      if (w < q) off += x;
+      all += x;
+    }
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-      if (tq + j < t1) rows[(size_t)(tq + j) * 256 + d] = kLbInc | (off + v[j]);
-    base += s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d];
+      if (tq + j < t1) rows[(size_t)(tq + j) * R + d] = kLbInc | (off + v[j]);
+    base += all;
   }
 }
 
@@ -140,16 +156,22 @@ struct PassArgs {
 };
 
 struct PassVariant {
-  int block, items, lbw, late, minw;
+  int block, items, lbw, late, minw, split, ul, rb;
   int (*launch)(hipStream_t s, const PassArgs& a);
 };
 
+// Digit passes of a sort of K keys with RB-bit digits.
+template <typename K>
+constexpr int sort_passes(int rb) {
+  return (8 * (int)sizeof(K) + rb - 1) / rb;
+}
+
 template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, bool LATE, int MINW, int MODE, bool UNI, int SPLIT,
-          int UL>
+          int UL, int RB>
 int launch_pass(hipStream_t s, const PassArgs& a) {
   // Timed launches carry their events in the dispatch (no extra packets).
   ProfScope ps("k_onesweep", s, true);
-  auto kern = k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI, SPLIT, UL>;
+  auto kern = k_onesweep<K, KV, BLOCK, ITEMS, LBW, MODE, LATE, MINW, UNI, SPLIT, UL, RB>;
   if (ps.start) {
     hipExtLaunchKernelGGL(kern, dim3(a.ntiles), dim3(BLOCK), 0, s, ps.start, ps.stop, 0, static_cast<const K*>(a.kin),
                           static_cast<K*>(a.kout), a.vin, a.vout, a.n, a.gtab, a.lb, a.ctr, a.ntiles, a.shift, a.fmode,
@@ -166,9 +188,11 @@ int launch_pass(hipStream_t s, const PassArgs& a) {
 }
 
 // PA_VARIANT(key, payload, block, items, look-back width, late look-back, min waves/SIMD,
-//            uniform-digit fast path, staging rounds, lanes per ranking unit)
-#define PA_VARIANT(K, KV, B, I, W, P, M, U, S, L) \
-  { B, I, W, P, M, &launch_pass<K, KV, B, I, W, P, M, kModePlain, U, S, L> }
+//            uniform-digit fast path, staging rounds, lanes per ranking unit), 8-bit digits;
+// PA_VARIANT_RB(..., digit bits) for other widths.
+#define PA_VARIANT_RB(K, KV, B, I, W, P, M, U, S, L, RB) \
+  { B, I, W, P, M, S, L, RB, &launch_pass<K, KV, B, I, W, P, M, kModePlain, U, S, L, RB> }
+#define PA_VARIANT(K, KV, B, I, W, P, M, U, S, L) PA_VARIANT_RB(K, KV, B, I, W, P, M, U, S, L, 8)
 
 // Size classes of a sort call: n <= 2^16, <= 2^20, <= 2^24, larger.
 constexpr int kClasses = 4;
@@ -186,7 +210,16 @@ struct Variants<uint32_t, false> {
       PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, true, 1, 32),  PA_VARIANT(uint32_t, false, 512, 48, 16, true, 4, true, 2, 32),
       PA_VARIANT(uint32_t, false, 256, 48, 8, true, 4, true, 2, 32),  PA_VARIANT(uint32_t, false, 512, 48, 8, true, 4, true, 2, 16),
       PA_VARIANT(uint32_t, false, 512, 16, 4, true, 8, true, 1, 16),  PA_VARIANT(uint32_t, false, 512, 24, 4, true, 6, true, 1, 16),
-      PA_VARIANT(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16)};
+      PA_VARIANT(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16),
+      // 4- to 7-bit digits (8, 7, 6, 5 passes): fewer bins per tile, more passes.
+      PA_VARIANT_RB(uint32_t, false, 512, 16, 4, true, 8, true, 1, 16, 7),
+      PA_VARIANT_RB(uint32_t, false, 512, 16, 4, true, 8, true, 1, 16, 6),
+      PA_VARIANT_RB(uint32_t, false, 512, 16, 4, true, 8, true, 1, 16, 5),
+      PA_VARIANT_RB(uint32_t, false, 512, 16, 4, true, 8, true, 1, 16, 4),
+      PA_VARIANT_RB(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16, 7),
+      PA_VARIANT_RB(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16, 6),
+      PA_VARIANT_RB(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16, 5),
+      PA_VARIANT_RB(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16, 4)};
   // 512 x 16 up to 2^24 keys (4M: 0.209 -> 0.180 ms against 512 x 48; even at 16M); above, the 24,576-key
   // split tile as 768 x 32 (12 waves per workgroup, two workgroups per CU: 100M passes 188-227 us against
   // 194-233 us for 512 x 48).
@@ -204,7 +237,11 @@ struct Variants<uint64_t, false> {
   static constexpr PassVariant table[] = {PA_VARIANT(uint64_t, false, 512, 8, 4, true, 8, true, 1, 32),
                                           PA_VARIANT(uint64_t, false, 512, 12, 4, true, 6, true, 1, 32),
                                           PA_VARIANT(uint64_t, false, 512, 16, 8, true, 4, true, 1, 16),
-                                          PA_VARIANT(uint64_t, false, 512, 24, 8, true, 4, true, 2, 16)};
+                                          PA_VARIANT(uint64_t, false, 512, 24, 8, true, 4, true, 2, 16),
+                                          PA_VARIANT_RB(uint64_t, false, 512, 16, 8, true, 4, true, 1, 16, 7),
+                                          PA_VARIANT_RB(uint64_t, false, 512, 16, 8, true, 4, true, 1, 16, 6),
+                                          PA_VARIANT_RB(uint64_t, false, 512, 16, 8, true, 4, true, 1, 16, 5),
+                                          PA_VARIANT_RB(uint64_t, false, 512, 16, 8, true, 4, true, 1, 16, 4)};
   // 512 x 16 with 16-lane units up to 2^20 keys (1M: 0.155 -> 0.149 ms), the 512 x 24 split with
   // 16-lane units above (4M: 0.331 -> 0.254 ms, 16M: 0.798 -> 0.632 ms, 100M: 4.77 -> 3.57 ms).
   static constexpr int by_class[kClasses] = {0, 2, 3, 3};
@@ -218,8 +255,10 @@ struct Variants<uint64_t, true> {
   static constexpr int by_class[kClasses] = {0, 0, 1, 1};
 };
 
-// Choices recorded by the autotuner (-1: none) for u32 keys-only sorts.
-static std::atomic<int> g_tuned_u32[kClasses] = {{-1}, {-1}, {-1}, {-1}};
+// Choices set by the autotuner, a tuning cache or pa_sort_shape_set (-1:
+// none) for keys-only sorts: [0] u32 keys (also float and signed 32-bit),
+// [1] u64 keys.
+static std::atomic<int> g_tuned[2][kClasses] = {{{-1}, {-1}, {-1}, {-1}}, {{-1}, {-1}, {-1}, {-1}}};
 
 template <typename K, bool KV>
 constexpr size_t variant_count() {
@@ -227,7 +266,8 @@ constexpr size_t variant_count() {
 }
 
 // PA_SORT_VARIANT (an index into the key type's table) overrides every
-// class; otherwise the tuned choice, otherwise the built-in one.
+// class; otherwise the tuned choice, otherwise the built-in one.  The first
+// call loads the tuning cache named by PA_TUNE_CACHE, if any.
 template <typename K, bool KV>
 int variant_index(size_t n) {
   static const int forced = [] {
@@ -236,12 +276,20 @@ int variant_index(size_t n) {
     return (v >= 0 && (size_t)v < variant_count<K, KV>()) ? v : -1;
   }();
   if (forced >= 0) return forced;
+  tune_cache_autoload();
   const int c = size_class(n);
-  if (std::is_same<K, uint32_t>::value && !KV) {
-    const int t = g_tuned_u32[c].load(std::memory_order_relaxed);
+  if constexpr (!KV) {
+    const int t = g_tuned[sizeof(K) == 8][c].load(std::memory_order_relaxed);
     if (t >= 0) return t;
   }
   return Variants<K, KV>::by_class[c];
+}
+
+// Names of the candidate shapes, as stored in tuning caches.
+inline std::string variant_name(const PassVariant& v) {
+  char b[96];
+  std::snprintf(b, sizeof(b), "%dx%d/lb%d/sp%d/ul%d/rb%d/w%d", v.block, v.items, v.lbw, v.split, v.ul, v.rb, v.minw);
+  return b;
 }
 
 template <typename K, bool KV>
@@ -260,30 +308,35 @@ constexpr unsigned kPrefixGroups = 128;  // blocks of the pass-0 prefix scan
 // layout).
 template <typename K, bool KV>
 size_t sort_scratch_bytes(const PassVariant& var, size_t n) {
-  constexpr int D = sizeof(K);
+  const size_t D = sort_passes<K>(var.rb), R = size_t(1) << var.rb;
   const size_t rows = sort_rows(var, n);
   Layout L;
-  L.take((size_t)D * 256 * 4);            // counts
-  L.take((size_t)D * 256 * 4);            // bases
+  L.take(D * R * 4);                      // counts
+  L.take(D * R * 4);                      // bases
   L.take(D * 4);                          // tile counters
-  L.take((size_t)D * rows * 256 * 4);     // look-back table
-  L.take((size_t)kPrefixGroups * 256 * 4);  // pass-0 prefix partial sums
+  L.take(D * rows * R * 4);               // look-back table
+  L.take((size_t)kPrefixGroups * R * 4);  // pass-0 prefix partial sums
   return L.off;
 }
 
-template <typename K, bool KV>
-int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, char* scratch,
-                 hipStream_t s, uint32_t xform) {
-  constexpr int D = sizeof(K);
+// The sort with RB-bit digits: D = ceil(key bits / RB) passes.  Scratch
+// layout: counts [D][R], digit bases [D][R], tile counters [D], look-back
+// rows [D][tiles][R], prefix partials [groups][R].  With an odd D the last
+// pass writes into the temporary arrays and the result is copied back.
+template <typename K, bool KV, int RB>
+int enqueue_sort_rb(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, char* scratch,
+                    hipStream_t s, uint32_t xform) {
+  constexpr int D = sort_passes<K>(RB);
+  constexpr size_t R = size_t(1) << RB;
   const size_t rows = sort_rows(var, n);
-  const size_t ncount = (size_t)D * 256;
+  const size_t ncount = (size_t)D * R;
   Layout L;
   uint32_t* counts = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
   uint32_t* bases = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
   uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch + L.take(D * 4));
-  const size_t lb_bytes = (size_t)D * rows * 256 * 4;
+  const size_t lb_bytes = (size_t)D * rows * R * 4;
   uint32_t* lb = reinterpret_cast<uint32_t*>(scratch + L.take(lb_bytes));
-  uint32_t* prefix_partial = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)kPrefixGroups * 256 * 4));
+  uint32_t* prefix_partial = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)kPrefixGroups * R * 4));
 
   {
     ProfScope ps("k_sort_prepare", s);
@@ -295,7 +348,7 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
   // Two small kernels then turn pass 0's rows into inclusive prefixes, so
   // pass 0 runs without look-back waits; the first also writes the digit
   // bases.
-  const size_t pass_rows_bytes = rows * 256 * 4;
+  const size_t pass_rows_bytes = rows * R * 4;
   const size_t zero_bytes = lb_bytes - pass_rows_bytes;
   // Tiles per block: about n / 512 keys per block (two blocks per CU), at
   // least one tile and at most 49,152 keys, so a small block's fixed costs
@@ -307,16 +360,17 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
   const size_t grid = std::max<size_t>(1, (rows + tpb - 1) / tpb);
   {
     ProfScope ps("k_hist_digits", s);
-    hipLaunchKernelGGL(k_hist_tiles<K>, dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, (uint32_t)tile, lb,
-                       counts, reinterpret_cast<uint4*>(lb + rows * 256), zero_bytes / 16, xform);
+    hipLaunchKernelGGL((k_hist_tiles<K, RB>), dim3((unsigned)grid), dim3(kSortBlock), 0, s, keys, n, (uint32_t)tile,
+                       lb, counts, reinterpret_cast<uint4*>(lb + rows * R), zero_bytes / 16, xform);
   }
   const uint32_t groups = (uint32_t)std::min<size_t>(rows, kPrefixGroups);
   const uint32_t per = (uint32_t)((rows + groups - 1) / groups);
   {
     ProfScope ps("k_tile_prefix", s);
-    hipLaunchKernelGGL(k_tile_sums, dim3(groups + D), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per, prefix_partial,
-                       groups, counts, bases);
-    hipLaunchKernelGGL(k_tile_scan, dim3(groups), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per, prefix_partial);
+    hipLaunchKernelGGL(k_tile_sums<RB>, dim3(groups + D), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per,
+                       prefix_partial, groups, counts, bases);
+    hipLaunchKernelGGL(k_tile_scan<RB>, dim3(groups), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per,
+                       prefix_partial);
   }
   K* kin = keys;
   K* kout = ktmp;
@@ -328,14 +382,32 @@ int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint3
     if (xform && p == 0) fmode |= kFloatIn;
     if (xform && p == D - 1) fmode |= kFloatOut;
     if (xform == dev::kXformSigned) fmode |= kSignedKeys;
-    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * 256, lb + (size_t)p * rows * 256, ctr + p,
-               (uint32_t)rows, (uint32_t)(8 * p), fmode};
+    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * R, lb + (size_t)p * rows * R, ctr + p,
+               (uint32_t)rows, (uint32_t)(RB * p), fmode};
     PA_TRY(var.launch(s, a));
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
+  if (D % 2) {
+    // The result is in the temporary arrays.
+    PA_HIP(hipMemcpyAsync(keys, ktmp, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    if constexpr (KV) PA_HIP(hipMemcpyAsync(vals, vtmp, n * 4, hipMemcpyDeviceToDevice, s));
+  }
   PA_HIP(hipGetLastError());
   return PA_OK;
+}
+
+template <typename K, bool KV>
+int enqueue_sort(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, char* scratch,
+                 hipStream_t s, uint32_t xform) {
+  switch (var.rb) {
+    case 8: return enqueue_sort_rb<K, KV, 8>(var, keys, vals, ktmp, vtmp, n, scratch, s, xform);
+    case 7: return enqueue_sort_rb<K, KV, 7>(var, keys, vals, ktmp, vtmp, n, scratch, s, xform);
+    case 6: return enqueue_sort_rb<K, KV, 6>(var, keys, vals, ktmp, vtmp, n, scratch, s, xform);
+    case 5: return enqueue_sort_rb<K, KV, 5>(var, keys, vals, ktmp, vtmp, n, scratch, s, xform);
+    case 4: return enqueue_sort_rb<K, KV, 4>(var, keys, vals, ktmp, vtmp, n, scratch, s, xform);
+    default: return arg_error("sort: digits of 4 to 8 bits");
+  }
 }
 
 int launch_hist_digits_u32(const uint32_t* keys, size_t n, uint32_t* counts, hipStream_t s) {
@@ -387,7 +459,7 @@ int enqueue_large_sort(K* keys, K* tmp, size_t n, char* scratch, hipStream_t s, 
   std::vector<std::pair<size_t, size_t>> runs;  // (first key, length)
   for (size_t off = 0; off < n; off += csize) {
     const size_t len = std::min(csize, n - off);
-    // An even number of digit passes leaves each chunk's result in keys.
+    // Each chunk's result is in keys (an odd pass count copies it back).
     PA_TRY((enqueue_sort<K, false>(var, keys + off, nullptr, tmp + off, nullptr, len, scratch, s, dev::kXformNone)));
     runs.emplace_back(off, len);
   }
@@ -522,6 +594,144 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   return PA_OK;
 }
 
+// Times every candidate shape of K keys at one size per class (up to max_n
+// keys) on a fresh random array and records the fastest per class.  Each
+// timing covers the whole sort, including the copy back of an odd pass count.
+template <typename K>
+int autotune_keys(size_t max_n, hipStream_t s) {
+  if (max_n < 2) return arg_error("pa_autotune_sort: need max_n >= 2");
+  static const size_t rep[kClasses] = {size_t(1) << 16, size_t(1) << 20, size_t(1) << 24, size_t(100000000)};
+  const size_t nmax = std::min(max_n, std::min(rep[kClasses - 1], kMaxSortN));
+  constexpr size_t NV = variant_count<K, false>();
+  const int target = sizeof(K) == 8;
+  size_t scratch_bytes = 0;
+  for (size_t v = 0; v < NV; ++v)
+    scratch_bytes = std::max(scratch_bytes, sort_scratch_bytes<K, false>(Variants<K, false>::table[v], nmax));
+  K *pristine = nullptr, *keys = nullptr, *tmp = nullptr;
+  char* scratch = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = PA_OK;
+  auto fail = [&](hipError_t e, const char* what) {
+    st = hip_fail(e, what, __FILE__, __LINE__);
+    return st;
+  };
+  do {
+    hipError_t e;
+    if ((e = hipMalloc(&pristine, nmax * sizeof(K))) != hipSuccess || (e = hipMalloc(&keys, nmax * sizeof(K))) != hipSuccess ||
+        (e = hipMalloc(&tmp, nmax * sizeof(K))) != hipSuccess || (e = hipMalloc(&scratch, scratch_bytes)) != hipSuccess) {
+      fail(e, "hipMalloc");
+      break;
+    }
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+      fail(e, "hipEventCreate");
+      break;
+    }
+    if constexpr (sizeof(K) == 8)
+      st = pa_fill_random_u64(reinterpret_cast<uint64_t*>(pristine), nmax, 0x5eed, s);
+    else
+      st = pa_fill_random_u32(reinterpret_cast<uint32_t*>(pristine), nmax, 0x5eed, s);
+    if (st != PA_OK) break;
+    size_t prev_n = 0;
+    int prev_best = -1;
+    for (int c = 0; c < kClasses && st == PA_OK; ++c) {
+      const size_t n = std::min(rep[c], nmax);
+      if (n == prev_n) {
+        g_tuned[target][c].store(prev_best);
+        continue;
+      }
+      float best_ms = 0.f;
+      int best = -1;
+      for (size_t v = 0; v < NV && st == PA_OK; ++v) {
+        const PassVariant& var = Variants<K, false>::table[v];
+        float total = 0.f;
+        for (int rep_i = 0; rep_i < 4 && st == PA_OK; ++rep_i) {  // first run warms up, three are timed
+          if ((e = hipMemcpyAsync(keys, pristine, n * sizeof(K), hipMemcpyDeviceToDevice, s)) != hipSuccess) {
+            fail(e, "hipMemcpyAsync");
+            break;
+          }
+          (void)hipEventRecord(e0, s);
+          if ((st = enqueue_sort<K, false>(var, keys, nullptr, tmp, nullptr, n, scratch, s, dev::kXformNone)) != PA_OK)
+            break;
+          (void)hipEventRecord(e1, s);
+          if ((e = hipEventSynchronize(e1)) != hipSuccess) {
+            fail(e, "hipEventSynchronize");
+            break;
+          }
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (rep_i > 0) total += ms;
+        }
+        if (st == PA_OK && (best < 0 || total < best_ms)) {
+          best_ms = total;
+          best = (int)v;
+        }
+      }
+      if (st == PA_OK) {
+        g_tuned[target][c].store(best);
+        prev_n = n;
+        prev_best = best;
+      }
+    }
+  } while (false);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(pristine);
+  (void)hipFree(keys);
+  (void)hipFree(tmp);
+  (void)hipFree(scratch);
+  return st;
+}
+
+// The keys-only sort targets of the tuning state and caches.
+template <typename K>
+static TuneTarget keys_target(const char* key) {
+  TuneTarget t;
+  t.key = key;
+  t.classes = kClasses;
+  t.count = [] { return (int)variant_count<K, false>(); };
+  t.name = [](int i) { return variant_name(Variants<K, false>::table[i]); };
+  t.get = [](int c) { return g_tuned[sizeof(K) == 8][c].load(); };
+  t.set = [](int c, int i) { g_tuned[sizeof(K) == 8][c].store(i); };
+  return t;
+}
+
+const TuneTarget* sort_tune_target(int target) {
+  static const TuneTarget t32 = keys_target<uint32_t>("u32");
+  static const TuneTarget t64 = keys_target<uint64_t>("u64");
+  return target == PA_TUNE_U32 ? &t32 : target == PA_TUNE_U64 ? &t64 : nullptr;
+}
+
+template <typename K>
+static void fill_shape(const PassVariant& v, pa_sort_shape* out) {
+  out->block = v.block;
+  out->items = v.items;
+  out->lookback = v.lbw;
+  out->split = v.split;
+  out->lanes_per_unit = v.ul;
+  out->digit_bits = v.rb;
+  out->passes = sort_passes<K>(v.rb);
+  out->queue_max_keys = 0;
+}
+
+int sort_shape_info(int target, int index, pa_sort_shape* out) {
+  if (target == PA_TUNE_U32) {
+    if (index < 0 || (size_t)index >= variant_count<uint32_t, false>()) return arg_error("pa_sort_shape_info: index");
+    fill_shape<uint32_t>(Variants<uint32_t, false>::table[index], out);
+  } else {
+    if (index < 0 || (size_t)index >= variant_count<uint64_t, false>()) return arg_error("pa_sort_shape_info: index");
+    fill_shape<uint64_t>(Variants<uint64_t, false>::table[index], out);
+  }
+  return PA_OK;
+}
+
+int sort_shape_choice(int target, size_t n) {
+  return target == PA_TUNE_U64 ? variant_index<uint64_t, false>(n) : variant_index<uint32_t, false>(n);
+}
+
+int autotune_sort(int target, size_t max_n, hipStream_t s) {
+  return target == PA_TUNE_U64 ? autotune_keys<uint64_t>(max_n, s) : autotune_keys<uint32_t>(max_n, s);
+}
+
 }  // namespace pa
 
 using namespace pa;
@@ -587,88 +797,15 @@ int pa_sort_kv_u64_u32(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t*
 }
 
 int pa_autotune_sort_u32(size_t max_n, pa_stream_t stream) {
-  if (max_n < 2) return arg_error("pa_autotune_sort_u32: need max_n >= 2");
-  hipStream_t s = (hipStream_t)stream;
-  static const size_t rep[kClasses] = {size_t(1) << 16, size_t(1) << 20, size_t(1) << 24, size_t(100000000)};
-  const size_t nmax = std::min(max_n, std::min(rep[kClasses - 1], kMaxSortN));
-  constexpr size_t NV = variant_count<uint32_t, false>();
-  size_t scratch_bytes = 0;
-  for (size_t v = 0; v < NV; ++v)
-    scratch_bytes = std::max(scratch_bytes, sort_scratch_bytes<uint32_t, false>(Variants<uint32_t, false>::table[v], nmax));
-  uint32_t *pristine = nullptr, *keys = nullptr, *tmp = nullptr;
-  char* scratch = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  int st = PA_OK;
-  auto fail = [&](hipError_t e, const char* what) {
-    st = hip_fail(e, what, __FILE__, __LINE__);
-    return st;
-  };
-  do {
-    hipError_t e;
-    if ((e = hipMalloc(&pristine, nmax * 4)) != hipSuccess || (e = hipMalloc(&keys, nmax * 4)) != hipSuccess ||
-        (e = hipMalloc(&tmp, nmax * 4)) != hipSuccess || (e = hipMalloc(&scratch, scratch_bytes)) != hipSuccess) {
-      fail(e, "hipMalloc");
-      break;
-    }
-    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
-      fail(e, "hipEventCreate");
-      break;
-    }
-    if ((st = pa_fill_random_u32(pristine, nmax, 0x5eed, stream)) != PA_OK) break;
-    size_t prev_n = 0;
-    int prev_best = -1;
-    for (int c = 0; c < kClasses && st == PA_OK; ++c) {
-      const size_t n = std::min(rep[c], nmax);
-      if (n == prev_n) {
-        g_tuned_u32[c].store(prev_best);
-        continue;
-      }
-      float best_ms = 0.f;
-      int best = -1;
-      for (size_t v = 0; v < NV && st == PA_OK; ++v) {
-        const PassVariant& var = Variants<uint32_t, false>::table[v];
-        float total = 0.f;
-        for (int rep_i = 0; rep_i < 4 && st == PA_OK; ++rep_i) {  // first run warms up, three are timed
-          if ((e = hipMemcpyAsync(keys, pristine, n * 4, hipMemcpyDeviceToDevice, s)) != hipSuccess) {
-            fail(e, "hipMemcpyAsync");
-            break;
-          }
-          (void)hipEventRecord(e0, s);
-          if ((st = enqueue_sort<uint32_t, false>(var, keys, nullptr, tmp, nullptr, n, scratch, s, dev::kXformNone)) != PA_OK) break;
-          (void)hipEventRecord(e1, s);
-          if ((e = hipEventSynchronize(e1)) != hipSuccess) {
-            fail(e, "hipEventSynchronize");
-            break;
-          }
-          float ms = 0.f;
-          (void)hipEventElapsedTime(&ms, e0, e1);
-          if (rep_i > 0) total += ms;
-        }
-        if (st == PA_OK && (best < 0 || total < best_ms)) {
-          best_ms = total;
-          best = (int)v;
-        }
-      }
-      if (st == PA_OK) {
-        g_tuned_u32[c].store(best);
-        prev_n = n;
-        prev_best = best;
-      }
-    }
-  } while (false);
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  (void)hipFree(pristine);
-  (void)hipFree(keys);
-  (void)hipFree(tmp);
-  (void)hipFree(scratch);
-  return st;
+  return pa_autotune_sort(PA_TUNE_U32, max_n, stream);
 }
 
 int pa_sort_variant_u32(size_t n) { return variant_index<uint32_t, false>(n); }
 
 int pa_autotune_reset(void) {
-  for (int c = 0; c < kClasses; ++c) g_tuned_u32[c].store(-1);
+  for (int t = 0; t < 2; ++t)
+    for (int c = 0; c < kClasses; ++c) g_tuned[t][c].store(-1);
+  seg_tune_reset();
   return PA_OK;
 }
 

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 }
 
 
-// One stable 8-bit digit pass (onesweep).  A block ranks a tile of
+// One stable RB-bit digit pass (onesweep), R = 2^RB digits (RB = 4..8).  A block ranks a tile of
 // BLOCK x ITEMS keys in LDS, publishes the tile's per-digit counts to a
 // decoupled look-back table, resolves its global offsets from the
 // predecessors' words and writes the tile out in digit order.
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 // Ranking units are groups of UL lanes (32 or 16).  Unit u owns the USPAN =
 // UL x ITEMS consecutive keys [u*USPAN, (u+1)*USPAN) of the tile; item i of
 // lane l is key u*USPAN + i*UL + l, so (unit, item, lane) order is input order
-// and a load instruction reads 64/UL contiguous runs.  Each unit has 256 LDS
+// and a load instruction reads 64/UL contiguous runs.  Each unit has R LDS
 // slots: one part collects the lanes holding a digit in the current item
 // (each lane ORs its bit), the other counts that digit's keys seen so far in
 // the unit.  One read returns both, so a key's stable rank in its unit is
@@ -138,15 +138,16 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 // of one wave execute in program order and OR commutes, so the result does
 // not depend on the order in which the LDS applies the lanes.
 //
-// Partial tiles are padded with all-ones keys: their digit is 255 in every
-// pass, so they rank after every real key of the tile and occupy the staging
+// Partial tiles are padded with all-ones keys: their digit is the largest any
+// key can have in every pass (R - 1, or all ones in a narrower last digit),
+// so they rank after every real key of the tile and occupy the staging
 // slots [valid, TILE), which are never written out.  Only the last tile of an
 // array (or segment) is partial, and no tile looks back past the first tile
 // of its own array, so the padded count it publishes is never consumed.
 //
 // Segmented mode: tile_info[t] = (segment << 32) | tile index within the
 // segment; the segment's input/output addresses and length come from seg_in /
-// seg_out / seg_len, its digit counts from gtab + segment * gstride * 256 and
+// seg_out / seg_len, its digit counts from gtab + segment * gstride * R and
 // its first look-back row from seg_row[segment] (a segment's rows are
 // contiguous; its tiles may be claimed interleaved with other segments').
 // Plain mode: gtab holds the pass's exclusive digit bases.
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_digits(const K* __restrict_
 // tile larger than the LDS staging area can be held in registers: the block
 // then has more keys in flight per byte of LDS and per look-back.
 template <typename K, bool KV, int BLOCK, int ITEMS, int LBW, int MODE = kModePlain, bool LATE = false, int MINW = 1,
-          bool UNI = true, int SPLIT = 1, int UL = 32>
+          bool UNI = true, int SPLIT = 1, int UL = 32, int RB = 8>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void k_onesweep(
     const K* __restrict__ kin, K* __restrict__ kout, const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
     uint32_t n, const uint32_t* __restrict__ gtab, uint32_t* __restrict__ lb, uint32_t* __restrict__ tile_counter,
@@ -175,7 +176,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     const uint32_t* __restrict__ seg_len = nullptr, uint32_t gstride = 0,
     const uint32_t* __restrict__ seg_row = nullptr) {
   constexpr bool SEG = MODE == kModeSeg;
-  static_assert(BLOCK >= 256 && BLOCK <= 1024 && BLOCK % 64 == 0, "one thread per digit is needed");
+  static_assert(RB >= 4 && RB <= 8, "digits of 4 to 8 bits");
+  constexpr uint32_t R = 1u << RB;
+  static_assert(BLOCK >= (int)R && BLOCK <= 1024 && BLOCK % 64 == 0, "one thread per digit is needed");
   static_assert(!(SEG && KV), "segmented mode sorts keys only");
   static_assert(UL == 32 || UL == 16, "ranking units are 16 or 32 lanes");
   constexpr int UNITS = BLOCK / UL;
@@ -186,9 +189,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   static_assert(ITEMS % SPLIT == 0 && (SPLIT == 1 || !KV), "split staging is for keys-only tiles");
   constexpr int STAGE = TILE / SPLIT;      // staging slots per round
   constexpr int RITEMS = ITEMS / SPLIT;    // write-out slots per thread per round
-  static_assert(STAGE * sizeof(K) >= UNITS * 256 * SLOTB, "rank slots must fit the key staging area");
+  static_assert(STAGE * sizeof(K) >= UNITS * R * SLOTB, "rank slots must fit the key staging area");
   __shared__ __align__(16) K s_keys[STAGE];
-  __shared__ uint32_t s_goff[256];
+  __shared__ uint32_t s_goff[R];
   __shared__ uint32_t s_wsum[2][4];
   __shared__ uint32_t s_claim;
 
@@ -200,13 +203,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   const uint32_t lanebit = 1u << ul;
   // Rank slots of this thread's unit (they alias the key staging area, which
   // is idle until the ranking is over).
-  uint64_t* const uslot = reinterpret_cast<uint64_t*>(s_keys) + unit * 256;
+  uint64_t* const uslot = reinterpret_cast<uint64_t*>(s_keys) + unit * R;
   uint32_t* const uword = reinterpret_cast<uint32_t*>(uslot);
   // High words of every unit's slots, for the per-digit pass over units.
   uint32_t* const hiword = reinterpret_cast<uint32_t*>(s_keys) + 1;
-  // 16-lane units: slot (u, d) is word u * 256 + d, count in the high half.
+  // 16-lane units: slot (u, d) is word u * R + d, count in the high half.
   uint32_t* const slot16 = reinterpret_cast<uint32_t*>(s_keys);
-  uint32_t* const uslot16 = slot16 + unit * 256;
+  uint32_t* const uslot16 = slot16 + unit * R;
   const bool float_in = (fmode & kFloatIn) != 0;
   const bool float_out = (fmode & kFloatOut) != 0;
   const uint32_t xform = (fmode & kSignedKeys) ? dev::kXformSigned : dev::kXformFloat;
@@ -214,8 +217,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   // Clear the rank slots while the claim is in flight.
   {
     uint4* z = reinterpret_cast<uint4*>(s_keys);
+    constexpr uint32_t ZV = UNITS * R * SLOTB / 16;  // 16-byte words of rank slots
 #pragma unroll
-    for (int r = 0; r < UNITS * 256 * SLOTB / 16 / BLOCK; ++r) z[tid + r * BLOCK] = make_uint4(0, 0, 0, 0);
+    for (uint32_t i = tid; i < ZV; i += BLOCK) z[i] = make_uint4(0, 0, 0, 0);
   }
   if (tid == 0) s_claim = atomicAdd(tile_counter, 1u);
   __syncthreads();
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     in = reinterpret_cast<const K*>(seg_in[seg]);
     out = reinterpret_cast<K*>(seg_out[seg]);
     len = seg_len[seg];
-    g = gtab + (size_t)seg * gstride * 256;
+    g = gtab + (size_t)seg * gstride * R;
   } else {
     local = claimed;
     in = kin;
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   };
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t d = dev::digit8<K>(key[i], shift);
+    const uint32_t d = dev::digit_bits<K, RB>(key[i], shift);
     if (UNI && __all(d == __builtin_amdgcn_readfirstlane(d))) {
       // Every lane holds the same digit (constant or presorted runs): the
       // peers of each unit are all its lanes, so no lane mask is needed and
@@ -360,24 +364,24 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   __syncthreads();
 
   // Per digit: prefix over units, tile count, publish the aggregate, scans
-  // over the 256 digits of the tile counts (and, in SEG mode, of the
+  // over the R digits of the tile counts (and, in SEG mode, of the
   // segment's digit counts).
   uint32_t cnt = 0, gval = 0, cinc = 0, ginc = 0;
-  if (tid < 256) {
+  if (tid < R) {
 #pragma unroll
     for (int u = 0; u < UNITS; ++u) {
       if constexpr (UL == 32) {
-        const uint32_t c = hiword[2 * (u * 256 + tid)];
-        hiword[2 * (u * 256 + tid)] = cnt;
+        const uint32_t c = hiword[2 * (u * R + tid)];
+        hiword[2 * (u * R + tid)] = cnt;
         cnt += c;
       } else {
-        const uint32_t c = slot16[u * 256 + tid] >> 16;
-        slot16[u * 256 + tid] = cnt << 16;
+        const uint32_t c = slot16[u * R + tid] >> 16;
+        slot16[u * R + tid] = cnt << 16;
         cnt += c;
       }
     }
     if (!(fmode & kPrefixReady))
-      dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
+      dev::atomic_store_agent(&lb[(size_t)row * R + tid], (local == 0 ? kLbInc : kLbAgg) | cnt);
     if constexpr (SEG) gval = g[tid];
     cinc = dev::wave_incl_scan(cnt, lane);
     if constexpr (SEG) ginc = dev::wave_incl_scan(gval, lane);
@@ -396,9 +400,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   uint32_t w[LBW];
   auto poll = [&]() {
     const uint32_t avail = p - first_row + 1;
-    const uint32_t* at = lb + (size_t)p * 256 + tid;  // one address, immediate offsets
+    const uint32_t* at = lb + (size_t)p * R + tid;  // one address, immediate offsets
 #pragma unroll
-    for (int j = 0; j < LBW; ++j) w[j] = (uint32_t)j < avail ? dev::atomic_load_agent(at - j * 256) : kLbInc;
+    for (int j = 0; j < LBW; ++j) w[j] = (uint32_t)j < avail ? dev::atomic_load_agent(at - j * R) : kLbInc;
   };
   auto consume = [&]() {  // true once an inclusive prefix is reached
     uint32_t used = 0;
@@ -424,11 +428,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     if (local > 0) {
       if (!polled) poll();
       while (!consume()) poll();
-      if (!(fmode & kPrefixReady)) dev::atomic_store_agent(&lb[(size_t)row * 256 + tid], kLbInc | (excl + cnt));
+      if (!(fmode & kPrefixReady)) dev::atomic_store_agent(&lb[(size_t)row * R + tid], kLbInc | (excl + cnt));
     }
     s_goff[tid] = gstart + excl - tstart;
   };
-  if (tid < 256) {
+  if (tid < R) {
     uint32_t cpre = 0, gpre = 0;
     for (uint32_t wv = 0; wv < wave; ++wv) {
       cpre += s_wsum[0][wv];
@@ -439,9 +443,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 #pragma unroll
     for (int u = 0; u < UNITS; ++u) {
       if constexpr (UL == 32)
-        hiword[2 * (u * 256 + tid)] += tstart;
+        hiword[2 * (u * R + tid)] += tstart;
       else
-        slot16[u * 256 + tid] += tstart << 16;
+        slot16[u * R + tid] += tstart << 16;
     }
     if (!LATE) resolve(false);
   }
@@ -458,8 +462,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     for (int i = g0; i < g0 + LG && i < ITEMS; ++i) asm volatile("" : "+v"(key[i]));
 #pragma unroll
     for (int i = g0; i < g0 + LG && i < ITEMS; ++i) {
-      const uint32_t dg = dev::digit8<K>(key[i], shift);
-      const uint32_t start = UL == 32 ? hiword[2 * (unit * 256 + dg)] : uslot16[dg] >> 16;
+      const uint32_t dg = dev::digit_bits<K, RB>(key[i], shift);
+      const uint32_t start = UL == 32 ? hiword[2 * (unit * R + dg)] : uslot16[dg] >> 16;
       if constexpr (PACK)
         rank[i >> 1] += (i & 1) ? start << 16 : start;  // halves stay below 2^16: no carry
       else
@@ -490,7 +494,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       for (int i = 0; i < G; ++i) wk[i] = s_keys[tid + (g0 + i) * BLOCK];
 #pragma unroll
       for (int i = 0; i < G; ++i) {
-        gp[i] = s_goff[dev::digit8<K>(wk[i], shift)] + r * STAGE + tid + (g0 + i) * BLOCK;
+        gp[i] = s_goff[dev::digit_bits<K, RB>(wk[i], shift)] + r * STAGE + tid + (g0 + i) * BLOCK;
         if constexpr (KV) gpos[g0 + i] = gp[i];
         if constexpr (decltype(fo)::value) wk[i] = dev::key_inv<K>(wk[i], xform);
       }
@@ -523,9 +527,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
       // LATE: the first look-back round trip is issued halfway through the
       // scatter, when half of the keys and ranks no longer need registers.
       scatter(0, ITEMS / 2, 0);
-      if (LATE && tid < 256 && local > 0) poll();
+      if (LATE && tid < R && local > 0) poll();
       scatter(ITEMS / 2, ITEMS, 0);
-      if (LATE && tid < 256) resolve(true);
+      if (LATE && tid < R) resolve(true);
     } else {
       scatter(0, ITEMS, r);
     }
@@ -551,8 +555,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 
 // Histogram read that also prepares pass 0's look-back rows: blocks take
 // whole tiles of `tile` keys (pass 0's tiles), write each tile's digit-0
-// counts to rows[t][256] and accumulate all D digit histograms into counts
-// [D][256].  Also clears the look-back rows of passes 1..D-1 (zero_vec
+// counts to rows[t][R] and accumulate all D digit histograms into counts
+// [D][R] (R = 2^RB, D = ceil(key bits / RB)).  Also clears the look-back rows of passes 1..D-1 (zero_vec
 // 16-byte words at zero_ptr).
 //
 // Counting uses lane-private sub-histograms (SubHist, packed 16-bit
@@ -561,16 +565,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 // bin sees at most 1/C of the block's keys).
 constexpr uint32_t kHistTilesMaxKeys = 1u << 19;
 
-template <typename K>
+template <typename K, int RB = 8>
 __global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__ keys, size_t n, uint32_t tile,
                                                            uint32_t* __restrict__ rows, uint32_t* __restrict__ counts,
                                                            uint4* __restrict__ zero_ptr, size_t zero_vec,
                                                            uint32_t xform) {
-  constexpr int D = sizeof(K);
+  constexpr int D = (8 * (int)sizeof(K) + RB - 1) / RB;
+  constexpr uint32_t R = 1u << RB;
   constexpr int PER = 16 / sizeof(K);
-  constexpr int C = D == 4 ? 32 : 16;  // copies: 4 x 16 KB (u32) or 8 x 8 KB (u64) of LDS
+  // Copies: 32 for u32 keys, 16 for u64 (8-bit digits: 4 x 16 KB or 8 x 8 KB of LDS).
+  constexpr int C = sizeof(K) == 4 ? 32 : 16;
   using H = dev::SubHist<C, true>;
-  constexpr uint32_t W = H::words(256);  // words per digit
+  constexpr uint32_t W = H::words(R);  // words per digit
   __shared__ __align__(16) uint32_t s_h[D * W];
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
@@ -585,11 +591,11 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__
     for (int d = 0; d < D; ++d) {
       uint32_t dg[PER];
 #pragma unroll
-      for (int j = 0; j < PER; ++j) dg[j] = dev::digit8<K>(k[j], 8 * d);
+      for (int j = 0; j < PER; ++j) dg[j] = dev::digit_bits<K, RB>(k[j], RB * d);
       H::template add_all<PER>(s_h + d * W, dg, lane);
     }
   };
-  uint32_t acc0 = 0;  // thread b < 256: digit-0 count of bin b over the block's tiles
+  uint32_t acc0 = 0;  // thread b < R: digit-0 count of bin b over the block's tiles
   for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const size_t lo = t * tile, hi = min(lo + tile, n);
     size_t scalar_from = lo;
@@ -621,24 +627,24 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__
       K kk = keys[i];
       if (xform) kk = dev::key_fwd<K>(kk, xform);
 #pragma unroll
-      for (int d = 0; d < D; ++d) H::add(s_h + d * W, dev::digit8<K>(kk, 8 * d), lane, 1u);
+      for (int d = 0; d < D; ++d) H::add(s_h + d * W, dev::digit_bits<K, RB>(kk, RB * d), lane, 1u);
     }
     __syncthreads();
-    if (tid < 256) {
+    if (tid < R) {
       const uint32_t c = H::sum(s_h, tid);
-      rows[t * 256 + tid] = c;
+      rows[t * R + tid] = c;
       acc0 += c;
     }
     __syncthreads();
     for (uint32_t i = tid; i < W / 4; i += kSortBlock) reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0, 0, 0, 0);
     __syncthreads();
   }
-  if (tid < 256) {
+  if (tid < R) {
     if (acc0) atomicAdd(&counts[tid], acc0);
 #pragma unroll
     for (int d = 1; d < D; ++d) {
       const uint32_t c = H::sum(s_h + d * W, tid);
-      if (c) atomicAdd(&counts[d * 256 + tid], c);
+      if (c) atomicAdd(&counts[d * R + tid], c);
     }
   }
 }

@@ -73,6 +73,30 @@ def test_counting_pass_is_stable_digit_sort(orc, shift):
     np.testing.assert_array_equal(orc.counting_pass_u32(a, shift), a[np.argsort(d, kind="stable")])
 
 
+@pytest.mark.parametrize("bits", [4, 5, 6, 7, 8])
+@pytest.mark.parametrize("wide", [False, True])
+def test_sort_radix_bits_every_width(orc, bits, wide):
+    """The width-generic SortRadixInnerPar: sorted output at every width (odd
+    pass counts included: 7 and 5 for u32 at 5 and 7 bits), and after p
+    passes the array is stably sorted by its low p * bits bits -- which pins
+    the digit order and the stability of each pass (a wrong shift or an
+    unstable scatter fails it)."""
+    n = 50_003
+    a = P.gen_u64(77 + bits, n) if wide else P.gen_u32(77 + bits, n)
+    if not wide:
+        a[::7] = a[3]  # ties
+    full = orc.sort_radix_bits(a, bits)
+    np.testing.assert_array_equal(full, np.sort(a))
+    kb = 64 if wide else 32
+    passes = (kb + bits - 1) // bits
+    for p in (1, 2, passes - 1):
+        got = orc.sort_radix_bits(a, bits, max_passes=p, thr=4096)
+        low = min(kb, p * bits)
+        mask = np.uint64((1 << low) - 1) if wide else np.uint32((1 << low) - 1)
+        expect = a[np.argsort(a & mask, kind="stable")]
+        np.testing.assert_array_equal(got, expect)
+
+
 @pytest.mark.parametrize("n", [0, 1, 100, 100_003])
 def test_sort_u64_full_range(orc, n):
     a = P.gen_u64(1235, n)
