@@ -151,6 +151,27 @@ def bench_segmented(pa, torch, count=10_000, seed=1239):
     ms_graph = e0.elapsed_time(e1)
     graph_same = graph_same and bool(torch.equal(work, work2))
     plan.close()
+    # Tuned: pa_autotune_segmented_u32 times every segmented shape (onesweep
+    # tile and digit width, queue bound) on random keys of these lengths; the
+    # batch then runs with its choice and must equal the built-in result.
+    # The plain u32 classes are tuned too (the per-array calls' shapes).
+    pa.autotune_segmented_u32(lens)
+    pa.autotune_sort(pa.TUNE_U32, 100_000_000)
+    seg_choice = pa.sort_shape_choice(pa.TUNE_SEG_U32)
+    work2.copy_(pristine)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    pa.segmented_sort_u32(work2, tmp, offs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_tuned = e0.elapsed_time(e1)
+    tuned_same = bool(torch.equal(work, work2))
+    u32_shapes = pa.sort_shapes(pa.TUNE_U32)
+    tuned = {"segmented_shape": pa.sort_shapes(pa.TUNE_SEG_U32)[seg_choice], "ms_batched": round(ms_tuned, 3),
+             "bit_exact_vs_builtin": tuned_same,
+             "u32_class_shapes": [u32_shapes[pa.sort_shape_choice(pa.TUNE_U32, n)]
+                                  for n in (1 << 16, 1 << 20, 1 << 24, 100_000_000)]}
+    pa.autotune_reset()
     del slices, work, work2, tmp, pristine
     torch.cuda.empty_cache()
     return {"workload": f"{count} u32 arrays, lengths log-uniform [64, 4e6], 5% presorted", "keys": total,
@@ -158,7 +179,7 @@ def bench_segmented(pa, torch, count=10_000, seed=1239):
             "gkeys_per_s_batched": round(total / ms_batch / 1e6, 2),
             "gkeys_per_s_per_array": round(total / ms_single / 1e6, 2),
             "speedup": round(ms_single / ms_batch, 2), "bit_exact_vs_per_array": same,
-            "ms_graph_plan_replay": round(ms_graph, 3), "graph_bit_exact": graph_same}
+            "ms_graph_plan_replay": round(ms_graph, 3), "graph_bit_exact": graph_same, "tuned": tuned}
 
 
 def bench_serve(pa, torch, jobs=500, seed=1240, loads=(0.5, 0.8, 0.95)):

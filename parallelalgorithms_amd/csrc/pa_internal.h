@@ -71,7 +71,7 @@ int release_stream_scratch(hipStream_t s);
 size_t sort_call_scratch_bytes(bool u64, size_t n);
 size_t select_call_scratch_bytes(bool u64, size_t n);
 size_t hist_call_scratch_bytes(unsigned bits);
-int segmented_ptrs_scratch_bytes(uint32_t* const* keys, const uint64_t* lens, size_t num, size_t* bytes);
+int segmented_lens_scratch_bytes(const uint64_t* lens, size_t num, size_t* bytes);
 int ensure_pinned(StreamCtx& ctx, size_t bytes, void** out);
 // A pinned upload buffer of >= bytes whose previous copy has finished (waits
 // only if all ring slots are still in flight).  After enqueuing the copy that

@@ -763,13 +763,23 @@ int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets
 }  // extern "C"
 
 namespace pa {
-int segmented_ptrs_scratch_bytes(uint32_t* const* keys, const uint64_t* lens, size_t num, size_t* bytes) {
+// Scratch of pa_segmented_sort_ptrs_u32 for these segment lengths: the same
+// plan and layout, from the lengths alone (no pointer checks).
+int segmented_lens_scratch_bytes(const uint64_t* lens, size_t num, size_t* bytes) {
   *bytes = 0;
   if (num == 0) return PA_OK;
-  if (!keys || !lens) return arg_error("segmented sort: null table");
+  if (!lens) return arg_error("segmented sort: null length list");
   SegPlan P;
   uint64_t large_total = 0;
-  PA_TRY(build_ptr_plan(keys, lens, num, "segmented sort", &P, &large_total));
+  for (size_t i = 0; i < num; ++i) {
+    if (lens[i] >= (1ull << 30)) {
+      set_last_error("segmented sort: a segment holds 2^30 or more keys");
+      return PA_ERR_TOO_LARGE;
+    }
+    P.add(1, lens[i], 0);  // addresses do not enter the layout
+    if (lens[i] > kLdsMax) large_total += lens[i];
+  }
+  P.finalize();
   *bytes = plan_layout(P, large_total * 4).total;
   return PA_OK;
 }

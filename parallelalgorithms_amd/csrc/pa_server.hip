@@ -101,14 +101,10 @@ static size_t call_bytes(const pa_job& j) {
 
 // Scratch of a coalesced batch: the segmented sort's plan for these arrays.
 static size_t batch_bytes(const SUnit& u) {
-  std::vector<uint32_t*> ptrs;
   std::vector<uint64_t> lens;
-  for (SJob* j : u.jobs) {
-    ptrs.push_back((uint32_t*)j->d.keys);
-    lens.push_back(j->d.n);
-  }
+  for (SJob* j : u.jobs) lens.push_back(j->d.n);
   size_t b = 0;
-  if (segmented_ptrs_scratch_bytes(ptrs.data(), lens.data(), ptrs.size(), &b) != PA_OK) {
+  if (segmented_lens_scratch_bytes(lens.data(), lens.size(), &b) != PA_OK) {
     // The launch reports the error; price the batch as its keys plus a block.
     b = 0;
     for (SJob* j : u.jobs) b += j->d.n * 8;
