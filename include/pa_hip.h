@@ -17,11 +17,11 @@
  *    run concurrently without sharing scratch.
  *  - Return value: PA_OK (0) or a pa_status code; nothing is retried or
  *    silently redirected to the CPU.
- *  - Sizes: keys-only sorts take any n that fits device memory (arrays of
- *    2^30 keys or more are sorted in chunks and merged); key-value sorts and
- *    segments of a segmented sort take n < 2^30; histograms, select,
- *    partition and the counting sort take n < 2^32.  Beyond a limit the call
- *    returns PA_ERR_TOO_LARGE.
+ *  - Sizes: sorts (keys-only and key-value) and segments of a segmented sort
+ *    take any n that fits device memory: arrays of 2^30 keys or more (past a
+ *    look-back word's 30-bit count) are sorted in chunks of fewer and merged
+ *    pairwise, stably; histograms, select, partition and the counting sort
+ *    take n < 2^32.  Beyond a limit the call returns PA_ERR_TOO_LARGE.
  *
  * Each declaration names the reference interface it replaces
  * (DragonSpit/ParallelAlgorithms, file:line).

@@ -171,6 +171,14 @@ void tune_cache_autosave();
 size_t merge_split_bytes(size_t n);
 template <typename K>
 int merge_enqueue(const K* a, size_t na, const K* b, size_t nb, K* out, size_t* split, hipStream_t s);
+// Stable merge of key-value runs (ties take the first run), same split table.
+template <typename K>
+int merge_kv_enqueue(const K* a, const uint32_t* va, size_t na, const K* b, const uint32_t* vb, size_t nb, K* out,
+                     uint32_t* vout, size_t* split, hipStream_t s);
+// Keys-only u32 sort of any length on device memory (chunks of < 2^30 keys
+// merged pairwise); tmp holds n keys, scratch large_sort_u32_scratch_bytes(n).
+size_t large_sort_u32_scratch_bytes(size_t n);
+int large_sort_u32_enqueue(uint32_t* keys, uint32_t* tmp, size_t n, char* scratch, hipStream_t s);
 
 }  // namespace pa
 

@@ -117,6 +117,83 @@ __global__ __launch_bounds__(kMergeBlock) void k_merge_tile(const K* __restrict_
   }
 }
 
+// Key-value form of k_merge_tile (ties take A, so merging an earlier run with
+// a later one keeps a stable sort stable): keys and payloads of the tile are
+// staged in LDS side by side, each thread records the staging slot of every
+// output it merges, and the write-out gathers key and payload through it.
+template <typename K>
+__global__ __launch_bounds__(kMergeBlock) void k_merge_tile_kv(const K* __restrict__ a, const uint32_t* __restrict__ va,
+                                                               size_t na, const K* __restrict__ b,
+                                                               const uint32_t* __restrict__ vb, size_t nb,
+                                                               const size_t* __restrict__ split, K* __restrict__ out,
+                                                               uint32_t* __restrict__ vout) {
+  __shared__ K s_in[kMergeTile];
+  __shared__ uint32_t s_v[kMergeTile];
+  __shared__ uint16_t s_src[kMergeTile];
+  static_assert(kMergeTile <= 65536, "16-bit staging slots");
+  const uint32_t tid = threadIdx.x;
+  const size_t tile = blockIdx.x;
+  const size_t d0 = tile * kMergeTile;
+  const size_t d1 = std::min(d0 + kMergeTile, na + nb);
+  const size_t i0 = split[tile], i1 = split[tile + 1];
+  const size_t j0 = d0 - i0, j1 = d1 - i1;
+  const uint32_t la = (uint32_t)(i1 - i0), lb = (uint32_t)(j1 - j0);
+  {
+    K v[kMergeItems];
+    uint32_t w[kMergeItems];
+#pragma unroll
+    for (int x = 0; x < kMergeItems; ++x) {
+      const uint32_t e = tid + x * kMergeBlock;
+      if (e < la) {
+        v[x] = __builtin_nontemporal_load(a + i0 + e);
+        w[x] = __builtin_nontemporal_load(va + i0 + e);
+      } else if (e < la + lb) {
+        v[x] = __builtin_nontemporal_load(b + j0 + (e - la));
+        w[x] = __builtin_nontemporal_load(vb + j0 + (e - la));
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < kMergeItems; ++x) {
+      const uint32_t e = tid + x * kMergeBlock;
+      if (e < la + lb) {
+        s_in[e] = v[x];
+        s_v[e] = w[x];
+      }
+    }
+  }
+  __syncthreads();
+  const K* sa = s_in;
+  const K* sb = s_in + la;
+  const uint32_t total = la + lb;
+  const uint32_t dt = std::min<uint32_t>(tid * kMergeItems, total);
+  uint32_t ia = (uint32_t)merge_path(sa, la, sb, lb, dt);
+  uint32_t ib = dt - ia;
+  uint32_t r[kMergeItems];
+#pragma unroll
+  for (int x = 0; x < kMergeItems; ++x) {
+    if (dt + x < total) {
+      const bool take_a = ib >= lb || (ia < la && sa[ia] <= sb[ib]);
+      r[x] = take_a ? ia : la + ib;
+      ia += take_a;
+      ib += !take_a;
+    }
+  }
+  __syncthreads();  // every merge read of s_in is done before s_src is written
+#pragma unroll
+  for (int x = 0; x < kMergeItems; ++x)
+    if (dt + x < total) s_src[dt + x] = (uint16_t)r[x];
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < kMergeItems; ++x) {
+    const uint32_t e = tid + x * kMergeBlock;
+    if (e < total) {
+      const uint32_t src = s_src[e];
+      __builtin_nontemporal_store(s_in[src], out + d0 + e);
+      __builtin_nontemporal_store(s_v[src], vout + d0 + e);
+    }
+  }
+}
+
 template <typename K>
 int merge_entry(const K* a, size_t na, const K* b, size_t nb, K* out, hipStream_t s, const char* who) {
   const size_t n = na + nb;
@@ -188,6 +265,31 @@ int merge_enqueue(const K* a, size_t na, const K* b, size_t nb, K* out, size_t* 
 }
 template int merge_enqueue<uint32_t>(const uint32_t*, size_t, const uint32_t*, size_t, uint32_t*, size_t*,
                                      hipStream_t);
+
+// Key-value runs (device memory) into device outputs, split table as above.
+template <typename K>
+int merge_kv_enqueue(const K* a, const uint32_t* va, size_t na, const K* b, const uint32_t* vb, size_t nb, K* out,
+                     uint32_t* vout, size_t* split, hipStream_t s) {
+  const size_t n = na + nb;
+  if (n == 0) return PA_OK;
+  const size_t ntiles = (n + kMergeTile - 1) / kMergeTile;
+  {
+    ProfScope ps("k_merge_partition", s);
+    hipLaunchKernelGGL(k_merge_partition<K>, dim3((unsigned)((ntiles + 1 + 255) / 256)), dim3(256), 0, s, a, na, b,
+                       nb, ntiles, split);
+  }
+  {
+    ProfScope ps("k_merge_tile_kv", s);
+    hipLaunchKernelGGL(k_merge_tile_kv<K>, dim3((unsigned)ntiles), dim3(kMergeBlock), 0, s, a, va, na, b, vb, nb,
+                       split, out, vout);
+  }
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+template int merge_kv_enqueue<uint32_t>(const uint32_t*, const uint32_t*, size_t, const uint32_t*, const uint32_t*,
+                                        size_t, uint32_t*, uint32_t*, size_t*, hipStream_t);
+template int merge_kv_enqueue<uint64_t>(const uint64_t*, const uint32_t*, size_t, const uint64_t*, const uint32_t*,
+                                        size_t, uint64_t*, uint32_t*, size_t*, hipStream_t);
 template int merge_enqueue<uint64_t>(const uint64_t*, size_t, const uint64_t*, size_t, uint64_t*, size_t*,
                                      hipStream_t);
 

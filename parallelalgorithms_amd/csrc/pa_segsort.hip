@@ -415,12 +415,22 @@ static uint64_t seg_queue_max(const SegVariant& v) {
 struct SegPlan {
   std::vector<uint64_t> small_ptr[kLdsClasses], queue_ptr, queue_tmp, large_in, large_tmp, tile_info;
   std::vector<uint32_t> small_len[kLdsClasses], queue_len, large_len, large_tiles, large_row;
+  // Segments of 2^30 keys or more (past a look-back word's 30-bit count):
+  // sorted one by one as the plain sort sorts such arrays, in chunks merged
+  // pairwise, after the other classes.
+  std::vector<uint64_t> huge_ptr, huge_tmp, huge_len;
   // The shape in force when the plan is made (a captured plan keeps it).
   const SegVariant* var = &seg_variant();
   const uint32_t tile = var->tile();
   const uint64_t queue_max = seg_queue_max(*var);
   void add(uint64_t ptr, uint64_t len, uint64_t tmp_ptr) {
     if (len <= 1) return;
+    if (len > kMaxSortN) {
+      huge_ptr.push_back(ptr);
+      huge_tmp.push_back(tmp_ptr);
+      huge_len.push_back(len);
+      return;
+    }
     for (int c = 0; c < kLdsClasses; ++c) {
       if (len <= kLdsLimit[c]) {
         small_ptr[c].push_back(ptr);
@@ -510,7 +520,7 @@ struct SegPlan {
 // table and `extra` bytes of ping-pong space for large segments.
 struct PlanLayout {
   size_t t_sp[kLdsClasses], t_sl[kLdsClasses], t_qp, t_qt, t_ql, t_li, t_lt, t_ll, t_lr, t_ti, table_bytes;
-  size_t o_tab, o_cnt, o_ctr, o_lb, o_work, total;
+  size_t o_tab, o_cnt, o_ctr, o_lb, o_huge, o_work, total;
 };
 
 static PlanLayout plan_layout(const SegPlan& P, size_t extra) {
@@ -537,6 +547,9 @@ static PlanLayout plan_layout(const SegPlan& P, size_t extra) {
   Y.o_cnt = L.take(nl * DR * 4);
   Y.o_ctr = L.take(64);
   Y.o_lb = L.take(DR * nt * 4);
+  size_t huge = 0;
+  for (uint64_t len : P.huge_len) huge = std::max(huge, large_sort_u32_scratch_bytes(len));
+  Y.o_huge = L.take(huge);
   Y.o_work = L.take(extra);
   Y.total = L.off;
   return Y;
@@ -553,6 +566,10 @@ static void assign_work(SegPlan& P, char* work) {
   for (size_t i = 0; i < P.large_tmp.size(); ++i) {
     P.large_tmp[i] = (uint64_t)(uintptr_t)work + off * 4;
     off += P.large_len[i];
+  }
+  for (size_t i = 0; i < P.huge_tmp.size(); ++i) {
+    P.huge_tmp[i] = (uint64_t)(uintptr_t)work + off * 4;
+    off += P.huge_len[i];
   }
 }
 
@@ -628,6 +645,9 @@ static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStre
       PA_TRY(V.pass(s, a));
     }
   }
+  for (size_t h = 0; h < P.huge_len.size(); ++h)
+    PA_TRY(large_sort_u32_enqueue(reinterpret_cast<uint32_t*>(P.huge_ptr[h]), reinterpret_cast<uint32_t*>(P.huge_tmp[h]),
+                                  P.huge_len[h], dev + Y.o_huge, s));
   PA_HIP(hipGetLastError());
   return PA_OK;
 }
@@ -658,10 +678,6 @@ static int build_ptr_plan(uint32_t* const* keys, const uint64_t* lens, size_t nu
   for (size_t i = 0; i < num_segments; ++i) {
     if (lens[i] == 0) continue;
     if (!keys[i]) return arg_error((std::string(who) + ": null segment pointer").c_str());
-    if (lens[i] >= (1ull << 30)) {
-      set_last_error(std::string(who) + ": a segment holds 2^30 or more keys");
-      return PA_ERR_TOO_LARGE;
-    }
     P->add((uint64_t)(uintptr_t)keys[i], lens[i], 0);
     if (lens[i] > kLdsMax) *large_total += lens[i];
   }
@@ -747,10 +763,6 @@ int pa_segmented_sort_u32(uint32_t* keys, uint32_t* tmp, const uint64_t* offsets
   uint64_t large_total = 0;
   for (size_t i = 0; i < num_segments; ++i) {
     const uint64_t len = offsets[i + 1] - offsets[i];
-    if (len >= (1ull << 30)) {
-      set_last_error("pa_segmented_sort_u32: a segment holds 2^30 or more keys");
-      return PA_ERR_TOO_LARGE;
-    }
     P.add((uint64_t)(uintptr_t)(keys + offsets[i]), len, tmp ? (uint64_t)(uintptr_t)(tmp + offsets[i]) : 0);
     if (len > kLdsMax) large_total += len;
   }
@@ -772,10 +784,6 @@ int segmented_lens_scratch_bytes(const uint64_t* lens, size_t num, size_t* bytes
   SegPlan P;
   uint64_t large_total = 0;
   for (size_t i = 0; i < num; ++i) {
-    if (lens[i] >= (1ull << 30)) {
-      set_last_error("segmented sort: a segment holds 2^30 or more keys");
-      return PA_ERR_TOO_LARGE;
-    }
     P.add(1, lens[i], 0);  // addresses do not enter the layout
     if (lens[i] > kLdsMax) large_total += lens[i];
   }
@@ -803,10 +811,6 @@ int pa_autotune_segmented_u32(const uint64_t* lens, size_t num_segments, pa_stre
   if (!lens || num_segments == 0) return arg_error("pa_autotune_segmented_u32: empty length list");
   uint64_t total = 0;
   for (size_t i = 0; i < num_segments; ++i) {
-    if (lens[i] >= (1ull << 30)) {
-      set_last_error("pa_autotune_segmented_u32: a segment holds 2^30 or more keys");
-      return PA_ERR_TOO_LARGE;
-    }
     total += lens[i];
   }
   if (total == 0) return arg_error("pa_autotune_segmented_u32: no keys");

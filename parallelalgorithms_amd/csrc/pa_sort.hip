@@ -442,17 +442,20 @@ inline size_t large_chunk_keys(size_t n) {
   return (n + chunks - 1) / chunks;
 }
 
-template <typename K>
+template <typename K, bool KV>
 size_t large_sort_scratch_bytes(size_t n) {
   const size_t c = large_chunk_keys(n);
-  return sort_scratch_bytes<K, false>(pick_variant<K, false>(c), c) + 256 + merge_split_bytes(n);
+  return sort_scratch_bytes<K, KV>(pick_variant<K, KV>(c), c) + 256 + merge_split_bytes(n);
 }
 
-template <typename K>
-int enqueue_large_sort(K* keys, K* tmp, size_t n, char* scratch, hipStream_t s, uint32_t xform) {
+// Key-value sorts take the same route: the chunks' stable sorts, then stable
+// merges (ties take the earlier run), so equal keys keep their input order.
+template <typename K, bool KV>
+int enqueue_large_sort(K* keys, uint32_t* vals, K* tmp, uint32_t* vtmp, size_t n, char* scratch, hipStream_t s,
+                       uint32_t xform) {
   const size_t csize = large_chunk_keys(n);
-  const PassVariant& var = pick_variant<K, false>(csize);
-  const size_t sort_bytes = sort_scratch_bytes<K, false>(var, csize);
+  const PassVariant& var = pick_variant<K, KV>(csize);
+  const size_t sort_bytes = sort_scratch_bytes<K, KV>(var, csize);
   size_t* split = reinterpret_cast<size_t*>(scratch + ((sort_bytes + 255) & ~size_t(255)));
   const unsigned xgrid = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
   if (xform) hipLaunchKernelGGL(k_key_xform<K>, dim3(xgrid), dim3(256), 0, s, keys, n, xform, 0u);
@@ -460,31 +463,48 @@ int enqueue_large_sort(K* keys, K* tmp, size_t n, char* scratch, hipStream_t s, 
   for (size_t off = 0; off < n; off += csize) {
     const size_t len = std::min(csize, n - off);
     // Each chunk's result is in keys (an odd pass count copies it back).
-    PA_TRY((enqueue_sort<K, false>(var, keys + off, nullptr, tmp + off, nullptr, len, scratch, s, dev::kXformNone)));
+    PA_TRY((enqueue_sort<K, KV>(var, keys + off, KV ? vals + off : nullptr, tmp + off, KV ? vtmp + off : nullptr, len,
+                                scratch, s, dev::kXformNone)));
     runs.emplace_back(off, len);
   }
   K* x = keys;
   K* y = tmp;
+  uint32_t* vx = vals;
+  uint32_t* vy = vtmp;
   while (runs.size() > 1) {
     std::vector<std::pair<size_t, size_t>> next;
     for (size_t i = 0; i + 1 < runs.size(); i += 2) {
       const auto [o1, l1] = runs[i];
       const auto [o2, l2] = runs[i + 1];
-      PA_TRY(merge_enqueue<K>(x + o1, l1, x + o2, l2, y + o1, split, s));
+      if constexpr (KV)
+        PA_TRY(merge_kv_enqueue<K>(x + o1, vx + o1, l1, x + o2, vx + o2, l2, y + o1, vy + o1, split, s));
+      else
+        PA_TRY(merge_enqueue<K>(x + o1, l1, x + o2, l2, y + o1, split, s));
       next.emplace_back(o1, l1 + l2);
     }
     if (runs.size() % 2) {
       const auto [o, l] = runs.back();
       PA_HIP(hipMemcpyAsync(y + o, x + o, l * sizeof(K), hipMemcpyDeviceToDevice, s));
+      if constexpr (KV) PA_HIP(hipMemcpyAsync(vy + o, vx + o, l * 4, hipMemcpyDeviceToDevice, s));
       next.push_back(runs.back());
     }
     runs.swap(next);
     std::swap(x, y);
+    std::swap(vx, vy);
   }
-  if (x != keys) PA_HIP(hipMemcpyAsync(keys, x, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+  if (x != keys) {
+    PA_HIP(hipMemcpyAsync(keys, x, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    if constexpr (KV) PA_HIP(hipMemcpyAsync(vals, vx, n * 4, hipMemcpyDeviceToDevice, s));
+  }
   if (xform) hipLaunchKernelGGL(k_key_xform<K>, dim3(xgrid), dim3(256), 0, s, keys, n, xform, 1u);
   PA_HIP(hipGetLastError());
   return PA_OK;
+}
+
+size_t large_sort_u32_scratch_bytes(size_t n) { return large_sort_scratch_bytes<uint32_t, false>(n); }
+
+int large_sort_u32_enqueue(uint32_t* keys, uint32_t* tmp, size_t n, char* scratch, hipStream_t s) {
+  return enqueue_large_sort<uint32_t, false>(keys, nullptr, tmp, nullptr, n, scratch, s, dev::kXformNone);
 }
 
 // ---------------------------------------------------------------------------
@@ -533,7 +553,7 @@ SortEntryLayout sort_entry_layout(size_t n, bool dk, bool dkt, bool dv, bool dvt
   const PassVariant& var = pick_variant<K, KV>(large ? large_chunk_keys(n) : n);
   SortEntryLayout Y;
   Layout L;
-  Y.o_sort = L.take(large ? large_sort_scratch_bytes<K>(n) : sort_scratch_bytes<K, KV>(var, n));
+  Y.o_sort = L.take(large ? large_sort_scratch_bytes<K, KV>(n) : sort_scratch_bytes<K, KV>(var, n));
   Y.o_k = dk ? 0 : L.take(n * sizeof(K));
   Y.o_kt = dkt ? 0 : L.take(n * sizeof(K));
   Y.o_v = dv ? 0 : L.take(n * 4);
@@ -555,10 +575,6 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   if (n == 0) return PA_OK;
   if (!keys || (KV && !vals)) return arg_error(who);
   const bool large = n > kMaxSortN;
-  if (KV && large) {
-    set_last_error(std::string(who) + ": key-value sorts take n < 2^30");
-    return PA_ERR_TOO_LARGE;
-  }
   StreamCtx& ctx = stream_ctx(s);
   std::lock_guard<std::mutex> lk(ctx.mu);
   const bool dk = is_device_ptr(keys);
@@ -582,10 +598,12 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
       // One workgroup sorts the whole array in LDS: one launch.
       PA_TRY(small_sort_enqueue_u32(reinterpret_cast<uint32_t*>(dkeys), n, s));
     } else if (large) {
-      PA_TRY(enqueue_large_sort<K>(dkeys, dktmp, n, b + o_sort, s, xform));
+      PA_TRY((enqueue_large_sort<K, false>(dkeys, nullptr, dktmp, nullptr, n, b + o_sort, s, xform)));
     } else {
       PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
     }
+  } else if (large) {
+    PA_TRY((enqueue_large_sort<K, true>(dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
   } else {
     PA_TRY((enqueue_sort<K, KV>(var, dkeys, dvals, dktmp, dvtmp, n, b + o_sort, s, xform)));
   }

@@ -1,4 +1,5 @@
-"""Keys-only sorts past the 30-bit count of a look-back status word.
+"""Sorts past the 30-bit count of a look-back status word: keys-only and
+key-value sorts, and a segment of a segmented sort.
 
 Arrays of 2^30 keys or more are sorted in equal chunks of at most 2^30 - 1
 keys and the sorted runs merged pairwise (the reference's radix-leaf merge sort shape,
@@ -84,3 +85,51 @@ def test_sort_i64_two_to_the_30_plus(pa):
     print(f"[large] i64 n={n}", flush=True)
     pa.sort_i64(d)
     assert torch.equal(d, ref)
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_sort_kv_two_to_the_30_plus_is_stable(pa, wide):
+    """Key-value sorts past 2^30 keys: chunks sorted stably and merged with
+    ties taken from the earlier run.  Keys drawn from 1,000,003 values (about
+    a thousand copies each) and payload = input index: the payload must be
+    torch's stable argsort, which pins both the order and the stability."""
+    import torch
+    import parallelalgorithms_amd as P
+    n = (1 << 30) + 4099
+    if wide:
+        k = torch.empty(n, dtype=torch.int64, device="cuda")
+        P.call("pa_fill_random_u64", k.data_ptr(), n, 105, None)
+        k = (k & 0x7FFFFFFFFFFFFFFF) % 1_000_003 * 0x100000001
+        key_view = k
+    else:
+        r = _random_i32(n, 106).to(torch.int64) & 0xFFFFFFFF
+        key_view = r % 1_000_003 * 4099
+        k = key_view.to(torch.int32)
+        del r
+    v = torch.arange(n, dtype=torch.int32, device="cuda")
+    ref = torch.sort(key_view, stable=True)
+    print(f"[large] kv {'u64' if wide else 'u32'} n={n}", flush=True)
+    (P.sort_kv_u64_u32 if wide else P.sort_kv_u32_u32)(k, v)
+    got_k = k if wide else k.to(torch.int64) & 0xFFFFFFFF
+    assert torch.equal(got_k, ref.values)
+    assert torch.equal(v.to(torch.int64), ref.indices)
+
+
+def test_segmented_with_a_segment_past_two_to_the_30(pa):
+    """A segment of 2^30 + 13 keys between small, queue-class and onesweep-class
+    segments: the large one is sorted like a plain array of that size."""
+    import numpy as np
+    import torch
+    lens = [1000, (1 << 30) + 13, 70_000, 5, 300_000, 1]
+    offs = np.zeros(len(lens) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    total = int(offs[-1])
+    d = _random_i32(total, 107)
+    src = d.to(torch.int64) & 0xFFFFFFFF
+    print(f"[large] segmented with one segment of {lens[1]} keys", flush=True)
+    pa.segmented_sort_u32(d, None, offs)
+    got = d.to(torch.int64) & 0xFFFFFFFF
+    del d
+    for i in range(len(lens)):
+        a, b = int(offs[i]), int(offs[i + 1])
+        assert torch.equal(got[a:b], src[a:b].sort().values), i
