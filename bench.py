@@ -294,6 +294,14 @@ def bench_configs(pa, torch, n=100_000_000, reps=5):
         del src, ref, hist_ref
     timed("partition_u32_median", lambda: pa.partition_u32(w32, n // 2, r32),
           fresh=lambda: w32.copy_(torch.from_numpy(pa.gen_u32(1234, n).view(np.int32)).to(dev)))
+    # The radix-leaf merge sort (pa_merge_sort_u32) at the default leaf of
+    # n / 8 keys: the leaves' sort (36 B/key) and three merge rounds (8 B/key
+    # each) = 60 B/key.
+    s32 = torch.from_numpy(pa.gen_u32(1234, n).view(np.int32)).to(dev)
+    ms_ref = u(s32).sort().values
+    timed("merge_sort_u32_leaves_n_8", lambda: pa.merge_sort_u32(w32, 0, t32), fresh=lambda: w32.copy_(s32),
+          bytes_per_unit=60, check=lambda: torch.equal(u(w32), ms_ref))
+    del s32, ms_ref
     # Config 3: float keys with specials, merge of two sorted 100M runs, and
     # the 200M u64 key-value sort (below).
     bits = workloads.f32_with_specials(n, 1236)

@@ -6,7 +6,10 @@
 // (ParallelMergeSort.h:53-638).  The reference's sorts recurse over TBB tasks
 // and merge with merge_parallel_L5; for scalar keys every one of them yields
 // the unique ascending order of its range, which is what one GPU radix sort of
-// the range produces (pa_sort_keys.h).  Element types: 32/64-bit integers,
+// the range produces (pa_sort_keys.h).  The radix-leaf family
+// (parallel_merge_sort_hybrid_radix and its _inner and _single_buffer forms)
+// keeps the reference's shape instead: leaves radix-sorted, merged upward
+// (pa_merge_sort_u32).  Element types: 32/64-bit integers,
 // float, double.  Thresholds, stability flags and memory-fraction arguments
 // steer the reference's CPU recursion and have no effect on that result.
 //
@@ -21,6 +24,9 @@
 #define PA_DROPIN_PARALLELMERGESORT_H
 
 #include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <thread>
 
 #include "ParallelMerge.h"
 #include "RadixSortLsdParallel.h"
@@ -31,6 +37,40 @@ namespace ParallelAlgorithms {
 
 namespace pa_msort_detail {
 using pa_sort_keys_detail::sort_inclusive;
+// The radix-leaf merge sort on the GPU (pa_merge_sort_u32): leaves of `leaf`
+// keys radix-sorted, then merged upward.  src[l, r] inclusive; the result in
+// dst[l, r] (src kept) for src_to_dst, else in src[l, r] with dst[l, r] as
+// the ping-pong buffer, as the reference uses it.
+inline void merge_sort_radix_leaves(unsigned* src, size_t l, size_t r, unsigned* dst, bool src_to_dst, size_t leaf,
+                                    const char* who) {
+  if (r < l) return;
+  const size_t n = r - l + 1;
+  if (n < pa_host_small::kSortBelow && pa_host_small::on_host(src) && (!src_to_dst || pa_host_small::on_host(dst))) {
+    unsigned* t = src;
+    if (src_to_dst) {
+      for (size_t i = l; i <= r; ++i) dst[i] = src[i];
+      t = dst;
+    }
+    pa_host_small::insertion_sort(t + l, n);
+    return;
+  }
+  unsigned* target = src;
+  unsigned* work = dst ? dst + l : nullptr;
+  if (src_to_dst) {
+    if (!dst) throw std::invalid_argument(std::string(who) + ": dst is null");
+    pa_sort_keys_detail::check(pa_copy(dst + l, src + l, n * sizeof(unsigned), pa_thread_stream()), who);
+    target = dst;
+    work = nullptr;  // src keeps its input
+  }
+  pa_sort_keys_detail::check(pa_merge_sort_u32(target + l, work, n, leaf == 0 ? 1 : leaf, pa_thread_stream()), who);
+  pa_sort_keys_detail::check(pa_synchronize(pa_thread_stream()), who);
+}
+// The reference's leaf size for n keys: parallelThreshold, raised to
+// n / hardware threads when that is larger (ParallelMergeSort.h:288-292).
+inline size_t leaf_keys(size_t n, size_t threshold) {
+  const size_t p = std::thread::hardware_concurrency();
+  return (p > 0 && threshold * p < n) ? n / p : threshold;
+}
 // The int-bounded forms allow r = -1 for an empty range (ParallelMergeSort.h:74-77).
 template <class T>
 inline void sort_int_bounds(T* src, int l, int r, T* dst, bool src_to_dst, const char* who) {
@@ -116,25 +156,30 @@ inline void parallel_merge_sort_hybrid(_Type* src, size_t l, size_t r, _Type* ds
   pa_sort_keys_detail::sort_inclusive(src, l, r, dst, srcToDst, "parallel_merge_sort_hybrid");
 }
 
-// ParallelMergeSort.h:255-283 (radix-sorted leaves, merged upward).
+// ParallelMergeSort.h:255-283: radix-sorted leaves of parallelThreshold keys,
+// merged upward -- on the GPU, the leaves as one segmented batch and the
+// merges as merge-path rounds (pa_merge_sort_u32).
 inline void parallel_merge_sort_hybrid_radix_inner(unsigned* src, size_t l, size_t r, unsigned* dst,
                                                    bool srcToDst = true, size_t parallelThreshold = 32 * 1024) {
-  (void)parallelThreshold;
-  pa_sort_keys_detail::sort_inclusive(src, l, r, dst, srcToDst, "parallel_merge_sort_hybrid_radix_inner");
+  pa_msort_detail::merge_sort_radix_leaves(src, l, r, dst, srcToDst, parallelThreshold,
+                                           "parallel_merge_sort_hybrid_radix_inner");
 }
 
-// ParallelMergeSort.h:285-295.
+// ParallelMergeSort.h:285-295: the same, with the leaf size raised to
+// n / hardware threads.
 inline void parallel_merge_sort_hybrid_radix(unsigned* src, size_t l, size_t r, unsigned* dst, bool srcToDst = true,
                                              size_t parallelThreshold = 24 * 1024) {
-  (void)parallelThreshold;
-  pa_sort_keys_detail::sort_inclusive(src, l, r, dst, srcToDst, "parallel_merge_sort_hybrid_radix");
+  const size_t n = r >= l ? r - l + 1 : 0;
+  pa_msort_detail::merge_sort_radix_leaves(src, l, r, dst, srcToDst, pa_msort_detail::leaf_keys(n, parallelThreshold),
+                                           "parallel_merge_sort_hybrid_radix");
 }
 
 // ParallelMergeSort.h:297-307.
 inline void parallel_merge_sort_hybrid_radix_single_buffer(unsigned* src, size_t l, size_t r, unsigned* dst,
                                                            bool srcToDst = true, size_t parallelThreshold = 24 * 1024) {
-  (void)parallelThreshold;
-  pa_sort_keys_detail::sort_inclusive(src, l, r, dst, srcToDst, "parallel_merge_sort_hybrid_radix_single_buffer");
+  const size_t n = r >= l ? r - l + 1 : 0;
+  pa_msort_detail::merge_sort_radix_leaves(src, l, r, dst, srcToDst, pa_msort_detail::leaf_keys(n, parallelThreshold),
+                                           "parallel_merge_sort_hybrid_radix_single_buffer");
 }
 
 // ParallelMergeSort.h:310-325 (serial in the reference).

@@ -97,6 +97,18 @@ int pa_sort_u32_noalloc(uint32_t* keys, uint32_t* tmp, size_t n, pa_stream_t str
  * reference. */
 int pa_sort_two_phase_u32(uint32_t* a, uint32_t* b, size_t n, pa_stream_t stream);
 
+/* Merge sort with radix-sorted leaves: replaces parallel_merge_sort_hybrid_radix
+ * (ParallelMergeSort.h:255-295: leaves of parallelThreshold keys sorted by the
+ * LSD radix sort, then merged upward with merge_parallel_L5).  The leaves
+ * keys[i * leaf_keys, (i + 1) * leaf_keys) are sorted (u32: as one segmented
+ * batch; u64: one sort each) and merged pairwise, round after round, by the
+ * merge-path kernels, ping-ponging with tmp (device or host memory, n keys;
+ * NULL: library scratch).  leaf_keys = 0 picks max(65,536, n / 8).  The
+ * result is the sorted array, in keys; one radix sort (pa_sort_u32) reaches
+ * it with fewer passes over memory. */
+int pa_merge_sort_u32(uint32_t* keys, uint32_t* tmp, size_t n, size_t leaf_keys, pa_stream_t stream);
+int pa_merge_sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, size_t leaf_keys, pa_stream_t stream);
+
 /* Pass shape per size class (n <= 2^16, <= 2^20, <= 2^24, larger).  The
  * reference fixes its work split by constants (SortRadixPar's quantum,
  * RadixSortLsdParallel.h:374-379); here each class has a built-in tile shape,

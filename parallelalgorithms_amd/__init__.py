@@ -46,6 +46,8 @@ _SIGS = {
     "pa_tune_cache_save": (_i, [ctypes.c_char_p]),
     "pa_tune_cache_load": (_i, [ctypes.c_char_p]),
     "pa_tune_device_key": (ctypes.c_char_p, []),
+    "pa_merge_sort_u32": (_i, [_vp, _vp, _sz, _sz, _vp]),
+    "pa_merge_sort_u64": (_i, [_vp, _vp, _sz, _sz, _vp]),
     "pa_sort_u64": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_f32": (_i, [_vp, _vp, _sz, _vp]),
     "pa_sort_f64": (_i, [_vp, _vp, _sz, _vp]),
@@ -281,6 +283,19 @@ def sort_u64(keys, tmp=None, stream=None):
     _check(keys, 8, "keys")
     _check(tmp, 8, "tmp", numel(keys))
     call("pa_sort_u64", _ptr(keys), _ptr(tmp), numel(keys), _stream(stream))
+
+
+def merge_sort_u32(keys, leaf_keys=0, tmp=None, stream=None):
+    """Radix-sorted leaves merged upward (pa_merge_sort_u32)."""
+    _check(keys, 4, "keys")
+    _check(tmp, 4, "tmp", numel(keys))
+    call("pa_merge_sort_u32", _ptr(keys), _ptr(tmp), numel(keys), leaf_keys, _stream(stream))
+
+
+def merge_sort_u64(keys, leaf_keys=0, tmp=None, stream=None):
+    _check(keys, 8, "keys")
+    _check(tmp, 8, "tmp", numel(keys))
+    call("pa_merge_sort_u64", _ptr(keys), _ptr(tmp), numel(keys), leaf_keys, _stream(stream))
 
 
 def sort_f32(keys, tmp=None, stream=None):

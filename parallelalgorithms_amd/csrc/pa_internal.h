@@ -179,6 +179,11 @@ int merge_kv_enqueue(const K* a, const uint32_t* va, size_t na, const K* b, cons
 // merged pairwise); tmp holds n keys, scratch large_sort_u32_scratch_bytes(n).
 size_t large_sort_u32_scratch_bytes(size_t n);
 int large_sort_u32_enqueue(uint32_t* keys, uint32_t* tmp, size_t n, char* scratch, hipStream_t s);
+// Merge sort leaves keys[i * leaf, (i + 1) * leaf) sorted as one segmented
+// batch (pa_segsort.hip), tmp as ping-pong space; ctx.mu must be held.
+size_t leaves_scratch_bytes_u32(size_t n, size_t leaf);
+int leaves_enqueue_u32(uint32_t* keys, uint32_t* tmp, size_t n, size_t leaf, char* scratch, hipStream_t s,
+                       StreamCtx& ctx);
 
 }  // namespace pa
 

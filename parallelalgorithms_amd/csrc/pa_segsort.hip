@@ -655,13 +655,9 @@ static int launch_plan(const SegPlan& P, const PlanLayout& Y, char* dev, hipStre
 // One-shot run on the stream's scratch: tables go up through pinned memory.
 // `extra` bytes of ping-pong space are reserved when no caller tmp exists.
 // ctx.mu must be held.
-static int run_plan_alloc(SegPlan& P, size_t extra, bool fill_tmp, hipStream_t s, StreamCtx& ctx) {
-  if (P.tile_info.size() >= (1ull << 32)) return arg_error("segmented sort: too many tiles");
-  const PlanLayout Y = plan_layout(P, extra);
-  void* base;
-  PA_TRY(ensure_scratch(ctx, Y.total, &base));
-  char* b = (char*)base;
-  if (fill_tmp) assign_work(P, b + Y.o_work);
+// Tables up through pinned memory, then the plan's kernels, on device memory
+// b laid out by Y.  ctx.mu must be held.
+static int run_plan_at(const SegPlan& P, const PlanLayout& Y, char* b, hipStream_t s, StreamCtx& ctx) {
   void* pinned;
   int slot;
   PA_TRY(acquire_pinned(ctx, Y.table_bytes, &pinned, &slot));
@@ -669,6 +665,38 @@ static int run_plan_alloc(SegPlan& P, size_t extra, bool fill_tmp, hipStream_t s
   PA_HIP(hipMemcpyAsync(b + Y.o_tab, pinned, Y.table_bytes, hipMemcpyHostToDevice, s));
   PA_TRY(release_pinned(ctx, slot));
   return launch_plan(P, Y, b, s);
+}
+
+static int run_plan_alloc(SegPlan& P, size_t extra, bool fill_tmp, hipStream_t s, StreamCtx& ctx) {
+  if (P.tile_info.size() >= (1ull << 32)) return arg_error("segmented sort: too many tiles");
+  const PlanLayout Y = plan_layout(P, extra);
+  void* base;
+  PA_TRY(ensure_scratch(ctx, Y.total, &base));
+  char* b = (char*)base;
+  if (fill_tmp) assign_work(P, b + Y.o_work);
+  return run_plan_at(P, Y, b, s, ctx);
+}
+
+// The leaves of a merge sort as one segmented batch: keys[i * leaf, (i + 1) *
+// leaf) for every i (the last one shorter), each with the same range of tmp
+// as its ping-pong space.
+static void leaves_plan(uintptr_t keys, uintptr_t tmp, size_t n, size_t leaf, SegPlan* P) {
+  for (size_t o = 0; o < n; o += leaf) P->add(keys + o * 4, std::min(leaf, n - o), tmp + o * 4);
+  P->finalize();
+}
+
+size_t leaves_scratch_bytes_u32(size_t n, size_t leaf) {
+  SegPlan P;
+  leaves_plan(0, 0, n, leaf, &P);
+  return plan_layout(P, 0).total;
+}
+
+int leaves_enqueue_u32(uint32_t* keys, uint32_t* tmp, size_t n, size_t leaf, char* scratch, hipStream_t s,
+                       StreamCtx& ctx) {
+  SegPlan P;
+  leaves_plan((uintptr_t)keys, (uintptr_t)tmp, n, leaf, &P);
+  if (P.tile_info.size() >= (1ull << 32)) return arg_error("merge sort: too many tiles");
+  return run_plan_at(P, plan_layout(P, 0), scratch, s, ctx);
 }
 
 // Builds the plan of a pointer-table call; returns PA_OK or an error status.

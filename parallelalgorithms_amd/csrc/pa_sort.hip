@@ -448,25 +448,13 @@ size_t large_sort_scratch_bytes(size_t n) {
   return sort_scratch_bytes<K, KV>(pick_variant<K, KV>(c), c) + 256 + merge_split_bytes(n);
 }
 
-// Key-value sorts take the same route: the chunks' stable sorts, then stable
-// merges (ties take the earlier run), so equal keys keep their input order.
+// Merges adjacent sorted runs of keys (covering [0, n) in order) pairwise,
+// round after round, ping-ponging with tmp, until one run is left; the result
+// ends in keys.  An odd run out is copied through its round.  Ties take the
+// earlier run, so stable runs merge stably.  split: merge_split_bytes(n).
 template <typename K, bool KV>
-int enqueue_large_sort(K* keys, uint32_t* vals, K* tmp, uint32_t* vtmp, size_t n, char* scratch, hipStream_t s,
-                       uint32_t xform) {
-  const size_t csize = large_chunk_keys(n);
-  const PassVariant& var = pick_variant<K, KV>(csize);
-  const size_t sort_bytes = sort_scratch_bytes<K, KV>(var, csize);
-  size_t* split = reinterpret_cast<size_t*>(scratch + ((sort_bytes + 255) & ~size_t(255)));
-  const unsigned xgrid = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
-  if (xform) hipLaunchKernelGGL(k_key_xform<K>, dim3(xgrid), dim3(256), 0, s, keys, n, xform, 0u);
-  std::vector<std::pair<size_t, size_t>> runs;  // (first key, length)
-  for (size_t off = 0; off < n; off += csize) {
-    const size_t len = std::min(csize, n - off);
-    // Each chunk's result is in keys (an odd pass count copies it back).
-    PA_TRY((enqueue_sort<K, KV>(var, keys + off, KV ? vals + off : nullptr, tmp + off, KV ? vtmp + off : nullptr, len,
-                                scratch, s, dev::kXformNone)));
-    runs.emplace_back(off, len);
-  }
+int merge_rounds(K* keys, uint32_t* vals, K* tmp, uint32_t* vtmp, size_t n,
+                 std::vector<std::pair<size_t, size_t>> runs, size_t* split, hipStream_t s) {
   K* x = keys;
   K* y = tmp;
   uint32_t* vx = vals;
@@ -496,6 +484,29 @@ int enqueue_large_sort(K* keys, uint32_t* vals, K* tmp, uint32_t* vtmp, size_t n
     PA_HIP(hipMemcpyAsync(keys, x, n * sizeof(K), hipMemcpyDeviceToDevice, s));
     if constexpr (KV) PA_HIP(hipMemcpyAsync(vals, vx, n * 4, hipMemcpyDeviceToDevice, s));
   }
+  return PA_OK;
+}
+
+// Key-value sorts take the same route: the chunks' stable sorts, then stable
+// merges (ties take the earlier run), so equal keys keep their input order.
+template <typename K, bool KV>
+int enqueue_large_sort(K* keys, uint32_t* vals, K* tmp, uint32_t* vtmp, size_t n, char* scratch, hipStream_t s,
+                       uint32_t xform) {
+  const size_t csize = large_chunk_keys(n);
+  const PassVariant& var = pick_variant<K, KV>(csize);
+  const size_t sort_bytes = sort_scratch_bytes<K, KV>(var, csize);
+  size_t* split = reinterpret_cast<size_t*>(scratch + ((sort_bytes + 255) & ~size_t(255)));
+  const unsigned xgrid = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+  if (xform) hipLaunchKernelGGL(k_key_xform<K>, dim3(xgrid), dim3(256), 0, s, keys, n, xform, 0u);
+  std::vector<std::pair<size_t, size_t>> runs;  // (first key, length)
+  for (size_t off = 0; off < n; off += csize) {
+    const size_t len = std::min(csize, n - off);
+    // Each chunk's result is in keys (an odd pass count copies it back).
+    PA_TRY((enqueue_sort<K, KV>(var, keys + off, KV ? vals + off : nullptr, tmp + off, KV ? vtmp + off : nullptr, len,
+                                scratch, s, dev::kXformNone)));
+    runs.emplace_back(off, len);
+  }
+  PA_TRY((merge_rounds<K, KV>(keys, vals, tmp, vtmp, n, runs, split, s)));
   if (xform) hipLaunchKernelGGL(k_key_xform<K>, dim3(xgrid), dim3(256), 0, s, keys, n, xform, 1u);
   PA_HIP(hipGetLastError());
   return PA_OK;
@@ -609,6 +620,56 @@ int sort_entry(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, hipSt
   }
   if (!dk) PA_TRY(stage_to_host(ctx, keys, dkeys, n * sizeof(K)));
   if (KV && !dv) PA_TRY(stage_to_host(ctx, vals, dvals, n * 4));
+  return PA_OK;
+}
+
+// Merge sort: the leaves keys[i * leaf, (i + 1) * leaf) radix-sorted (u32:
+// as one segmented batch; u64: one onesweep sort each), then merged upward
+// pairwise with merge-path merges -- the shape of the reference's
+// parallel_merge_sort_hybrid_radix (ParallelMergeSort.h:255-295: radix-sorted
+// leaves, merge_parallel_L5 on the way up).  Host arrays are staged.
+template <typename K>
+int merge_sort_entry(K* keys, K* tmp, size_t n, size_t leaf, hipStream_t s, const char* who) {
+  if (n == 0) return PA_OK;
+  if (!keys) return arg_error(who);
+  if (leaf == 0) leaf = std::max<size_t>(size_t(1) << 16, (n + 7) / 8);
+  leaf = std::min(leaf, n);
+  if (sizeof(K) == 8) leaf = std::min(leaf, kMaxSortN);
+  StreamCtx& ctx = stream_ctx(s);
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  const bool dk = is_device_ptr(keys);
+  const bool dkt = tmp && is_device_ptr(tmp);
+  size_t leaf_bytes;
+  if constexpr (sizeof(K) == 4)
+    leaf_bytes = leaves_scratch_bytes_u32(n, leaf);
+  else
+    leaf_bytes = sort_scratch_bytes<K, false>(pick_variant<K, false>(leaf), leaf);
+  Layout L;
+  const size_t o_leaf = L.take(leaf_bytes);
+  const size_t o_split = L.take(merge_split_bytes(n));
+  const size_t o_k = dk ? 0 : L.take(n * sizeof(K));
+  const size_t o_kt = dkt ? 0 : L.take(n * sizeof(K));
+  void* base = nullptr;
+  PA_TRY(ensure_scratch(ctx, L.off, &base));
+  char* b = static_cast<char*>(base);
+  K* dkeys = dk ? keys : reinterpret_cast<K*>(b + o_k);
+  K* dtmp = dkt ? tmp : reinterpret_cast<K*>(b + o_kt);
+  if (!dk) PA_TRY(stage_to_device(ctx, dkeys, keys, n * sizeof(K)));
+  std::vector<std::pair<size_t, size_t>> runs;
+  for (size_t o = 0; o < n; o += leaf) runs.emplace_back(o, std::min(leaf, n - o));
+  if constexpr (sizeof(K) == 4) {
+    PA_TRY(leaves_enqueue_u32(reinterpret_cast<uint32_t*>(dkeys), reinterpret_cast<uint32_t*>(dtmp), n, leaf,
+                              b + o_leaf, s, ctx));
+  } else {
+    for (const auto& r : runs) {
+      const PassVariant& var = pick_variant<K, false>(r.second);
+      PA_TRY((enqueue_sort<K, false>(var, dkeys + r.first, nullptr, dtmp + r.first, nullptr, r.second, b + o_leaf, s,
+                                     dev::kXformNone)));
+    }
+  }
+  PA_TRY((merge_rounds<K, false>(dkeys, nullptr, dtmp, nullptr, n, runs, reinterpret_cast<size_t*>(b + o_split), s)));
+  PA_HIP(hipGetLastError());
+  if (!dk) PA_TRY(stage_to_host(ctx, keys, dkeys, n * sizeof(K)));
   return PA_OK;
 }
 
@@ -825,6 +886,14 @@ int pa_autotune_reset(void) {
     for (int c = 0; c < kClasses; ++c) g_tuned[t][c].store(-1);
   seg_tune_reset();
   return PA_OK;
+}
+
+int pa_merge_sort_u32(uint32_t* keys, uint32_t* tmp, size_t n, size_t leaf_keys, pa_stream_t stream) {
+  return merge_sort_entry<uint32_t>(keys, tmp, n, leaf_keys, (hipStream_t)stream, "pa_merge_sort_u32");
+}
+
+int pa_merge_sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, size_t leaf_keys, pa_stream_t stream) {
+  return merge_sort_entry<uint64_t>(keys, tmp, n, leaf_keys, (hipStream_t)stream, "pa_merge_sort_u64");
 }
 
 int pa_sort_two_phase_u32(uint32_t* a, uint32_t* b, size_t n, pa_stream_t stream) {

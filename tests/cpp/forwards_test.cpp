@@ -10,7 +10,9 @@
 //   2. Host cut-offs: sorts below 100 keys and merges up to 32,768 keys of
 //      host arrays stay on the calling thread, in the GPU path's order (float
 //      total order with NaNs and signed zeros included).
-//   3. merge_parallel_L5 and SortRadixPar called from 8 host threads at once
+//   3. The radix-leaf merge sorts (parallel_merge_sort_hybrid_radix and its
+//      _inner and _single_buffer forms) in both result directions.
+//   4. merge_parallel_L5 and SortRadixPar called from 8 host threads at once
 //      (the reference calls merge_parallel_L5 from many TBB tasks,
 //      ParallelMergeSort.h:117-118), GPU-sized and host-sized, every result
 //      checked against std::merge / std::sort.
@@ -27,6 +29,7 @@
 
 #include "HistogramParallel.h"
 #include "ParallelMerge.h"
+#include "ParallelMergeSort.h"
 #include "RadixSortLsdParallel.h"
 #include "SortParallel.h"
 
@@ -161,6 +164,31 @@ static void forwards() {
   }
 }
 
+// The radix-leaf merge sorts (ParallelMergeSort.h:255-307) with both result
+// directions and leaf sizes from every segmented class.
+static void merge_sorts() {
+  const size_t n = 2000003;
+  const std::vector<unsigned> u = rand_u32(n, 5151);
+  std::vector<unsigned> sorted(u);
+  std::sort(sorted.begin(), sorted.end());
+  for (size_t thr : {size_t(4096), size_t(70000), size_t(24 * 1024), size_t(1) << 20}) {
+    std::vector<unsigned> a(u), d(n, 0);
+    ParallelAlgorithms::parallel_merge_sort_hybrid_radix_inner(a.data(), 0, n - 1, d.data(), false, thr);
+    EXPECT(a == sorted, "parallel_merge_sort_hybrid_radix_inner(srcToDst=false, thr=%zu)", thr);
+    std::vector<unsigned> b(u), e(n, 0);
+    ParallelAlgorithms::parallel_merge_sort_hybrid_radix_inner(b.data(), 0, n - 1, e.data(), true, thr);
+    EXPECT(e == sorted && b == u, "parallel_merge_sort_hybrid_radix_inner(srcToDst=true, thr=%zu)", thr);
+  }
+  std::vector<unsigned> c(u), w(n);
+  ParallelAlgorithms::parallel_merge_sort_hybrid_radix_single_buffer(c.data(), 0, n - 1, w.data(), false);
+  EXPECT(c == sorted, "parallel_merge_sort_hybrid_radix_single_buffer");
+  std::vector<unsigned> part(u), pw(n);
+  ParallelAlgorithms::parallel_merge_sort_hybrid_radix(part.data(), 100, n - 101, pw.data(), false, 5000);
+  std::vector<unsigned> pe(u);
+  std::sort(pe.begin() + 100, pe.end() - 100);
+  EXPECT(part == pe, "parallel_merge_sort_hybrid_radix(l=100, r=n-101)");
+}
+
 static uint32_t order_bits(float f) {
   uint32_t x;
   std::memcpy(&x, &f, 4);
@@ -239,6 +267,7 @@ static void concurrent_callers() {
 
 int main() {
   forwards();
+  merge_sorts();
   host_cutoffs();
   concurrent_callers();
   std::printf(failures ? "FAILED (%d)\n" : "ALL OK\n", failures.load());
