@@ -119,6 +119,68 @@ __global__ __launch_bounds__(kHistBlock) void k_hist_u8(const uint8_t* __restric
   }
 }
 
+// k_hist_u8 in one launch: every block adds its 256 counts into the stream's
+// control words ctl[0..255] and takes a ticket from ctl[256]; the block that
+// takes the last ticket reads and clears the counts (atomic exchange), writes
+// them widened to out and resets the ticket, so the words are zero again for
+// the next call and no clearing or widening launch is needed.
+__global__ __launch_bounds__(kHistBlock) void k_hist_u8_once(const uint8_t* __restrict__ data, size_t n,
+                                                             uint32_t* __restrict__ ctl, uint64_t* __restrict__ out) {
+  using H = dev::SubHist<32, false>;
+  __shared__ __align__(16) uint32_t s_h[H::words(256)];
+  __shared__ bool s_last;
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  for (uint32_t i = tid; i < H::words(256) / 4; i += kHistBlock) reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  size_t head = (16 - ((uintptr_t)data & 15)) & 15;
+  if (head > n) head = n;
+  const size_t nvec = (n - head) / 16;
+  const uint4* vk = reinterpret_cast<const uint4*>(data + head);
+  const size_t stride = (size_t)gridDim.x * kHistBlock;
+  constexpr int U = 4;
+  for (size_t v0 = (size_t)blockIdx.x * kHistBlock + tid; v0 < nvec; v0 += stride * U) {
+    uint4 x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (v0 + u * stride < nvec) x[u] = vk[v0 + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (v0 + u * stride >= nvec) continue;
+      const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+      uint32_t dg[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) dg[j] = (w[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+      H::add_all<16>(s_h, dg, lane);
+    }
+  }
+  if (blockIdx.x == 0) {
+    const size_t tail = head + nvec * 16;
+    const size_t extra = head + (n - tail);
+    for (size_t e = tid; e < extra; e += kHistBlock) {
+      size_t idx = e < head ? e : tail + (e - head);
+      H::add(s_h, data[idx], lane, 1u);
+    }
+  }
+  __syncthreads();
+  if (tid < 256) {
+    const uint32_t c = H::sum(s_h, tid);
+    if (c) {
+      // Device-scope atomics are applied where every CU sees them; waiting
+      // for the returned value means this one has been applied before the
+      // barrier and the ticket below (no fence, which would write back L2).
+      const uint32_t old = atomicAdd(&ctl[tid], c);
+      asm volatile("" ::"v"(old));
+    }
+  }
+  __syncthreads();
+  if (tid == 0) s_last = atomicAdd(&ctl[256], 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (s_last) {
+    if (tid < 256) out[tid] = atomicExch(&ctl[tid], 0u);
+    if (tid == 0) atomicExch(&ctl[256], 0u);
+  }
+}
+
 // Counting sort of bytes, the fill half (CountingSortParallel.h:30-76: after
 // the histogram, bin b's value fills [start[b], start[b] + count[b])).  Every
 // block scans the 256 counts in LDS, then each thread writes whole 16-byte
@@ -377,20 +439,21 @@ int pa_hist_u8(const uint8_t* bytes, size_t n, uint64_t* counts, pa_stream_t str
   const bool dk = n == 0 || is_device_ptr(bytes);
   const bool dout = is_device_ptr(counts);
   Layout L;
-  size_t o_c32 = L.take(256 * 4), o_c64 = L.take(256 * 8), o_in = dk ? 0 : L.take(n);
+  size_t o_c64 = L.take(256 * 8), o_in = dk ? 0 : L.take(n);
   void* base;
   PA_TRY(ensure_scratch(ctx, L.off, &base));
   char* b = (char*)base;
-  uint32_t* c32 = (uint32_t*)(b + o_c32);
   uint64_t* c64 = dout ? counts : (uint64_t*)(b + o_c64);
+  uint32_t* ctl = nullptr;
+  PA_TRY(stream_ctl(ctx, &ctl));
   const void* din;
   PA_TRY(stage_in(bytes, n, b + o_in, dk, s, &din));
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, c32, (size_t)256);
-  if (n) {
+  {
+    // n = 0 still runs one block, which writes the zero counts.
     ProfScope ps("k_hist_u8", s);
-    hipLaunchKernelGGL(k_hist_u8, dim3(hist_grid(n / 16 + 1)), dim3(kHistBlock), 0, s, (const uint8_t*)din, n, c32);
+    hipLaunchKernelGGL(k_hist_u8_once, dim3(n ? hist_grid(n / 16 + 1) : 1), dim3(kHistBlock), 0, s,
+                       (const uint8_t*)din, n, ctl, c64);
   }
-  hipLaunchKernelGGL(k_widen, dim3(1), dim3(256), 0, s, c32, c64, (size_t)256);
   PA_HIP(hipGetLastError());
   return deliver_u64(c64, counts, 256, dout, s);
 }

@@ -53,7 +53,15 @@ struct StreamCtx {
   void* pinned = nullptr;  // pinned host staging for small results
   size_t pinned_bytes = 0;
   PinnedSlot ring[4];      // pinned upload buffers reused once their copy is done
+  // Device control words (kStreamCtlWords u32), zero when allocated; a kernel
+  // that uses them leaves them zero again when it ends, so they never need
+  // clearing before a call.  Calls on one stream run in order, so one set per
+  // stream suffices.
+  uint32_t* ctl = nullptr;
 };
+constexpr size_t kStreamCtlWords = 1024;
+// The stream's control words (allocated and zeroed on first use).
+int stream_ctl(StreamCtx& ctx, uint32_t** out);
 
 StreamCtx& stream_ctx(hipStream_t s);
 // Ensures ctx.scratch holds >= bytes; on growth waits for the stream first,

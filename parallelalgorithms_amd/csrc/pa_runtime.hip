@@ -126,6 +126,23 @@ int ensure_scratch(StreamCtx& ctx, size_t bytes, void** out) {
   return PA_OK;
 }
 
+int stream_ctl(StreamCtx& ctx, uint32_t** out) {
+  if (!ctx.ctl) {
+    uint32_t* p = nullptr;
+    PA_HIP(hipMalloc(&p, kStreamCtlWords * 4));
+    // Once per stream, before any kernel uses the words: a plain clear,
+    // waited for (the words then stay zero from call to call).
+    if (hipMemset(p, 0, kStreamCtlWords * 4) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(p);
+      return hip_fail(hipErrorUnknown, "stream control words", __FILE__, __LINE__);
+    }
+    ctx.ctl = p;
+  }
+  *out = ctx.ctl;
+  return PA_OK;
+}
+
 size_t stream_scratch_held(hipStream_t s) {
   StreamCtx& ctx = stream_ctx(s);
   std::lock_guard<std::mutex> lk(ctx.mu);

@@ -111,3 +111,23 @@ def test_start_of_bins(pa, orc, quantum, digit):
     out = _dev_u64(nq * 256)
     pa.start_of_bins_u32(to_dev(a), quantum, nq, digit, out)
     np.testing.assert_array_equal(_host(out).reshape(nq, 256), expect)
+
+
+def test_hist_bytes_back_to_back_on_two_streams(pa, orc):
+    """The byte histogram keeps its running counts in per-stream control words
+    that its last block reads and clears: calls in a row, of every size
+    (empty, one byte, unaligned, large), alternating between two streams,
+    each give exactly their own counts."""
+    import torch
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    sizes = [0, 1, 15, 17, 100_003, 5_000_000, 1, 0, 33_333_333, 64]
+    for i, n in enumerate(sizes):
+        data = (pa.gen_u32(50 + i, max(n, 1)) % 251).astype(np.uint8)[:n]
+        d = to_dev(np.concatenate([np.zeros(3, np.uint8), data]))[3:]  # odd start address
+        out = _dev_u64(256)
+        st = streams[i % 2]
+        with torch.cuda.stream(st):
+            pa.hist_u8(d, out, stream=st.cuda_stream)
+        st.synchronize()
+        expect = orc.hist_bytes(data, 0, n) if n else np.zeros(256, np.uint64)
+        np.testing.assert_array_equal(_host(out), expect, err_msg=f"call {i} n={n}")
