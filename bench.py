@@ -261,6 +261,18 @@ def bench_configs(pa, torch, n=100_000_000, reps=5):
     def u(t):  # int32 view of u32 keys -> exact int64 values
         return t.to(torch.int64) & 0xFFFFFFFF
 
+    # Library reference point on the same GPU: torch.sort (rocPRIM's radix
+    # sort) of the 100M keys as int32 -- signed order, the same work as a u32
+    # sort.  Not a baseline the JSON line divides by; context only.
+    src_i32 = torch.from_numpy(pa.gen_u32(1234, n).view(np.int32)).to(dev)
+    ref_vals = {}
+
+    def torch_sort():
+        ref_vals["v"] = torch.sort(src_i32).values
+
+    timed("torch_sort_int32_100M", torch_sort, bytes_per_unit=36,
+          check=lambda: bool((ref_vals["v"][1:] >= ref_vals["v"][:-1]).all()))
+    del src_i32, ref_vals
     # Config 1: 10M uniform u32 (mt19937_64 seed 1234), 8-bit digits.
     m = 10_000_000
     s10 = torch.from_numpy(pa.gen_u32(1234, m).view(np.int32)).to(dev)
