@@ -556,13 +556,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 // Histogram read that also prepares pass 0's look-back rows: blocks take
 // whole tiles of `tile` keys (pass 0's tiles), write each tile's digit-0
 // counts to rows[t][R] and accumulate all D digit histograms into counts
-// [D][R] (R = 2^RB, D = ceil(key bits / RB)).  Also clears the look-back rows of passes 1..D-1 (zero_vec
-// 16-byte words at zero_ptr).
+// [D][R] (R = 2^RB, D = ceil(key bits / RB)).  Also clears the look-back rows
+// of passes 1..D-1 (zero_vec 16-byte words at zero_ptr).
 //
-// Counting uses lane-private sub-histograms (SubHist, packed 16-bit
-// counters): every key costs D conflict-free LDS adds.  A block counts at
-// most kHistTilesMaxKeys keys, so no packed counter can overflow (a copy of a
-// bin sees at most 1/C of the block's keys).
+// Counting uses lane-private sub-histograms (packed 16-bit counters, copy
+// lane % C of digit value v in word (v >> 1) * C + lane % C, half v & 1):
+// every key costs D conflict-free LDS adds, each with a bit-field extract for
+// the word and one multiply-add for the increment (1 or 1 << 16).  Digit 0's
+// counters are never cleared between tiles: a tile's row is the difference of
+// the running sums at its end and at the previous tile's end.  A block counts
+// at most kHistTilesMaxKeys keys, so no packed counter can overflow (a copy
+// of a bin sees at most 1/C of the block's keys) and the running sums are
+// exact.
 constexpr uint32_t kHistTilesMaxKeys = 1u << 19;
 
 template <typename K, int RB = 8>
@@ -575,27 +580,36 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__
   constexpr int PER = 16 / sizeof(K);
   // Copies: 32 for u32 keys, 16 for u64 (8-bit digits: 4 x 16 KB or 8 x 8 KB of LDS).
   constexpr int C = sizeof(K) == 4 ? 32 : 16;
+  constexpr int LOGC = C == 32 ? 5 : 4;
   using H = dev::SubHist<C, true>;
   constexpr uint32_t W = H::words(R);  // words per digit
   __shared__ __align__(16) uint32_t s_h[D * W];
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
+  const uint32_t lane_c = lane % C;
   for (uint32_t i = tid; i < D * W / 4; i += kSortBlock) reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0, 0, 0, 0);
   const size_t gstride = (size_t)gridDim.x * kSortBlock;
   for (size_t i = (size_t)blockIdx.x * kSortBlock + tid; i < zero_vec; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
   const bool aligned = ((uintptr_t)keys & 15) == 0 && (tile % PER) == 0;
   const size_t ntiles = (n + tile - 1) / tile;
-  auto count = [&](const K (&k)[PER]) {
+  auto add_key = [&](K k) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      uint32_t dg[PER];
-#pragma unroll
-      for (int j = 0; j < PER; ++j) dg[j] = dev::digit_bits<K, RB>(k[j], RB * d);
-      H::template add_all<PER>(s_h + d * W, dg, lane);
+      uint32_t hi, lo;
+      if constexpr (sizeof(K) == 4) {
+        // Bits past bit 31 read as zero, so a narrower last digit needs no mask.
+        hi = __builtin_amdgcn_ubfe((uint32_t)k, RB * d + 1, RB - 1);
+        lo = __builtin_amdgcn_ubfe((uint32_t)k, RB * d, 1);
+      } else {
+        const uint32_t dg = dev::digit_bits<K, RB>(k, RB * d);
+        hi = dg >> 1;
+        lo = dg & 1u;
+      }
+      atomicAdd(&s_h[d * W + ((hi << LOGC) | lane_c)], lo * 65535u + 1u);
     }
   };
-  uint32_t acc0 = 0;  // thread b < R: digit-0 count of bin b over the block's tiles
+  uint32_t run0 = 0;  // thread b < R: digit-0 count of bin b over the block's tiles so far
   for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const size_t lo = t * tile, hi = min(lo + tile, n);
     size_t scalar_from = lo;
@@ -614,11 +628,8 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__
           if (v0 + u * kSortBlock >= nvec) continue;
           K k[PER];
           __builtin_memcpy(k, &x[u], 16);
-          if (xform) {
 #pragma unroll
-            for (int j = 0; j < PER; ++j) k[j] = dev::key_fwd<K>(k[j], xform);
-          }
-          count(k);
+          for (int j = 0; j < PER; ++j) add_key(xform ? dev::key_fwd<K>(k[j], xform) : k[j]);
         }
       }
       scalar_from = lo + (size_t)nvec * PER;
@@ -626,21 +637,20 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_tiles(const K* __restrict__
     for (size_t i = scalar_from + tid; i < hi; i += kSortBlock) {
       K kk = keys[i];
       if (xform) kk = dev::key_fwd<K>(kk, xform);
-#pragma unroll
-      for (int d = 0; d < D; ++d) H::add(s_h + d * W, dev::digit_bits<K, RB>(kk, RB * d), lane, 1u);
+      add_key(kk);
     }
     __syncthreads();
     if (tid < R) {
       const uint32_t c = H::sum(s_h, tid);
-      rows[t * R + tid] = c;
-      acc0 += c;
+      rows[t * R + tid] = c - run0;
+      run0 = c;
     }
-    __syncthreads();
-    for (uint32_t i = tid; i < W / 4; i += kSortBlock) reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0, 0, 0, 0);
+    // The next tile's adds must not reach digit 0's counters before every
+    // thread has read them.
     __syncthreads();
   }
   if (tid < R) {
-    if (acc0) atomicAdd(&counts[tid], acc0);
+    if (run0) atomicAdd(&counts[tid], run0);
 #pragma unroll
     for (int d = 1; d < D; ++d) {
       const uint32_t c = H::sum(s_h + d * W, tid);
