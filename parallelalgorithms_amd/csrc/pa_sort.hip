@@ -23,7 +23,8 @@
 //
 // HBM layout: keys (n x K) ping-pong between `keys` and `tmp`; D passes with
 // D = sizeof(K) is even, so the result lands back in `keys`.  Scratch per
-// call: counts D x 256 u32, tile counters D u32, look-back D x tiles x 256 u32.
+// call: counts D x 256 u32, tile counters (8 for pass 0, one per later pass),
+// look-back D x tiles x 256 u32.
 #include <algorithm>
 #include <atomic>
 #include <mutex>
@@ -313,14 +314,15 @@ size_t sort_scratch_bytes(const PassVariant& var, size_t n) {
   Layout L;
   L.take(D * R * 4);                      // counts
   L.take(D * R * 4);                      // bases
-  L.take(D * 4);                          // tile counters
+  L.take(sort_tile_counters((int)D) * 4);  // tile counters
   L.take(D * rows * R * 4);               // look-back table
   L.take((size_t)kPrefixGroups * R * 4);  // pass-0 prefix partial sums
   return L.off;
 }
 
 // The sort with RB-bit digits: D = ceil(key bits / RB) passes.  Scratch
-// layout: counts [D][R], digit bases [D][R], tile counters [D], look-back
+// layout: counts [D][R], digit bases [D][R], tile counters [kXcds + D - 1]
+// (pass 0 claims per XCD, the later passes from one counter each), look-back
 // rows [D][tiles][R], prefix partials [groups][R].  With an odd D the last
 // pass writes into the temporary arrays and the result is copied back.
 template <typename K, bool KV, int RB>
@@ -333,14 +335,15 @@ int enqueue_sort_rb(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, ui
   Layout L;
   uint32_t* counts = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
   uint32_t* bases = reinterpret_cast<uint32_t*>(scratch + L.take(ncount * 4));
-  uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch + L.take(D * 4));
+  uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch + L.take(sort_tile_counters(D) * 4));
   const size_t lb_bytes = (size_t)D * rows * R * 4;
   uint32_t* lb = reinterpret_cast<uint32_t*>(scratch + L.take(lb_bytes));
   uint32_t* prefix_partial = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)kPrefixGroups * R * 4));
 
   {
     ProfScope ps("k_sort_prepare", s);
-    hipLaunchKernelGGL(k_sort_prepare, dim3(1), dim3(256), 0, s, counts, (uint32_t)ncount, ctr, (uint32_t)D);
+    hipLaunchKernelGGL(k_sort_prepare, dim3(1), dim3(256), 0, s, counts, (uint32_t)ncount, ctr,
+                       sort_tile_counters(D));
   }
   const size_t tile = (size_t)var.block * var.items;
   // One read: all digit histograms, pass 0's per-tile digit counts (into
@@ -382,7 +385,7 @@ int enqueue_sort_rb(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, ui
     if (xform && p == 0) fmode |= kFloatIn;
     if (xform && p == D - 1) fmode |= kFloatOut;
     if (xform == dev::kXformSigned) fmode |= kSignedKeys;
-    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * R, lb + (size_t)p * rows * R, ctr + p,
+    PassArgs a{kin, kout, vin, vout, (uint32_t)n, bases + (size_t)p * R, lb + (size_t)p * rows * R, ctr + (p == 0 ? 0 : kXcds - 1 + p),
                (uint32_t)rows, (uint32_t)(RB * p), fmode};
     PA_TRY(var.launch(s, a));
     std::swap(kin, kout);

@@ -14,6 +14,21 @@ constexpr uint32_t kLbInc = 2u << 30;
 constexpr uint32_t kLbMask = (1u << 30) - 1;
 constexpr size_t kMaxSortN = (size_t(1) << 30) - 1;
 
+// Tile claims of a pass whose prefixes are ready (no tile waits on another):
+// tiles go to XCDs in groups of kXcdGroup consecutive tiles (XCD y owns the
+// groups g with g % kXcds == y) and each XCD claims its own groups in order
+// from its own counter, so neighbouring tiles run at the same time on one XCD
+// and the partial cache lines their runs share in every bin meet in that
+// XCD's L2.  A block whose XCD has no tiles left takes the next XCD's, so every
+// tile is claimed whatever the placement of the blocks.  The XCD index is the
+// low 4 bits of hardware register HW_REG_XCC_ID (20), read with s_getreg
+// (immediate: size 4 - 1 << 11 | offset 0 << 6 | register 20).
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdGroup = 16;
+constexpr int kHwRegXccId = (3 << 11) | 20;
+// Tile counters of one sort: kXcds for pass 0, one for each later pass.
+constexpr uint32_t sort_tile_counters(int passes) { return kXcds + (uint32_t)passes - 1; }
+
 // Pass flags: key transform on load / on store (kSignedKeys selects the
 // two's-complement transform, else the IEEE-754 one); kPrefixReady: every
 // tile's inclusive prefix was published before the pass (pass 0 of the plain
@@ -221,7 +236,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
 #pragma unroll
     for (uint32_t i = tid; i < ZV; i += BLOCK) z[i] = make_uint4(0, 0, 0, 0);
   }
-  if (tid == 0) s_claim = atomicAdd(tile_counter, 1u);
+  if (tid == 0) {
+    if (fmode & kPrefixReady) {
+      const uint32_t x = __builtin_amdgcn_s_getreg(kHwRegXccId) & (kXcds - 1);
+      uint32_t t = ntiles;
+      for (uint32_t k = 0; k < kXcds && t == ntiles; ++k) {
+        const uint32_t y = (x + k) & (kXcds - 1);
+        const uint32_t j = atomicAdd(tile_counter + y, 1u);
+        const uint64_t c = ((uint64_t)(j / kXcdGroup) * kXcds + y) * kXcdGroup + j % kXcdGroup;
+        if (c < ntiles) t = (uint32_t)c;
+      }
+      s_claim = t;
+    } else {
+      s_claim = atomicAdd(tile_counter, 1u);
+    }
+  }
   __syncthreads();
   // Uniform across the block: the tile geometry below lives in scalar registers.
   const uint32_t claimed = __builtin_amdgcn_readfirstlane(s_claim);
