@@ -306,7 +306,8 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   C comp{s_stage[wave], buf + (size_t)blockIdx.x * region, region, &s_ctr, lane};
   for (uint32_t i = tid; i < H::words(256) / 4; i += kSelBlock) reinterpret_cast<uint4*>(s_sub)[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
-  K lo_c = 0, hi = 0;  // max(~key), max(key) over this thread's keys
+  K mn = ~K(0), hi = 0;  // min(key), max(key) over this thread's keys
+  const uint32_t lane_c = lane & 31;
   size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
   if (head > n) head = n;
   const size_t nvec = (n - head) / PER;
@@ -327,15 +328,15 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
       if (v0 + u * stride >= nvec) continue;
       K kk[PER];
       __builtin_memcpy(kk, &x[u], 16);
-      uint32_t dg[PER];
 #pragma unroll
       for (int j = 0; j < PER; ++j) {
-        dg[j] = dev::digit8<K>(kk[j], 8 * (D - 1));
-        lo_c = umax<K>(lo_c, (K)~kk[j]);
+        const uint32_t dg = dev::digit8<K>(kk[j], 8 * (D - 1));
+        mn = kk[j] < mn ? kk[j] : mn;
         hi = umax<K>(hi, kk[j]);
-        hits |= (uint32_t)(dg[j] - band_lo <= band_w) << (u * PER + j);
+        hits |= (uint32_t)(dg - band_lo <= band_w) << (u * PER + j);
+        // Copy lane % 32 of bin dg: word dg * 32 + lane % 32 (the SubHist<32, false> layout).
+        atomicAdd(&s_sub[(dg << 5) | lane_c], 1u);
       }
-      H::template add_all<PER>(s_sub, dg, lane);
     }
     if (band) comp.take(x, hits);
   }
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
         __builtin_memcpy(&x[0], &key, sizeof(K));
         const uint32_t dg = dev::digit8<K>(key, 8 * (D - 1));
         H::add(s_sub, dg, lane, 1u);
-        lo_c = umax<K>(lo_c, (K)~key);
+        mn = key < mn ? key : mn;
         hi = umax<K>(hi, key);
         hits = dg - band_lo <= band_w;
       }
@@ -365,7 +366,8 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
     if (tid == 0) band_counts[blockIdx.x] = (uint32_t)(s_ctr < 0xFFFFFFFFull ? s_ctr : 0xFFFFFFFFull);
   }
   // Wave maxima, then the block's, stored without atomics (the next launch
-  // reduces the blocks' values).
+  // reduces the blocks' values); the minimum is kept as the maximum of ~key.
+  K lo_c = (K)~mn;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     lo_c = umax<K>(lo_c, (K)__shfl_xor(lo_c, off, 64));
