@@ -48,13 +48,69 @@ __device__ __forceinline__ size_t merge_path(const K* a, size_t na, const K* b, 
   return lo;
 }
 
+// The same split as merge_path, for runs in global memory, where every probe
+// is a dependent HBM round trip: the probes are placed by interpolation.
+// P(i) = a[i] <= b[d - 1 - i] is true below the split and false from it on,
+// and f(i) = a[i] - b[d - 1 - i] rises with i, so the split is near the zero
+// crossing of the line through the two nearest probed points (one with
+// P true, one with P false).  A step that does not halve the interval is
+// followed by a bisection step, so there are at most about twice as many
+// probes as a binary search makes; on evenly spread keys a handful of probes
+// leave an interval of 32 or fewer, which the binary search finishes.  Only
+// where the probes go depends on the key values (through f, in double), never
+// the result: every decision is the exact comparison P.
+template <typename K>
+__device__ size_t merge_path_interp(const K* a, size_t na, const K* b, size_t nb, size_t d) {
+  size_t lo = d > nb ? d - nb : 0;
+  size_t hi = d < na ? d : na;  // invariant: P true below lo, false from hi on
+  if (hi - lo > 32) {
+    const K a0 = a[lo], b0 = b[d - 1 - lo], a1 = a[hi - 1], b1 = b[d - hi];
+    if (!(a0 <= b0)) return lo;
+    if (a1 <= b1) return hi;
+    // Known points: pl (P true, f <= 0) and ph (P false, f > 0).
+    size_t pl = lo, ph = hi - 1;
+    double fl = (double)a0 - (double)b0, fh = (double)a1 - (double)b1;
+    lo = pl + 1;
+    hi = ph;
+    bool bisect = false;
+    while (hi - lo > 32) {
+      size_t mid = lo + (hi - lo) / 2;
+      if (!bisect && fh > fl) {
+        const double x = (double)pl + (double)(ph - pl) * (-fl / (fh - fl));
+        const double m = x < (double)lo ? (double)lo : (x > (double)(hi - 1) ? (double)(hi - 1) : x);
+        mid = (size_t)m;
+      }
+      const K am = a[mid], bm = b[d - 1 - mid];
+      const size_t before = hi - lo;
+      if (am <= bm) {
+        lo = mid + 1;
+        pl = mid;
+        fl = (double)am - (double)bm;
+      } else {
+        hi = mid;
+        ph = mid;
+        fh = (double)am - (double)bm;
+      }
+      bisect = 2 * (hi - lo) > before;
+    }
+  }
+  while (lo < hi) {
+    const size_t mid = lo + (hi - lo) / 2;
+    if (a[mid] <= b[d - 1 - mid])
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_merge_partition(const K* __restrict__ a, size_t na, const K* __restrict__ b,
                                                          size_t nb, size_t ntiles, size_t* __restrict__ split) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t > ntiles) return;
   const size_t d = std::min(t * (size_t)kMergeTile, na + nb);
-  split[t] = merge_path(a, na, b, nb, d);
+  split[t] = merge_path_interp(a, na, b, nb, d);
 }
 
 template <typename K>

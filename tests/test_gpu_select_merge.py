@@ -173,3 +173,58 @@ def test_merge_u64_and_host_pointers(pa, orc):
     hout = np.zeros(t.size, dtype=np.uint64)
     pa.merge_u64(x, y, hout)  # host memory: staged, synchronous
     np.testing.assert_array_equal(hout, expect)
+
+
+def _skewed_runs(kind, na, nb):
+    """Sorted u32 runs whose values are spread so that a line through two
+    probed points predicts the merge split badly (the tile splits must still
+    be exact): cubic growth, one run entirely below the other, a lone huge
+    outlier at the end of each run, long runs of one repeated value, and
+    runs that interleave only in a narrow band."""
+    i = np.arange(max(na, nb), dtype=np.float64)
+    if kind == "cubic":
+        x = (i[:na] ** 3 / max(1.0, (na - 1)) ** 3 * 4.0e9).astype(np.uint32)
+        y = (np.sqrt(i[:nb] / max(1, nb - 1)) * 4.0e9).astype(np.uint32)
+    elif kind == "disjoint":
+        x = np.arange(na, dtype=np.uint32)
+        y = np.arange(nb, dtype=np.uint32) + np.uint32(na + 7)
+    elif kind == "outlier":
+        x = np.arange(na, dtype=np.uint32)
+        y = np.arange(nb, dtype=np.uint32) * np.uint32(2)
+        x[-1] = 0xFFFFFFFF
+        y[-1] = 0xFFFFFFFE
+    elif kind == "plateaus":
+        x = (np.arange(na) // 50_000).astype(np.uint32) * np.uint32(3)
+        y = (np.arange(nb) // 70_001).astype(np.uint32) * np.uint32(3) + np.uint32(1)
+    else:  # band: the runs overlap only in the middle tenth
+        x = np.sort(np.concatenate([np.zeros(na * 9 // 20, np.uint32),
+                                    np.random.default_rng(41).integers(1000, 2000, na - na * 9 // 20 - na * 9 // 20, dtype=np.uint32),
+                                    np.full(na * 9 // 20, 0xFFFFFFFF, np.uint32)]))
+        y = np.sort(np.random.default_rng(42).integers(900, 2100, nb, dtype=np.uint32))
+    return np.sort(x), np.sort(y)
+
+
+@pytest.mark.parametrize("kind", ["cubic", "disjoint", "outlier", "plateaus", "band"])
+def test_merge_split_search_skewed_values(pa, kind):
+    """The tile splits are found by an interpolation-placed search in HBM;
+    whatever the value spread, the merged output must be the stable merge
+    (checked against numpy's stable sort of the concatenation)."""
+    na, nb = 1_000_003, 777_777
+    x, y = _skewed_runs(kind, na, nb)
+    expect = np.sort(np.concatenate([x, y]), kind="stable")
+    out = dev_empty(na + nb, np.uint32)
+    pa.merge_u32(to_dev(x), to_dev(y), out)
+    np.testing.assert_array_equal(to_host(out, np.uint32), expect)
+
+
+def test_merge_u64_split_search_near_top(pa):
+    """u64 keys near 2^64, where differences of key values lose their low bits
+    as doubles: the split search may only use them to place probes."""
+    base = np.uint64(0xFFFFFFFF00000000)
+    x = base + np.arange(0, 2_000_000, 2, dtype=np.uint64)
+    y = base + np.arange(1, 1_500_001, 3, dtype=np.uint64)
+    y[-5:] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    expect = np.sort(np.concatenate([x, y]), kind="stable")
+    out = dev_empty(x.size + y.size, np.uint64)
+    pa.merge_u64(to_dev(x), to_dev(y), out)
+    np.testing.assert_array_equal(to_host(out, np.uint64), expect)
