@@ -49,3 +49,35 @@ def test_noalloc_rejects_what_it_cannot_do(pa):
     h = np.zeros(1000, dtype=np.uint32)
     assert lib.pa_sort_u32_noalloc(h.ctypes.data, h.ctypes.data, h.size, stream) == PA_ERR_ARG
     assert lib.pa_sort_u32_noalloc(None, None, 10, stream) == PA_ERR_ARG
+
+
+def test_noalloc_sort_in_a_captured_graph(pa):
+    """The entry point is for code under HIP graph capture: a 10M-key sort is
+    captured once (kernels only; the sort enqueues no memset or allocation)
+    and replayed three times, each time on fresh keys written eagerly into
+    the captured buffer, with the program's own work and synchronisations
+    between replays; every replay must equal torch.sort of its keys."""
+    import torch
+    lib = pa.load()
+    n = 10_000_000
+    keys = dev_empty(n, np.uint32)
+    tmp = dev_empty(n, np.uint32)
+    # One eager call first: looks up the static array's address and loads the
+    # kernels outside the capture.
+    keys.copy_(to_dev(pa.gen_u32(80, n)))
+    assert _noalloc(lib, keys, tmp, n, torch.cuda.current_stream().cuda_stream) == 0
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        rc = _noalloc(lib, keys, tmp, n, torch.cuda.current_stream().cuda_stream)
+    assert rc == 0, lib.pa_last_error_string()
+    for seed in (81, 82, 83):
+        print(f"[noalloc graph] replay seed={seed}", flush=True)
+        fresh = to_dev(pa.gen_u32(seed, n))
+        keys.copy_(fresh)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        ref = torch.sort(fresh.to(torch.int64) & 0xFFFFFFFF).values.to(torch.int32)
+        assert torch.equal(keys, ref), seed
