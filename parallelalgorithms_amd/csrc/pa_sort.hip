@@ -11,15 +11,18 @@
 //
 // GPU form:
 //   k_sort_prepare   zero the digit counts and tile counters.
-//   k_hist_digits    one read of the keys -> D x 256 counts (the TwoPhase
-//                    idea); LDS-privatised bins, wave-uniform fast path for
-//                    constant/presorted runs; also zeroes the look-back table.
-//   k_onesweep       per digit: a tile of 512 x ITEMS keys is ranked stably in
-//                    LDS with wave64 ballot matching, each tile's per-bin count
-//                    is published to a decoupled look-back table (replacing the
-//                    serial quantum x bin prefix), and keys are staged in LDS in
-//                    digit order so the global writes are runs per bin (the GPU
-//                    form of the 64-deep de-randomisation buffers).
+//   k_hist_tiles     one read of the keys (the TwoPhase idea): every digit's
+//                    histogram, pass 0's per-tile digit counts, and the
+//                    clearing of the later passes' look-back rows.
+//   k_tile_sums,     pass 0's per-tile counts -> inclusive prefixes (as if
+//   k_tile_scan      every pass-0 tile had resolved its look-back) and the
+//                    digit bases of every pass.
+//   k_onesweep       per digit: a tile of BLOCK x ITEMS keys is ranked stably
+//                    in LDS, each tile's per-bin count is published to a
+//                    decoupled look-back table (replacing the serial
+//                    quantum x bin prefix), and keys are staged in LDS in
+//                    digit order so the global writes are runs per bin (the
+//                    GPU form of the 64-deep de-randomisation buffers).
 //
 // HBM layout: keys (n x K) ping-pong between `keys` and `tmp`; D passes with
 // D = sizeof(K) is even, so the result lands back in `keys`.  Scratch per
