@@ -120,9 +120,11 @@ __global__ __launch_bounds__(256) void k_select_init(SelState* st, const K* __re
 // error has a standard deviation of at most sqrt(S) / 2, so the bins holding
 // sample ranks within 4 of those deviations of it (a contiguous range) hold
 // the answer with near certainty.  The band is used only when the sample puts
-// at most a quarter of the keys in it, so its keys fit the compaction buffer
-// (cap = n / 2) with room to spare; a miss or an overflow only costs the
-// second read the select would do anyway.  Threads 0..255 take part; s_band
+// at most half of the keys in it, so its keys fit the compaction buffer
+// (cap = n / 2) unless the sample misjudged them; a miss or an overflow only
+// costs the second read the select would do anyway.  (A band straddling the
+// boundary of two large bins, as a presorted input's middle does, holds a
+// third of the keys.)  Threads 0..255 take part; s_band
 // receives (lo, hi, on).
 __device__ void sample_band(const uint32_t* __restrict__ sample, uint64_t k, uint64_t n, uint32_t* s_h,
                             uint32_t* s_wsum, uint32_t* s_band) {
@@ -157,7 +159,7 @@ __device__ void sample_band(const uint32_t* __restrict__ sample, uint64_t k, uin
   if (tid == 0) {
     uint64_t in_band = 0;
     for (uint32_t b = s_band[0]; b <= s_band[1]; ++b) in_band += s_h[b];
-    s_band[2] = S == kSelSample && in_band * 4 <= S;
+    s_band[2] = S == kSelSample && in_band * 2 <= S;
   }
   __syncthreads();
 }

@@ -228,3 +228,30 @@ def test_merge_u64_split_search_near_top(pa):
     out = dev_empty(x.size + y.size, np.uint64)
     pa.merge_u64(to_dev(x), to_dev(y), out)
     np.testing.assert_array_equal(to_host(out, np.uint64), expect)
+
+
+def test_select_band_overflow_falls_back(pa):
+    """The first read compacts the keys of a band of top-digit bins predicted
+    from strided sample runs (256 runs of 128 keys at positions
+    run * (n - 128) / 255) when the sample puts at most half of the keys in
+    it.  Here the sampled runs hold 45% band keys and the rest of the array
+    70%, so the band is taken and overflows the n/2-key buffer: the select
+    must fall back and still return sorted[k]."""
+    n = 4_000_000
+    rng = np.random.default_rng(77)
+    sampled = np.zeros(n, dtype=bool)
+    for run in range(256):
+        b0 = run * (n - 128) // 255
+        sampled[b0:b0 + 128] = True
+    band = np.uint32(5 << 24)
+    other = np.uint32(200 << 24)
+    low = rng.integers(0, 1 << 24, n, dtype=np.uint32)
+    p_band = np.where(sampled, 0.45, 0.70)
+    a = np.where(rng.random(n) < p_band, band | low, other | low).astype(np.uint32)
+    srt = np.sort(a)
+    assert (srt >> 24 == 5).sum() > n // 2  # the band really overflows the buffer
+    d = to_dev(a)
+    for k in (n * 3 // 10, n // 2, n - 1):
+        res = np.zeros(1, dtype=np.uint32)
+        pa.select_u32(d, k, res)
+        assert int(res[0]) == int(srt[k]), k
