@@ -235,14 +235,24 @@ struct WaveCompactor {
     if (staged + wtot > STAGE) flush();
     __builtin_amdgcn_wave_barrier();
     uint32_t pos = staged + inc - nm;
+    // Per vector, each lane stores its hits lowest first, one a round, the
+    // key picked from the vector's PER keys by selects: hits are sparse, so
+    // a round in which any lane of the wave has a second hit in one vector
+    // is rare, against PER vote-guarded stores per vector.
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       K xs[PER];
       __builtin_memcpy(xs, &x[u], 16);
+      uint32_t h = (hits >> (u * PER)) & ((1u << PER) - 1u);
+      while (__any(h != 0)) {
+        if (h) {
+          const uint32_t c = (uint32_t)__builtin_ctz(h);
+          K v = xs[0];
 #pragma unroll
-      for (int j = 0; j < PER; ++j) {
-        const bool hit = (hits >> (u * PER + j)) & 1u;
-        if (__any(hit) && hit) stage[pos++] = xs[j];  // no LDS instruction when no lane of the wave hits
+          for (int j = 1; j < PER; ++j) v = c == (uint32_t)j ? xs[j] : v;
+          stage[pos++] = v;
+          h &= h - 1u;
+        }
       }
     }
     __builtin_amdgcn_wave_barrier();
