@@ -187,10 +187,29 @@ __device__ void pick_bin(const uint32_t* s_row, uint64_t k, uint32_t* s_wsum, ui
   __syncthreads();
 }
 
-// Adds a block's LDS counts into the global histogram (zero bins skipped).
-__device__ __forceinline__ void flush_counts(const uint32_t* s_hist, uint32_t* ghist, uint32_t nbins) {
-  for (uint32_t i = threadIdx.x; i < nbins; i += kSelBlock)
-    if (s_hist[i]) atomicAdd(&ghist[i], s_hist[i]);
+// Global histograms are kept as kSelCopies copies of the 256 bins: block b
+// adds its counts into copy b % kSelCopies, so the adds of the 512 blocks of a
+// launch meet 32 to an address rather than 512 (adds to one address are
+// applied one after another), and a reader sums the copies.
+constexpr uint32_t kSelCopies = 16;
+constexpr uint32_t kSelHistWords = 256 * kSelCopies;
+
+__device__ __forceinline__ uint32_t hist_read(const uint32_t* __restrict__ g, uint32_t bin) {
+  uint32_t c[kSelCopies];
+#pragma unroll
+  for (uint32_t i = 0; i < kSelCopies; ++i) c[i] = g[i * 256 + bin];
+  uint32_t t = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kSelCopies; ++i) t += c[i];
+  return t;
+}
+
+// Adds a block's LDS counts into its copy of a global histogram (zero bins
+// skipped).
+__device__ __forceinline__ void flush_counts(const uint32_t* s_hist, uint32_t* ghist) {
+  uint32_t* const g = ghist + (blockIdx.x % kSelCopies) * 256;
+  for (uint32_t i = threadIdx.x; i < 256; i += kSelBlock)
+    if (s_hist[i]) atomicAdd(&g[i], s_hist[i]);
 }
 
 template <typename K>
@@ -281,7 +300,7 @@ struct WaveCompactor {
 // its share whose top digit lies in the band into its own region of buf
 // (cap / grid keys at buf + b * (cap / grid); band_counts[b] keys, more than
 // the region when it overflowed), so no counter is shared between blocks.
-// Four 16-byte loads per thread are in flight per step.
+// Four 16-byte loads per thread per step, issued a step ahead.
 template <typename K>
 __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict__ keys, size_t n,
                                                             uint32_t* __restrict__ ghist, K* __restrict__ mm,
@@ -304,6 +323,21 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   __shared__ uint32_t s_band[3];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t k = S->k, cap = S->cap;
+  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
+  if (head > n) head = n;
+  const size_t nvec = (n - head) / PER;
+  const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
+  const size_t stride = (size_t)gridDim.x * kSelBlock;
+  // Every lane of a wave runs the same iterations (the bound uses the wave's
+  // first index), so the wave-level compaction stays convergent.
+  const size_t wfirst = (size_t)blockIdx.x * kSelBlock + wave * 64;
+  // The loop is software-pipelined: a step's loads are issued before the
+  // previous step's keys are counted and compacted, and the first step's
+  // before the band is known.
+  uint4 nx[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (wfirst + lane + u * stride < nvec) nx[u] = vk[wfirst + lane + u * stride];
   sample_band(sample, k, n, s_hist, s_wsum, s_band);
   const uint32_t band = s_band[2];
   const uint32_t band_lo = s_band[0], band_w = s_band[1] - s_band[0];
@@ -320,21 +354,15 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   __syncthreads();
   K mn = ~K(0), hi = 0;  // min(key), max(key) over this thread's keys
   const uint32_t lane_c = lane & 31;
-  size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
-  if (head > n) head = n;
-  const size_t nvec = (n - head) / PER;
-  const uint4* vk = reinterpret_cast<const uint4*>(keys + head);
-  const size_t stride = (size_t)gridDim.x * kSelBlock;
-  // Every lane of a wave runs the same iterations (the bound uses the wave's
-  // first index), so the wave-level compaction stays convergent.
-  const size_t wfirst = (size_t)blockIdx.x * kSelBlock + wave * 64;
   for (size_t w0 = wfirst; w0 < nvec; w0 += stride * U) {
     const size_t v0 = w0 + lane;
     uint4 x[U];
     uint32_t hits = 0;
 #pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = nx[u];
+#pragma unroll
     for (int u = 0; u < U; ++u)
-      if (v0 + u * stride < nvec) x[u] = vk[v0 + u * stride];
+      if (v0 + (u + U) * stride < nvec) nx[u] = vk[v0 + (u + U) * stride];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (v0 + u * stride >= nvec) continue;
@@ -400,7 +428,7 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   }
   if (tid < 256) {
     const uint32_t c = H::sum(s_sub, tid);
-    if (c) atomicAdd(&ghist[tid], c);
+    if (c) atomicAdd(&ghist[(blockIdx.x % kSelCopies) * 256 + tid], c);
   }
 }
 
@@ -495,11 +523,11 @@ __device__ SelState step_state(int step, const SelState* __restrict__ S, const u
                                uint32_t* s_hist, SelShared& sh) {
   SelState t = S[step];
   if (step == 0) {
-    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_hist[i] = ghist_first[i];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_hist[i] = hist_read(ghist_first, i);
     __syncthreads();
     // Minimum and maximum over the first read's per-block values (that
     // launch had as many blocks as this one).
-    const K* mm = reinterpret_cast<const K*>(ghist_first + 256);
+    const K* mm = reinterpret_cast<const K*>(ghist_first + kSelHistWords);
     K lo_c = 0, hi = 0;
     for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
       lo_c = umax<K>(lo_c, mm[2 * i]);
@@ -543,7 +571,7 @@ __device__ SelState step_state(int step, const SelState* __restrict__ S, const u
       }
     }
   } else if (!t.done) {
-    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_hist[i] = ghist_prev[i];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_hist[i] = hist_read(ghist_prev, i);
     __syncthreads();
     const bool compacted = !t.src && t.compact_now;
     pick_state(s_hist, t, compacted, compacted ? *ctr_prev : 0, sh);
@@ -573,10 +601,10 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
   __shared__ uint32_t s_wcnt[WAVES];
   __shared__ unsigned long long s_bbase;
   __shared__ SelShared sh;
-  const SelState t = step_state<K>(step, S, ghist_first, step ? ghist + (step - 1) * 256 : nullptr,
+  const SelState t = step_state<K>(step, S, ghist_first, step ? ghist + (step - 1) * kSelHistWords : nullptr,
                                    step ? ctr + (step - 1) : nullptr, band_counts, S + step + 1, s_hist, sh);
   if (t.done) return;  // uniform for the whole grid
-  uint32_t* const gh = ghist + step * 256;
+  uint32_t* const gh = ghist + step * kSelHistWords;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const K prefix = (K)t.prefix, pmask = (K)t.pmask;
   const uint32_t shift = (uint32_t)t.shift;
@@ -648,7 +676,7 @@ __global__ __launch_bounds__(kSelBlock) void k_select_pass(const K* __restrict__
   }
   if (compact) comp.template finish<WAVES>(s_wcnt, &s_bbase, tid, wave);
   __syncthreads();
-  flush_counts(s_hist, gh, 256);
+  flush_counts(s_hist, gh);
 }
 
 // The selected key from the last step's state (picking from the last
@@ -664,7 +692,7 @@ __global__ __launch_bounds__(256) void k_select_final(const SelState* __restrict
   __shared__ SelShared sh;
   SelState t = *S_last;
   if (!t.done) {
-    s_hist[threadIdx.x] = ghist_last[threadIdx.x];
+    s_hist[threadIdx.x] = hist_read(ghist_last, threadIdx.x);
     __syncthreads();
     const bool compacted = !t.src && t.compact_now;
     pick_state(s_hist, t, compacted, compacted ? *ctr_last : 0, sh);
@@ -677,13 +705,13 @@ __global__ __launch_bounds__(256) void k_select_final(const SelState* __restrict
 
 // Scratch: the D per-step states, then the zeroed control words (the top
 // digit's histogram from the first read, the per-block minima and maxima,
-// one histogram per pass, one 64-bit compaction counter per pass and the
-// band's), then the compaction buffer.
+// one histogram per pass, each histogram in kSelCopies copies, and one
+// 64-bit compaction counter per pass), then the compaction buffer.
 constexpr unsigned kSelMaxGrid = 512;  // blocks of the first read and the passes
 constexpr size_t kSelMinMaxWords = 2 * kSelMaxGrid * 2;  // two 64-bit words per block
 template <typename K>
 static constexpr size_t select_zero_words() {
-  return 256 + kSelMinMaxWords + (sizeof(K) - 1) * 256 + 2 * (sizeof(K) - 1);
+  return kSelHistWords + kSelMinMaxWords + (sizeof(K) - 1) * kSelHistWords + 2 * (sizeof(K) - 1);
 }
 // The sample's per-block counts and the band's per-block key counts follow
 // the cleared words (written in full before they are read, never cleared).
@@ -711,8 +739,8 @@ static void launch_select(const K* din, size_t n, size_t k, SelState* S, uint32_
                           K* dst, K* place, hipStream_t s) {
   constexpr int D = sizeof(K);
   uint32_t* ghist_first = z;  // followed by the minimum and maximum words
-  uint32_t* ghist = z + 256 + kSelMinMaxWords;
-  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ghist + (D - 1) * 256);
+  uint32_t* ghist = z + kSelHistWords + kSelMinMaxWords;
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ghist + (D - 1) * kSelHistWords);
   uint32_t* sample = reinterpret_cast<uint32_t*>(ctr + (D - 1));
   uint32_t* band_counts = sample + kSelSampleBlocks * 256;
   hipLaunchKernelGGL(k_select_init<K>, dim3(kSelSampleBlocks), dim3(256), 0, s, S, din, (uint64_t)k, (uint64_t)n,
@@ -722,14 +750,14 @@ static void launch_select(const K* din, size_t n, size_t k, SelState* S, uint32_
   {
     ProfScope ps("k_select_first", s);
     hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghist_first,
-                       reinterpret_cast<K*>(ghist_first + 256), S, buf, band_counts, sample);
+                       reinterpret_cast<K*>(ghist_first + kSelHistWords), S, buf, band_counts, sample);
   }
   for (int step = 0; step < D - 1; ++step) {
     ProfScope ps("k_select_pass", s);
     hipLaunchKernelGGL(k_select_pass<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, buf, step, S, ghist_first, ghist,
                        ctr, band_counts);
   }
-  hipLaunchKernelGGL(k_select_final<K>, dim3(1), dim3(256), 0, s, S + (D - 1), ghist + (D - 2) * 256, ctr + (D - 2),
+  hipLaunchKernelGGL(k_select_final<K>, dim3(1), dim3(256), 0, s, S + (D - 1), ghist + (D - 2) * kSelHistWords, ctr + (D - 2),
                      dst, place);
 }
 
