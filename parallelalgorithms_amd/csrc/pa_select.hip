@@ -59,109 +59,105 @@ struct SelState {
   uint32_t band_lo, band_hi, band_on;
 };
 
-// Keys sampled to predict the answer's top-digit bin: runs of kSelRun
-// consecutive keys at an even stride, kSelSampleRuns runs per block of the
-// sampling launch.
+// Keys sampled to predict the answer's top-digit bins: kSelRuns runs of
+// kSelRun consecutive keys at an even stride (single keys at that stride would
+// each cost a page walk), or every key when n < kSelSample.
 constexpr uint32_t kSelSample = 32768;
-constexpr uint32_t kSelSampleBlocks = 32;
-constexpr uint32_t kSelSampleRuns = 8;
-constexpr uint32_t kSelRun = kSelSample / (kSelSampleBlocks * kSelSampleRuns);
+constexpr uint32_t kSelInitBlock = 1024;
+constexpr uint32_t kSelRun = 128;  // two keys per lane of a wave
+constexpr uint32_t kSelRuns = kSelSample / kSelRun;
 
-// Initial state, cleared control words and the sample.  kSelSampleBlocks
-// blocks each read kSelSampleRuns runs of kSelRun consecutive keys at an even
-// stride over the array (single keys at that stride would each cost a page
-// walk), or together every key when n < kSelSample, and write their top-digit
-// counts to sample[block][256] (no atomics, so nothing needs clearing first).
-// All of a thread's loads are issued before any is counted.  Block 0 writes
-// the initial state; every block clears a share of the control words.
+// Initial state, cleared control words and the band, in one block.  The block
+// reads the sample (all of a thread's loads issued before any is counted;
+// wave w reads runs w, w + 16, ..., each run by its 64 lanes), counts the
+// sample's top digits in lane-private sub-histograms (a repeated key costs no
+// more than distinct ones) and picks the band.
+//
+// The band: the answer's rank k maps to sample rank k * S / n, whose sampling
+// error has a standard deviation of at most sqrt(S) / 2, so the top-digit bins
+// holding sample ranks within 4 of those deviations of it (a contiguous
+// range) hold the answer with near certainty.  The band is used only when the
+// sample puts at most half of the keys in it, so its keys fit the compaction
+// buffer (cap = n / 2) unless the sample misjudged them; a miss or an
+// overflow only costs the second read the select would do anyway.  (A band
+// straddling the boundary of two large bins, as a presorted input's middle
+// does, holds a third of the keys.)
 template <typename K>
-__global__ __launch_bounds__(256) void k_select_init(SelState* st, const K* __restrict__ keys, uint64_t k, uint64_t n,
-                                                     uint64_t cap, int32_t top_shift, uint32_t* zero, uint32_t nzero,
-                                                     uint32_t* __restrict__ sample) {
-  __shared__ uint32_t s_h[256];
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t i = blockIdx.x * 256 + tid; i < nzero; i += gridDim.x * 256) zero[i] = 0;
-  s_h[tid] = 0;
-  __syncthreads();
-  constexpr uint32_t SPT = kSelSample / kSelSampleBlocks / 256;  // keys per thread
-  constexpr uint32_t TPR = 256 / kSelSampleRuns;                 // threads per run
-  constexpr uint32_t RUNS = kSelSampleBlocks * kSelSampleRuns;
-  const uint64_t run = (uint64_t)blockIdx.x * kSelSampleRuns + tid / TPR, part = tid % TPR;
+__global__ __launch_bounds__(kSelInitBlock) void k_select_init(SelState* st, const K* __restrict__ keys, uint64_t k,
+                                                               uint64_t n, uint64_t cap, int32_t top_shift,
+                                                               uint32_t* zero, uint32_t nzero) {
+  using H = dev::SubHist<32, false>;
+  constexpr uint32_t WAVES = kSelInitBlock / 64;
+  constexpr uint32_t SPT = kSelSample / kSelInitBlock;  // keys per thread
+  static_assert(kSelRuns == WAVES * SPT / 2, "each wave reads SPT / 2 runs of two keys per lane");
+  __shared__ __align__(16) uint32_t s_sub[H::words(256)];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint64_t s_band[4];  // lo bin, hi bin, keys below the lo bin, keys up to the hi bin
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool all = n < kSelSample;
-  const uint64_t base = all ? 0 : run * (n - kSelRun) / (RUNS - 1);
+  const uint64_t S = all ? n : kSelSample;  // keys sampled
+  // Run r starts at floor(r * span / (kSelRuns - 1)) = r * q + r * rem / (kSelRuns - 1)
+  // (one 64-bit division per thread instead of one per run).
+  const uint64_t span = all ? 0 : n - kSelRun;
+  const uint64_t q = span / (kSelRuns - 1);
+  const uint32_t rem = (uint32_t)(span - q * (kSelRuns - 1));
   K v[SPT];
 #pragma unroll
   for (uint32_t j = 0; j < SPT; ++j) {
-    // all: thread g of the grid takes keys g, g + grid threads, ...; else
-    // key j * TPR + part of its run.
-    const uint64_t i = all ? (uint64_t)blockIdx.x * 256 + tid + (uint64_t)j * kSelSampleBlocks * 256
-                           : base + (uint64_t)j * TPR + part;
+    const uint32_t r = wave + WAVES * (j / 2);
+    const uint64_t i = all ? tid + (uint64_t)j * kSelInitBlock
+                           : r * q + (r * rem) / (kSelRuns - 1) + (j % 2) * 64 + lane;
     v[j] = i < n ? keys[i] : K(0);
   }
-#pragma unroll
-  for (uint32_t j = 0; j < SPT; ++j) {
-    const uint64_t i = all ? (uint64_t)blockIdx.x * 256 + tid + (uint64_t)j * kSelSampleBlocks * 256 : 0;
-    if (i < n) atomicAdd(&s_h[dev::digit8<K>(v[j], (uint32_t)top_shift)], 1u);
-  }
+  uint4* const z4 = reinterpret_cast<uint4*>(zero);  // zero is 16-byte aligned
+  for (uint32_t i = tid; i < nzero / 4; i += kSelInitBlock) z4[i] = make_uint4(0, 0, 0, 0);
+  for (uint32_t i = nzero / 4 * 4 + tid; i < nzero; i += kSelInitBlock) zero[i] = 0;
+  for (uint32_t i = tid; i < H::words(256) / 4; i += kSelInitBlock)
+    reinterpret_cast<uint4*>(s_sub)[i] = make_uint4(0, 0, 0, 0);
+  if (tid < 4) s_band[tid] = tid == 1 ? 255u : (tid == 3 ? S : 0u);
   __syncthreads();
-  sample[blockIdx.x * 256 + tid] = s_h[tid];
-  if (blockIdx.x == 0 && tid == 0) {
-    SelState t{};
-    t.k = k;
-    t.cand = n;
-    t.cap = cap;
-    t.shift = top_shift;
-    *st = t;
-  }
-}
-
-// The band from the sample (every block of the first read computes the same
-// one): the answer's rank k maps to sample rank k * S / n, whose sampling
-// error has a standard deviation of at most sqrt(S) / 2, so the bins holding
-// sample ranks within 4 of those deviations of it (a contiguous range) hold
-// the answer with near certainty.  The band is used only when the sample puts
-// at most half of the keys in it, so its keys fit the compaction buffer
-// (cap = n / 2) unless the sample misjudged them; a miss or an overflow only
-// costs the second read the select would do anyway.  (A band straddling the
-// boundary of two large bins, as a presorted input's middle does, holds a
-// third of the keys.)  Threads 0..255 take part; s_band
-// receives (lo, hi, on).
-__device__ void sample_band(const uint32_t* __restrict__ sample, uint64_t k, uint64_t n, uint32_t* s_h,
-                            uint32_t* s_wsum, uint32_t* s_band) {
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint64_t S = n < kSelSample ? n : kSelSample;
+#pragma unroll
+  for (uint32_t j = 0; j < SPT; ++j)
+    if (!all || tid + (uint64_t)j * kSelInitBlock < n) H::add(s_sub, dev::digit8<K>(v[j], (uint32_t)top_shift), lane, 1u);
+  __syncthreads();
   uint32_t c = 0, inc = 0;
   if (tid < 256) {
-    uint32_t p[kSelSampleBlocks];
-#pragma unroll
-    for (uint32_t b = 0; b < kSelSampleBlocks; ++b) p[b] = sample[b * 256 + tid];
-#pragma unroll
-    for (uint32_t b = 0; b < kSelSampleBlocks; ++b) c += p[b];
-    s_h[tid] = c;
+    c = H::sum(s_sub, tid);
     inc = dev::wave_incl_scan(c, lane);
     if (lane == 63) s_wsum[wave] = inc;
-    if (tid < 2) s_band[tid] = tid ? 255u : 0u;
   }
   __syncthreads();
   const uint64_t ks = k * S / n;
-  uint32_t delta = 1;
-  while ((uint64_t)delta * delta < S) ++delta;  // ceil(sqrt(S))
-  delta = 2 * delta + 1;                        // 4 deviations of at most sqrt(S) / 2
+  uint32_t delta = (uint32_t)sqrtf((float)S);  // then corrected to ceil(sqrt(S))
+  while ((uint64_t)delta * delta < S) ++delta;
+  while (delta > 1 && (uint64_t)(delta - 1) * (delta - 1) >= S) --delta;
+  delta = 2 * delta + 1;  // 4 deviations of at most sqrt(S) / 2
   const uint64_t lo_r = ks > delta ? ks - delta : 0, hi_r = ks + delta < S ? ks + delta : S - 1;
   if (tid < 256) {
     uint32_t pre = 0;
     for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
     const uint64_t below = pre + inc - c;
-    if (c > 0 && lo_r >= below && lo_r < below + c) s_band[0] = tid;
-    if (c > 0 && hi_r >= below && hi_r < below + c) s_band[1] = tid;
+    if (c > 0 && lo_r >= below && lo_r < below + c) {
+      s_band[0] = tid;
+      s_band[2] = below;
+    }
+    if (c > 0 && hi_r >= below && hi_r < below + c) {
+      s_band[1] = tid;
+      s_band[3] = below + c;
+    }
   }
   __syncthreads();
   if (tid == 0) {
-    uint64_t in_band = 0;
-    for (uint32_t b = s_band[0]; b <= s_band[1]; ++b) in_band += s_h[b];
-    s_band[2] = S == kSelSample && in_band * 2 <= S;
+    SelState t{};
+    t.k = k;
+    t.cand = n;
+    t.cap = cap;
+    t.shift = top_shift;
+    t.band_lo = (uint32_t)s_band[0];
+    t.band_hi = (uint32_t)s_band[1];
+    t.band_on = !all && (s_band[3] - s_band[2]) * 2 <= S;
+    *st = t;
   }
-  __syncthreads();
 }
 
 // Finds the bin b of `row` (256 counts, in s_row) that holds rank k; returns it
@@ -188,8 +184,8 @@ __device__ void pick_bin(const uint32_t* s_row, uint64_t k, uint32_t* s_wsum, ui
 }
 
 // Global histograms are kept as kSelCopies copies of the 256 bins: block b
-// adds its counts into copy b % kSelCopies, so the adds of the 512 blocks of a
-// launch meet 32 to an address rather than 512 (adds to one address are
+// adds its counts into copy b % kSelCopies, so the adds of a launch's 768
+// blocks meet 48 to an address rather than 768 (adds to one address are
 // applied one after another), and a reader sums the copies.
 constexpr uint32_t kSelCopies = 16;
 constexpr uint32_t kSelHistWords = 256 * kSelCopies;
@@ -300,29 +296,26 @@ struct WaveCompactor {
 // its share whose top digit lies in the band into its own region of buf
 // (cap / grid keys at buf + b * (cap / grid); band_counts[b] keys, more than
 // the region when it overflowed), so no counter is shared between blocks.
-// Four 16-byte loads per thread per step, issued a step ahead.
+// Two 16-byte loads per thread per step, issued a step ahead.
 template <typename K>
 __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict__ keys, size_t n,
                                                             uint32_t* __restrict__ ghist, K* __restrict__ mm,
                                                             SelState* __restrict__ S, K* __restrict__ buf,
-                                                            uint32_t* __restrict__ band_counts,
-                                                            const uint32_t* __restrict__ sample) {
+                                                            uint32_t* __restrict__ band_counts) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
-  constexpr int U = 4;
+  // Two loads per step keep the LDS at 48 KB (three blocks per CU).
+  constexpr int U = 2;
   constexpr int WAVES = kSelBlock / 64;
   using C = WaveCompactor<K, U>;
   using H = dev::SubHist<32, false>;
-  __shared__ uint32_t s_hist[256];
   __shared__ __align__(16) uint32_t s_sub[H::words(256)];
   __shared__ K s_mm[WAVES][2];
   __shared__ K s_stage[WAVES][C::STAGE];
   __shared__ uint32_t s_wcnt[WAVES];
   __shared__ unsigned long long s_bbase;
-  __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_band[3];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint64_t k = S->k, cap = S->cap;
+  const uint64_t cap = S->cap;
   size_t head = ((16 - ((uintptr_t)keys & 15)) & 15) / sizeof(K);
   if (head > n) head = n;
   const size_t nvec = (n - head) / PER;
@@ -332,20 +325,13 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   // first index), so the wave-level compaction stays convergent.
   const size_t wfirst = (size_t)blockIdx.x * kSelBlock + wave * 64;
   // The loop is software-pipelined: a step's loads are issued before the
-  // previous step's keys are counted and compacted, and the first step's
-  // before the band is known.
+  // previous step's keys are counted and compacted.
   uint4 nx[U];
 #pragma unroll
   for (int u = 0; u < U; ++u)
     if (wfirst + lane + u * stride < nvec) nx[u] = vk[wfirst + lane + u * stride];
-  sample_band(sample, k, n, s_hist, s_wsum, s_band);
-  const uint32_t band = s_band[2];
-  const uint32_t band_lo = s_band[0], band_w = s_band[1] - s_band[0];
-  if (blockIdx.x == 0 && tid == 0) {
-    S->band_lo = s_band[0];
-    S->band_hi = s_band[1];
-    S->band_on = band;
-  }
+  const uint32_t band = S->band_on;
+  const uint32_t band_lo = S->band_lo, band_w = S->band_hi - band_lo;
   __shared__ unsigned long long s_ctr;
   if (tid == 0) s_ctr = 0;
   const uint64_t region = cap / gridDim.x;
@@ -707,17 +693,20 @@ __global__ __launch_bounds__(256) void k_select_final(const SelState* __restrict
 // digit's histogram from the first read, the per-block minima and maxima,
 // one histogram per pass, each histogram in kSelCopies copies, and one
 // 64-bit compaction counter per pass), then the compaction buffer.
-constexpr unsigned kSelMaxGrid = 512;  // blocks of the first read and the passes
+// Blocks of the first read and the passes: three per CU (the first read's
+// LDS allows three), which measured 79 us for the first read of 100M keys
+// against 84 us with two.
+constexpr unsigned kSelMaxGrid = 768;
 constexpr size_t kSelMinMaxWords = 2 * kSelMaxGrid * 2;  // two 64-bit words per block
 template <typename K>
 static constexpr size_t select_zero_words() {
   return kSelHistWords + kSelMinMaxWords + (sizeof(K) - 1) * kSelHistWords + 2 * (sizeof(K) - 1);
 }
-// The sample's per-block counts and the band's per-block key counts follow
-// the cleared words (written in full before they are read, never cleared).
+// The band's per-block key counts follow the cleared words (written in full
+// before they are read, never cleared).
 template <typename K>
 static constexpr size_t select_control_words() {
-  return select_zero_words<K>() + kSelSampleBlocks * 256 + kSelMaxGrid;
+  return select_zero_words<K>() + kSelMaxGrid;
 }
 
 template <typename K>
@@ -741,16 +730,15 @@ static void launch_select(const K* din, size_t n, size_t k, SelState* S, uint32_
   uint32_t* ghist_first = z;  // followed by the minimum and maximum words
   uint32_t* ghist = z + kSelHistWords + kSelMinMaxWords;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ghist + (D - 1) * kSelHistWords);
-  uint32_t* sample = reinterpret_cast<uint32_t*>(ctr + (D - 1));
-  uint32_t* band_counts = sample + kSelSampleBlocks * 256;
-  hipLaunchKernelGGL(k_select_init<K>, dim3(kSelSampleBlocks), dim3(256), 0, s, S, din, (uint64_t)k, (uint64_t)n,
-                     (uint64_t)cap, (int32_t)(8 * (D - 1)), z, nz, sample);
+  uint32_t* band_counts = reinterpret_cast<uint32_t*>(ctr + (D - 1));
+  hipLaunchKernelGGL(k_select_init<K>, dim3(1), dim3(kSelInitBlock), 0, s, S, din, (uint64_t)k, (uint64_t)n,
+                     (uint64_t)cap, (int32_t)(8 * (D - 1)), z, nz);
   const size_t nvec = n / (16 / sizeof(K));
   const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + 4 * kSelBlock - 1) / (4 * kSelBlock), kSelMaxGrid));
   {
     ProfScope ps("k_select_first", s);
     hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghist_first,
-                       reinterpret_cast<K*>(ghist_first + kSelHistWords), S, buf, band_counts, sample);
+                       reinterpret_cast<K*>(ghist_first + kSelHistWords), S, buf, band_counts);
   }
   for (int step = 0; step < D - 1; ++step) {
     ProfScope ps("k_select_pass", s);
