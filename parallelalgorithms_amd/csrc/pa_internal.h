@@ -59,7 +59,7 @@ struct StreamCtx {
   // stream suffices.
   uint32_t* ctl = nullptr;
 };
-constexpr size_t kStreamCtlWords = 1024;
+constexpr size_t kStreamCtlWords = 8192;
 // The stream's control words (allocated and zeroed on first use).
 int stream_ctl(StreamCtx& ctx, uint32_t** out);
 
