@@ -63,19 +63,23 @@ constexpr unsigned kPrefixThreads = 1024;
 // Step 1: partial[g][d] = count of digit d over group g's tiles (blocks
 // g < groups).  The passes' blocks after them write the digit bases of pass
 // p = blockIdx.x - groups: base[p][d] = sum_{d' < d} counts[p][d'], the first
-// output slot of digit d.
+// output slot of digit d; and trivial[p] = 1 when one digit holds all n keys
+// (the pass would leave the order as it is).
 template <int RB>
 __global__ __launch_bounds__(kPrefixThreads) void k_tile_sums(const uint32_t* __restrict__ rows, uint32_t ntiles,
                                                               uint32_t per, uint32_t* __restrict__ partial,
                                                               uint32_t groups, const uint32_t* __restrict__ counts,
-                                                              uint32_t* __restrict__ base) {
+                                                              uint32_t* __restrict__ base, uint32_t n,
+                                                              uint32_t* __restrict__ trivial) {
   constexpr uint32_t R = 1u << RB, Q = kPrefixThreads / R;
   __shared__ uint32_t s_q[Q][R];
+  __shared__ uint32_t s_full;
   const uint32_t d = threadIdx.x % R, q = threadIdx.x / R;
   if (blockIdx.x >= groups) {
     // Threads 0..R-1 scan the pass's R counts.
     const uint32_t p = blockIdx.x - groups, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t c = 0, inc = 0;
+    if (threadIdx.x == 0) s_full = 0;
     if (q == 0) {
       c = counts[p * R + d];
       inc = dev::wave_incl_scan(c, lane);
@@ -86,7 +90,10 @@ __global__ __launch_bounds__(kPrefixThreads) void k_tile_sums(const uint32_t* __
       uint32_t pre = 0;
       for (uint32_t w = 0; w < wave; ++w) pre += s_q[0][w];
       base[p * R + d] = pre + inc - c;
+      if (c == n) s_full = 1;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) trivial[p] = s_full;
     return;
   }
   const uint32_t t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
@@ -103,13 +110,46 @@ __global__ __launch_bounds__(kPrefixThreads) void k_tile_sums(const uint32_t* __
 }
 
 // Step 2: group g adds the partials of groups < g, then scans its own tiles
-// in chunks of Q x 8: thread (q, d) loads 8 consecutive rows at once.
+// in chunks of Q x 8: thread (q, d) loads 8 consecutive rows at once.  When
+// roles is set, block 0 also writes the passes' roles from the trivial flags
+// into the top bits of their tile counters (kRoleSkip, kRoleSwap, see
+// pa_sort_kernels.h; pass 0 has kXcds counters, pass p > 0 counter
+// kXcds - 1 + p).  Trivial passes are skipped, except that when the number
+// of the others and the pass count D differ in parity the first trivial pass
+// still runs, so the data ends in the buffer the host expects (the keys for
+// an even D, the temporary array for an odd one, which the host copies
+// back).
 template <int RB>
 __global__ __launch_bounds__(kPrefixThreads) void k_tile_scan(uint32_t* __restrict__ rows, uint32_t ntiles,
-                                                              uint32_t per, const uint32_t* __restrict__ partial) {
+                                                              uint32_t per, const uint32_t* __restrict__ partial,
+                                                              const uint32_t* __restrict__ trivial, uint32_t passes,
+                                                              bool roles, uint32_t* __restrict__ counters) {
   constexpr uint32_t R = 1u << RB, Q = kPrefixThreads / R;
   __shared__ uint32_t s_q[Q][R];
   const uint32_t d = threadIdx.x % R, q = threadIdx.x / R;
+  if (roles && blockIdx.x == 0 && threadIdx.x == 0) {
+    uint32_t tcount = 0, keep = passes;  // keep: a trivial pass that runs anyway
+    for (uint32_t p = 0; p < passes; ++p)
+      if (!trivial[p])
+        ++tcount;
+      else if (keep == passes)
+        keep = p;
+    if ((tcount & 1u) == (passes & 1u)) keep = passes;  // parities agree: every trivial pass is skipped
+    uint32_t cur = 0;                                   // buffer holding the data: 0 keys, 1 temporary
+    for (uint32_t p = 0; p < passes; ++p) {
+      uint32_t r;
+      if (trivial[p] && p != keep) {
+        r = kRoleSkip;
+      } else {
+        r = (p & 1u) != cur ? kRoleSwap : 0u;
+        cur ^= 1u;
+      }
+      if (p == 0)
+        for (uint32_t x = 0; x < kXcds; ++x) counters[x] = r;
+      else
+        counters[kXcds - 1 + p] = r;
+    }
+  }
   uint32_t acc = 0;
 #pragma unroll 8
   for (uint32_t g = q; g < blockIdx.x; g += Q) acc += partial[g * R + d];
@@ -320,14 +360,16 @@ size_t sort_scratch_bytes(const PassVariant& var, size_t n) {
   L.take(sort_tile_counters((int)D) * 4);  // tile counters
   L.take(D * rows * R * 4);               // look-back table
   L.take((size_t)kPrefixGroups * R * 4);  // pass-0 prefix partial sums
+  L.take(D * 4);                          // trivial flags of the passes
   return L.off;
 }
 
 // The sort with RB-bit digits: D = ceil(key bits / RB) passes.  Scratch
 // layout: counts [D][R], digit bases [D][R], tile counters [kXcds + D - 1]
 // (pass 0 claims per XCD, the later passes from one counter each), look-back
-// rows [D][tiles][R], prefix partials [groups][R].  With an odd D the last
-// pass writes into the temporary arrays and the result is copied back.
+// rows [D][tiles][R], prefix partials [groups][R], the passes' trivial flags
+// [D].  With an odd D the data ends in the temporary arrays and
+// is copied back (the roles keep that parity when passes are skipped).
 template <typename K, bool KV, int RB>
 int enqueue_sort_rb(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, char* scratch,
                     hipStream_t s, uint32_t xform) {
@@ -342,6 +384,7 @@ int enqueue_sort_rb(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, ui
   const size_t lb_bytes = (size_t)D * rows * R * 4;
   uint32_t* lb = reinterpret_cast<uint32_t*>(scratch + L.take(lb_bytes));
   uint32_t* prefix_partial = reinterpret_cast<uint32_t*>(scratch + L.take((size_t)kPrefixGroups * R * 4));
+  uint32_t* trivial = reinterpret_cast<uint32_t*>(scratch + L.take(D * 4));
 
   {
     ProfScope ps("k_sort_prepare", s);
@@ -374,9 +417,9 @@ int enqueue_sort_rb(const PassVariant& var, K* keys, uint32_t* vals, K* ktmp, ui
   {
     ProfScope ps("k_tile_prefix", s);
     hipLaunchKernelGGL(k_tile_sums<RB>, dim3(groups + D), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per,
-                       prefix_partial, groups, counts, bases);
+                       prefix_partial, groups, counts, bases, (uint32_t)n, trivial);
     hipLaunchKernelGGL(k_tile_scan<RB>, dim3(groups), dim3(kPrefixThreads), 0, s, lb, (uint32_t)rows, per,
-                       prefix_partial);
+                       prefix_partial, trivial, (uint32_t)D, xform == 0, ctr);
   }
   K* kin = keys;
   K* kout = ktmp;

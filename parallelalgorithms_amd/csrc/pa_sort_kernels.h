@@ -35,6 +35,18 @@ constexpr uint32_t sort_tile_counters(int passes) { return kXcds + (uint32_t)pas
 // sort, from the histogram read), so tiles neither publish nor wait.
 enum : uint32_t { kFloatIn = 1u, kFloatOut = 2u, kPrefixReady = 4u, kSignedKeys = 8u };
 
+// Pass roles of a plain sort of integer keys, written after the histogram
+// read (k_tile_scan) into the top bits of each pass's tile counters, so every
+// claim returns them with its tile index at no extra cost: a pass whose digit
+// is the same for every key leaves the order as it is and is skipped
+// (kRoleSkip); the others permute, reading the buffer the previous permuting
+// pass wrote (kRoleSwap: the other one than the host's alternation assumes).
+// Sorts with a key transform (float, signed) run without roles: the transform
+// is applied in pass 0 and undone in the last pass, and making those two
+// passes a run-time choice cost the kernel 8 more spilled registers.  Tile
+// indices stay below 2^30.
+enum : uint32_t { kRoleSkip = 1u << 30, kRoleSwap = 1u << 31, kRoleBits = 3u << 30 };
+
 // Tile-to-array modes of the onesweep pass.
 enum : int {
   kModePlain = 0,  // one array, tiles in order, one look-back chain
@@ -239,22 +251,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
   if (tid == 0) {
     if (fmode & kPrefixReady) {
       const uint32_t x = __builtin_amdgcn_s_getreg(kHwRegXccId) & (kXcds - 1);
-      uint32_t t = ntiles;
+      uint32_t t = ntiles, bits = 0;
       for (uint32_t k = 0; k < kXcds && t == ntiles; ++k) {
         const uint32_t y = (x + k) & (kXcds - 1);
-        const uint32_t j = atomicAdd(tile_counter + y, 1u);
+        const uint32_t jb = atomicAdd(tile_counter + y, 1u);
+        const uint32_t j = jb & ~kRoleBits;
+        bits = jb & kRoleBits;
         const uint64_t c = ((uint64_t)(j / kXcdGroup) * kXcds + y) * kXcdGroup + j % kXcdGroup;
         if (c < ntiles) t = (uint32_t)c;
       }
-      s_claim = t;
+      s_claim = t | bits;
     } else {
       s_claim = atomicAdd(tile_counter, 1u);
     }
   }
   __syncthreads();
   // Uniform across the block: the tile geometry below lives in scalar registers.
-  const uint32_t claimed = __builtin_amdgcn_readfirstlane(s_claim);
-  if (claimed >= ntiles) return;
+  const uint32_t cv = __builtin_amdgcn_readfirstlane(s_claim);
+  const uint32_t rl = cv & kRoleBits;
+  const uint32_t claimed = cv & ~kRoleBits;
+  if (claimed >= ntiles || (rl & kRoleSkip)) return;
 
   // Geometry of the claimed tile.
   const K* in;
@@ -276,10 +292,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) v
     g = gtab + (size_t)seg * gstride * R;
   } else {
     local = claimed;
-    in = kin;
-    out = kout;
-    vsrc = vin;
-    vdst = vout;
+    const bool swap = (rl & kRoleSwap) != 0;
+    in = swap ? kout : kin;
+    out = swap ? const_cast<K*>(kin) : kout;
+    vsrc = swap ? vout : vin;
+    vdst = swap ? const_cast<uint32_t*>(vin) : vout;
     len = n;
     g = gtab;
   }
