@@ -206,7 +206,7 @@ def test_hist_one_byte_bincount(orc, shift, radix):
 
 
 def test_hist_top_byte_invariants_100M(orc):
-    # SURVEY invariants for the 100M top-byte config: presorted 0..n-1 puts
+    # Closed forms for the 100M top-byte histogram: presorted 0..n-1 puts
     # 2^24 keys in bins 0..4 and 16,113,920 in bin 5; constant 0x5A5A5A5A puts
     # all n in bin 0x5A.
     n = 100_000_000
