@@ -264,10 +264,11 @@ struct Variants<uint32_t, false> {
       PA_VARIANT_RB(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16, 6),
       PA_VARIANT_RB(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16, 5),
       PA_VARIANT_RB(uint32_t, false, 768, 32, 8, true, 6, true, 2, 16, 4)};
-  // 512 x 16 up to 2^24 keys (4M: 0.209 -> 0.180 ms against 512 x 48; even at 16M); above, the 24,576-key
-  // split tile as 768 x 32 (12 waves per workgroup, two workgroups per CU: 100M passes 188-227 us against
-  // 194-233 us for 512 x 48).
-  static constexpr int by_class[kClasses] = {2, 2, 6, 8};
+  // 512 x 16 up to 2^20 keys; 512 x 24 with 16-lane units up to 2^24 (a sort back to back: 1.2M 74 us,
+  // 2M 88, 4M 112.5, 8M 157, 16M 231.5 against 75.5, 87, 120, 174, 255 for 512 x 16; 768 x 32 is faster
+  // from 8M on but slower below 2M: 87 us at 1.2M); above, the 24,576-key split tile as 768 x 32 (12
+  // waves per workgroup, two workgroups per CU: 100M passes 188-227 us against 194-233 us for 512 x 48).
+  static constexpr int by_class[kClasses] = {2, 2, 7, 8};
 };
 template <>
 struct Variants<uint32_t, true> {

@@ -53,7 +53,7 @@ def test_size_class_choice_needs_no_device(monkeypatch):
     lib = P.load()
     assert lib.pa_autotune_reset() == 0
     assert [lib.pa_sort_variant_u32(n) for n in (1, 1 << 16, (1 << 16) + 1, 1 << 20)] == [2, 2, 2, 2]
-    assert [lib.pa_sort_variant_u32(n) for n in ((1 << 20) + 1, 1 << 24, (1 << 24) + 1, 100_000_000)] == [6, 6, 8, 8]
+    assert [lib.pa_sort_variant_u32(n) for n in ((1 << 20) + 1, 1 << 24, (1 << 24) + 1, 100_000_000)] == [7, 7, 8, 8]
     assert lib.pa_autotune_sort_u32(1, None) == 1  # needs max_n >= 2; rejected before any HIP call
     assert lib.pa_segmented_plan_create_u32(None, None, 0) is None
 
