@@ -9,13 +9,15 @@
 // same scheme over digits 56 .. 0 (an extension; the reference is 32-bit).
 //
 // GPU form (the caller's array is never modified):
-//   k_select_init   state, cleared control words, and a band of top-digit
-//                   bins predicted from an evenly strided sample of the keys
-//                   to hold the answer.
-//   k_select_first  one read of all keys builds the top digit's histogram
-//                   and the keys' minimum and maximum, and compacts the keys
-//                   of the band; when the answer's top digit is in the band,
-//                   the later passes read only those keys.
+//   k_select_init   (16 blocks) state, cleared control words, and the top
+//                   digits of an evenly strided sample of the keys, counted
+//                   per block.
+//   k_select_first  every block sums the sample's counts into a band of
+//                   top-digit bins predicted to hold the answer, then one
+//                   read of all keys builds the top digit's histogram and the
+//                   keys' minimum and maximum, and compacts the keys of the
+//                   band; when the answer's top digit is in the band, the
+//                   later passes read only those keys.
 //   k_select_pass   (launch i of D - 1; returns at once when the answer is
 //                   known) first derives its state from the counts the
 //                   previous launch left: launch 0 fixes the high bytes that
@@ -61,69 +63,92 @@ struct SelState {
 
 // Keys sampled to predict the answer's top-digit bins: kSelRuns runs of
 // kSelRun consecutive keys at an even stride (single keys at that stride would
-// each cost a page walk), or every key when n < kSelSample.
+// each cost a page walk), or every key when n < kSelSample.  The sample is
+// read by kSelInitBlocks blocks on as many CUs (one block alone took 8.5 us,
+// its loads and page walks queued on one CU), each writing the 256 top-digit
+// counts of its share; the first read sums them and picks the band.
 constexpr uint32_t kSelSample = 32768;
 constexpr uint32_t kSelInitBlock = 1024;
-constexpr uint32_t kSelRun = 128;  // two keys per lane of a wave
+constexpr uint32_t kSelInitBlocks = 16;
+constexpr uint32_t kSelRun = 128;  // two halves of 64 keys, one a wave load
 constexpr uint32_t kSelRuns = kSelSample / kSelRun;
+constexpr uint32_t kSelPartWords = kSelInitBlocks * 256;
 
-// Initial state, cleared control words and the band, in one block.  The block
-// reads the sample (all of a thread's loads issued before any is counted;
-// wave w reads runs w, w + 16, ..., each run by its 64 lanes), counts the
-// sample's top digits in lane-private sub-histograms (a repeated key costs no
-// more than distinct ones) and picks the band.
-//
-// The band: the answer's rank k maps to sample rank k * S / n, whose sampling
-// error has a standard deviation of at most sqrt(S) / 2, so the top-digit bins
-// holding sample ranks within 4 of those deviations of it (a contiguous
-// range) hold the answer with near certainty.  The band is used only when the
-// sample puts at most half of the keys in it, so its keys fit the compaction
-// buffer (cap = n / 2) unless the sample misjudged them; a miss or an
-// overflow only costs the second read the select would do anyway.  (A band
-// straddling the boundary of two large bins, as a presorted input's middle
-// does, holds a third of the keys.)
+// Initial state, cleared control words and the sample's per-block top-digit
+// counts.  Sample element e (0 <= e < kSelSample) is key e when every key is
+// sampled, else key e % kSelRun of run e / kSelRun; thread g of the grid takes
+// elements g and g + kSelInitBlocks * kSelInitBlock, so the 64 lanes of a wave
+// read 64 consecutive keys.  Every load is issued before any is counted; the
+// counts go to lane-private sub-histograms (a repeated key costs no more than
+// distinct ones), whose sums block b writes to parts[b][0..255].
 template <typename K>
 __global__ __launch_bounds__(kSelInitBlock) void k_select_init(SelState* st, const K* __restrict__ keys, uint64_t k,
                                                                uint64_t n, uint64_t cap, int32_t top_shift,
-                                                               uint32_t* zero, uint32_t nzero) {
+                                                               uint32_t* zero, uint32_t nzero,
+                                                               uint32_t* __restrict__ parts) {
   using H = dev::SubHist<32, false>;
-  constexpr uint32_t WAVES = kSelInitBlock / 64;
-  constexpr uint32_t SPT = kSelSample / kSelInitBlock;  // keys per thread
-  static_assert(kSelRuns == WAVES * SPT / 2, "each wave reads SPT / 2 runs of two keys per lane");
+  constexpr uint32_t T = kSelInitBlocks * kSelInitBlock;
+  constexpr uint32_t SPT = kSelSample / T;  // sample elements per thread
+  static_assert(SPT * T == kSelSample && kSelRun % 64 == 0, "whole runs, one wave load per 64 keys");
   __shared__ __align__(16) uint32_t s_sub[H::words(256)];
-  __shared__ uint32_t s_wsum[4];
-  __shared__ uint64_t s_band[4];  // lo bin, hi bin, keys below the lo bin, keys up to the hi bin
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  const uint32_t g = blockIdx.x * kSelInitBlock + tid;
   const bool all = n < kSelSample;
-  const uint64_t S = all ? n : kSelSample;  // keys sampled
   // Run r starts at floor(r * span / (kSelRuns - 1)) = r * q + r * rem / (kSelRuns - 1)
   // (one 64-bit division per thread instead of one per run).
   const uint64_t span = all ? 0 : n - kSelRun;
   const uint64_t q = span / (kSelRuns - 1);
   const uint32_t rem = (uint32_t)(span - q * (kSelRuns - 1));
   K v[SPT];
+  bool ok[SPT];
 #pragma unroll
   for (uint32_t j = 0; j < SPT; ++j) {
-    const uint32_t r = wave + WAVES * (j / 2);
-    const uint64_t i = all ? tid + (uint64_t)j * kSelInitBlock
-                           : r * q + (r * rem) / (kSelRuns - 1) + (j % 2) * 64 + lane;
-    v[j] = i < n ? keys[i] : K(0);
+    const uint32_t e = g + j * T, r = e / kSelRun;
+    const uint64_t i = all ? e : r * q + (uint64_t)(r * rem) / (kSelRuns - 1) + e % kSelRun;
+    ok[j] = i < n;
+    v[j] = ok[j] ? keys[i] : K(0);
   }
   uint4* const z4 = reinterpret_cast<uint4*>(zero);  // zero is 16-byte aligned
-  for (uint32_t i = tid; i < nzero / 4; i += kSelInitBlock) z4[i] = make_uint4(0, 0, 0, 0);
-  for (uint32_t i = nzero / 4 * 4 + tid; i < nzero; i += kSelInitBlock) zero[i] = 0;
+  for (uint32_t i = g; i < nzero / 4; i += T) z4[i] = make_uint4(0, 0, 0, 0);
+  if (blockIdx.x == 0)
+    for (uint32_t i = nzero / 4 * 4 + tid; i < nzero; i += kSelInitBlock) zero[i] = 0;
   for (uint32_t i = tid; i < H::words(256) / 4; i += kSelInitBlock)
     reinterpret_cast<uint4*>(s_sub)[i] = make_uint4(0, 0, 0, 0);
-  if (tid < 4) s_band[tid] = tid == 1 ? 255u : (tid == 3 ? S : 0u);
   __syncthreads();
 #pragma unroll
   for (uint32_t j = 0; j < SPT; ++j)
-    if (!all || tid + (uint64_t)j * kSelInitBlock < n) H::add(s_sub, dev::digit8<K>(v[j], (uint32_t)top_shift), lane, 1u);
+    if (ok[j]) H::add(s_sub, dev::digit8<K>(v[j], (uint32_t)top_shift), lane, 1u);
   __syncthreads();
-  uint32_t c = 0, inc = 0;
+  if (tid < 256) parts[blockIdx.x * 256 + tid] = H::sum(s_sub, tid);
+  if (g == 0) {
+    SelState t{};
+    t.k = k;
+    t.cand = n;
+    t.cap = cap;
+    t.shift = top_shift;
+    *st = t;  // band_* are written by the first read's block 0
+  }
+}
+
+// The band, computed by every block of the first read from the init's
+// per-block counts pc (thread tid < 256 holds bin tid's sum): the answer's
+// rank k maps to sample rank k * S / n, whose sampling error has a standard
+// deviation of at most sqrt(S) / 2, so the top-digit bins holding sample ranks
+// within 4 of those deviations of it (a contiguous range) hold the answer with
+// near certainty.  The band is used only when the sample puts at most half of
+// the keys in it, so its keys fit the compaction buffer (cap = n / 2) unless
+// the sample misjudged them; a miss or an overflow only costs the second read
+// the select would do anyway.  (A band straddling the boundary of two large
+// bins, as a presorted input's middle does, holds a third of the keys.)
+// Returns (lo bin, hi bin, on) in out[0..2]; every thread of the block calls it.
+__device__ void sample_band(uint32_t pc, uint64_t k, uint64_t n, uint32_t* s_wsum, uint64_t* s_band, uint32_t* out) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool all = n < kSelSample;
+  const uint64_t S = all ? n : kSelSample;  // keys sampled
+  uint32_t inc = 0;
+  if (tid < 4) s_band[tid] = tid == 1 ? 255u : (tid == 3 ? S : 0u);
   if (tid < 256) {
-    c = H::sum(s_sub, tid);
-    inc = dev::wave_incl_scan(c, lane);
+    inc = dev::wave_incl_scan(pc, lane);
     if (lane == 63) s_wsum[wave] = inc;
   }
   __syncthreads();
@@ -136,28 +161,20 @@ __global__ __launch_bounds__(kSelInitBlock) void k_select_init(SelState* st, con
   if (tid < 256) {
     uint32_t pre = 0;
     for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
-    const uint64_t below = pre + inc - c;
-    if (c > 0 && lo_r >= below && lo_r < below + c) {
+    const uint64_t below = pre + inc - pc;
+    if (pc > 0 && lo_r >= below && lo_r < below + pc) {
       s_band[0] = tid;
       s_band[2] = below;
     }
-    if (c > 0 && hi_r >= below && hi_r < below + c) {
+    if (pc > 0 && hi_r >= below && hi_r < below + pc) {
       s_band[1] = tid;
-      s_band[3] = below + c;
+      s_band[3] = below + pc;
     }
   }
   __syncthreads();
-  if (tid == 0) {
-    SelState t{};
-    t.k = k;
-    t.cand = n;
-    t.cap = cap;
-    t.shift = top_shift;
-    t.band_lo = (uint32_t)s_band[0];
-    t.band_hi = (uint32_t)s_band[1];
-    t.band_on = !all && (s_band[3] - s_band[2]) * 2 <= S;
-    *st = t;
-  }
+  out[0] = (uint32_t)s_band[0];
+  out[1] = (uint32_t)s_band[1];
+  out[2] = !all && (s_band[3] - s_band[2]) * 2 <= S;
 }
 
 // Finds the bin b of `row` (256 counts, in s_row) that holds rank k; returns it
@@ -301,7 +318,8 @@ template <typename K>
 __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict__ keys, size_t n,
                                                             uint32_t* __restrict__ ghist, K* __restrict__ mm,
                                                             SelState* __restrict__ S, K* __restrict__ buf,
-                                                            uint32_t* __restrict__ band_counts) {
+                                                            uint32_t* __restrict__ band_counts,
+                                                            const uint32_t* __restrict__ parts) {
   constexpr int D = sizeof(K);
   constexpr int PER = 16 / sizeof(K);
   // Two loads per step keep the LDS at 48 KB (three blocks per CU).
@@ -326,12 +344,33 @@ __global__ __launch_bounds__(kSelBlock) void k_select_first(const K* __restrict_
   const size_t wfirst = (size_t)blockIdx.x * kSelBlock + wave * 64;
   // The loop is software-pipelined: a step's loads are issued before the
   // previous step's keys are counted and compacted.
+  // The sample's per-block counts are loaded before the first step's keys, so
+  // the band is computed while those keys are in flight.
+  uint32_t pv[kSelInitBlocks] = {};
+  if (tid < 256) {
+#pragma unroll
+    for (uint32_t b = 0; b < kSelInitBlocks; ++b) pv[b] = parts[b * 256 + tid];
+  }
   uint4 nx[U];
 #pragma unroll
   for (int u = 0; u < U; ++u)
     if (wfirst + lane + u * stride < nvec) nx[u] = vk[wfirst + lane + u * stride];
-  const uint32_t band = S->band_on;
-  const uint32_t band_lo = S->band_lo, band_w = S->band_hi - band_lo;
+  __shared__ uint32_t s_bwsum[4];
+  __shared__ uint64_t s_band[4];
+  uint32_t bd[3];
+  {
+    uint32_t pc = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < kSelInitBlocks; ++b) pc += pv[b];
+    sample_band(pc, S->k, n, s_bwsum, s_band, bd);
+  }
+  if (blockIdx.x == 0 && tid == 0) {  // for the passes
+    S->band_lo = bd[0];
+    S->band_hi = bd[1];
+    S->band_on = bd[2];
+  }
+  const uint32_t band = bd[2];
+  const uint32_t band_lo = bd[0], band_w = bd[1] - bd[0];
   __shared__ unsigned long long s_ctr;
   if (tid == 0) s_ctr = 0;
   const uint64_t region = cap / gridDim.x;
@@ -702,11 +741,12 @@ template <typename K>
 static constexpr size_t select_zero_words() {
   return kSelHistWords + kSelMinMaxWords + (sizeof(K) - 1) * kSelHistWords + 2 * (sizeof(K) - 1);
 }
-// The band's per-block key counts follow the cleared words (written in full
-// before they are read, never cleared).
+// The band's per-block key counts and the sample's per-block top-digit counts
+// follow the cleared words (written in full before they are read, never
+// cleared).
 template <typename K>
 static constexpr size_t select_control_words() {
-  return select_zero_words<K>() + kSelMaxGrid;
+  return select_zero_words<K>() + kSelMaxGrid + kSelPartWords;
 }
 
 template <typename K>
@@ -731,14 +771,15 @@ static void launch_select(const K* din, size_t n, size_t k, SelState* S, uint32_
   uint32_t* ghist = z + kSelHistWords + kSelMinMaxWords;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ghist + (D - 1) * kSelHistWords);
   uint32_t* band_counts = reinterpret_cast<uint32_t*>(ctr + (D - 1));
-  hipLaunchKernelGGL(k_select_init<K>, dim3(1), dim3(kSelInitBlock), 0, s, S, din, (uint64_t)k, (uint64_t)n,
-                     (uint64_t)cap, (int32_t)(8 * (D - 1)), z, nz);
+  uint32_t* parts = band_counts + kSelMaxGrid;
+  hipLaunchKernelGGL(k_select_init<K>, dim3(kSelInitBlocks), dim3(kSelInitBlock), 0, s, S, din, (uint64_t)k,
+                     (uint64_t)n, (uint64_t)cap, (int32_t)(8 * (D - 1)), z, nz, parts);
   const size_t nvec = n / (16 / sizeof(K));
   const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((nvec + 4 * kSelBlock - 1) / (4 * kSelBlock), kSelMaxGrid));
   {
     ProfScope ps("k_select_first", s);
     hipLaunchKernelGGL(k_select_first<K>, dim3(grid), dim3(kSelBlock), 0, s, din, n, ghist_first,
-                       reinterpret_cast<K*>(ghist_first + kSelHistWords), S, buf, band_counts);
+                       reinterpret_cast<K*>(ghist_first + kSelHistWords), S, buf, band_counts, parts);
   }
   for (int step = 0; step < D - 1; ++step) {
     ProfScope ps("k_select_pass", s);
