@@ -134,6 +134,33 @@ def test_select_u64(pa, dist, n):
     np.testing.assert_array_equal(to_host(d, np.uint64), a)
 
 
+@pytest.mark.parametrize("dist", ["uniform", "presorted", "few", "nearly"])
+@pytest.mark.parametrize("n", [32_767, 32_768, 32_769, 33_000, 65_536 + 129])
+def test_select_sample_boundary(pa, orc, dist, n):
+    """Sizes around the 32,768-key sample: below it every key is sampled and
+    the band is off; from it on the sample is 256 runs of 128 keys read by 16
+    blocks (at n = 32,768 exactly every key once, just above it runs that
+    nearly touch), and the band the first read derives from their counts
+    decides which keys are compacted.  u32 against the oracle, u64 against
+    the order statistic."""
+    a = make_u32(dist, n)
+    d = to_dev(a)
+    ks = sorted({0, 1, n // 100, n // 3, n // 2, n - 21, n - 1})
+    for k in ks:
+        expect, _ = orc.select_radix_u32(a, k)
+        res = np.zeros(1, dtype=np.uint32)
+        pa.select_u32(d, k, res)
+        assert int(res[0]) == expect, (dist, n, k)
+    np.testing.assert_array_equal(to_host(d, np.uint32), a)
+    a64 = (a.astype(np.uint64) << np.uint64(32)) | a[::-1].astype(np.uint64)
+    srt = np.sort(a64)
+    d64 = to_dev(a64)
+    for k in ks:
+        res = np.zeros(1, dtype=np.uint64)
+        pa.select_u64(d64, k, res)
+        assert int(res[0]) == int(srt[k]), ("u64", dist, n, k)
+
+
 def _sorted_u32(seed, n, mod=None):
     import parallelalgorithms_amd as P
     x = P.gen_u32(seed, n)
